@@ -1,0 +1,138 @@
+/*
+ * lss_oracle.h -- CPU restatement of the reference's host ADI path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This library is the checker for the HIP implementation in linear_system_solvers_amd/.  Only the
+ * test suite (tests/), __graft_entry__.smoke() and bench.py's CPU-baseline leg may load it.  The
+ * product path never links, loads or calls anything in this directory.
+ *
+ * It restates, statement by statement and in the same floating-point operation order, the reference's
+ * Host / ThomasLU and Host / DoubleSweep implicit Heston ADI path:
+ *   pde_solvers/two_dimensional/heat_type/lss_2d_general_heston_equation.hpp:184-307      (solve)
+ *   pde_solvers/two_dimensional/heat_type/lss_2d_general_heston_equation_implicit_kernel.hpp:63-207, 958-1050
+ *   pde_solvers/two_dimensional/heat_type/splitting_method/lss_heat_{douglas_rachford,craig_sneyd,...}_method.hpp           (DR/CS/MCS/HV)
+ *   pde_solvers/two_dimensional/heat_type/implicit_coefficients/lss_2d_general_heston_equation_coefficients.hpp:43-99
+ *   pde_solvers/two_dimensional/heat_type/lss_2d_general_heston_equation_explicit_boundary.hpp:39-238
+ *   pde_solvers/two_dimensional/heat_type/transformation/lss_heston_boundary_transform.hpp:36-66
+ *   pde_solvers/transformation/lss_heat_data_transform.hpp:148-221
+ *   discretization/lss_grid_config.hpp:33-129, lss_grid_transform_config.hpp:103-188
+ *   sparse_solvers/tridiagonal/thomas_lu_solver/ and sparse_solvers/tridiagonal/double_sweep_solver/
+ * Each function below cites the lines it follows.  The same source is compiled for double (suffix _f64)
+ * and float (suffix _f32); compile with -ffp-contract=off so no multiply-add is fused.
+ *
+ * Parity pins (see tests/test_oracle_*.py): the committed 6-significant-digit surfaces that the
+ * reference's own print tests wrote (outputs/ text files, copied under tests/golden/reference_outputs/),
+ * the reference's closed-form u''=-2 line-solver cases, LAPACK banded solves, dense brute-force
+ * restatements of single stages on tiny grids, and invariants (constants, linearity, CS == MCS at
+ * zeta == theta).  "parity unpinned": the MCS (zeta - theta) term's -P sign at zeta != theta has no
+ * reference output that exercises it (the reference's MCS surface uses zeta == theta == 0.5).
+ */
+#ifndef LSS_ORACLE_H
+#define LSS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* enum values follow common/lss_enumerations.hpp:15-129 */
+enum { LSS_OR_DOUGLAS_RACHFORD = 0, LSS_OR_CRAIG_SNEYD = 1, LSS_OR_MOD_CRAIG_SNEYD = 2, LSS_OR_HUNDSDORFER_VERWER = 3 };
+enum { LSS_OR_EULER = 0, LSS_OR_THETA_30 = 1, LSS_OR_CRANK_NICOLSON = 2, LSS_OR_THETA_80 = 3 };
+enum { LSS_OR_CUDA_SOLVER = 0, LSS_OR_DOUBLE_SWEEP = 1, LSS_OR_SOR_SOLVER = 2, LSS_OR_THOMAS_LU = 3 };
+enum { LSS_OR_FORWARD = 0, LSS_OR_BACKWARD = 1 };
+enum { LSS_OR_BC_DIRICHLET = 0, LSS_OR_BC_NEUMANN = 1, LSS_OR_BC_ROBIN = 2 };
+
+/* User data of the PDE u_t = a u_xx + b u_yy + c u_xy + d u_x + e u_y + f u, given (like the
+ * reference's std::function closures) as callbacks.  Arguments arrive as fp values widened to
+ * double; the callback computes in double and the oracle rounds the result to fp, which is what
+ * the reference's test lambdas do (they return double expressions through std::function<fp(...)>). */
+typedef double (*lss_or_coef_fn)(void *ctx, double t, double x, double y);
+typedef double (*lss_or_init_fn)(void *ctx, double x, double y);
+typedef double (*lss_or_bnd_fn)(void *ctx, double t, double s);
+
+typedef struct lss_or_pde {
+    lss_or_coef_fn a, b, c, d, e, f;
+    lss_or_init_fn init;     /* terminal/initial condition G(x, y) */
+    lss_or_bnd_fn h_lower;   /* Dirichlet at x = x_lo, value(t, y)   */
+    lss_or_bnd_fn h_upper;   /* Neumann at x = x_hi, value(t, y)     */
+    lss_or_bnd_fn v_upper;   /* Dirichlet at y = y_hi, value(t, x)   */
+    void *ctx;
+} lss_or_pde;
+
+typedef struct lss_or_cfg {
+    double x_lo, x_hi, y_lo, y_hi, t_lo, t_hi; /* ranges (cast to fp like range<T>(static_cast<T>(..))) */
+    int64_t nx, ny, nt;                        /* number of space points 1 / 2, number of time points */
+    double accumulation_point, alpha_scale, beta_scale;
+    int32_t nonuniform;                        /* grid_enum: 0 Uniform, 1 Nonuniform */
+    int32_t splitting;                         /* splitting_method_enum */
+    double weighting;                          /* splitting_method_config weighting value */
+    int32_t implicit_scheme;                   /* implicit_pde_schemes_enum */
+    int32_t tridiagonal;                       /* tridiagonal_method_enum: ThomasLU or DoubleSweep */
+    int32_t direction;                         /* traverse_direction_enum */
+    /* Evaluation mode.  0 = faithful: every coefficient is evaluated through the transform chain at
+     * every use, as the reference does (this is the timed CPU baseline).  1 = cached: the transformed
+     * coefficient fields are evaluated once per loop time (once per solve if coeff_time_independent)
+     * and reused; the values are the same bits, only evaluated fewer times. */
+    int32_t cached;
+    int32_t coeff_time_independent;
+} lss_or_cfg;
+
+/* Heston and SABR data exactly as the reference's tests write their lambdas
+ * (unit_tests/common/lss_print_t.hpp:2104-2222 and :1976-2090). */
+typedef struct lss_or_heston {
+    double strike, rate, sig_sig, sig_kappa, sig_theta, rho;
+    double bnd_low;    /* Dirichlet value at S_min (tests: 0.0)  */
+    double bnd_high;   /* Neumann value at S_max (tests: -1.0)   */
+} lss_or_heston;
+
+typedef struct lss_or_sabr {
+    double strike, maturity, rate, sig_sig, rho, beta;
+    double bnd_low, bnd_high;
+} lss_or_sabr;
+
+void lss_or_heston_pde(const lss_or_heston *p, lss_or_pde *out);
+void lss_or_sabr_pde(const lss_or_sabr *p, lss_or_pde *out);
+
+/* Full solve: out is nx*ny row-major (row = x/S index, column = y/v index).  Returns 0 or <0. */
+int lss_or_solve_f64(const lss_or_pde *pde, const lss_or_cfg *cfg, double *out);
+int lss_or_solve_f32(const lss_or_pde *pde, const lss_or_cfg *cfg, float *out);
+
+/* One time step of the loop body (implicit_kernel.hpp:196-203) at loop time `time`:
+ * next = boundary A, splitting stages, boundary B, all from prev.  prev is not modified. */
+int lss_or_step_f64(const lss_or_pde *pde, const lss_or_cfg *cfg, const double *prev, double time, double *next);
+int lss_or_step_f32(const lss_or_pde *pde, const lss_or_cfg *cfg, const float *prev, float time, float *next);
+
+/* Sampled data the drop-in must reproduce (for tests): index maps (grid_config.hpp:110-118),
+ * loop times (implicit_kernel.hpp:171-206), physical grid points X(zeta_i), Y(eta_j). */
+int lss_or_grid_f64(const lss_or_cfg *cfg, uint32_t *idx1, uint32_t *idx2, double *xs, double *ys, double *times);
+int lss_or_grid_f32(const lss_or_cfg *cfg, uint32_t *idx1, uint32_t *idx2, float *xs, float *ys, float *times);
+
+/* One tridiagonal solve with boundary folding (thomas_lu_solver.hpp:125-191 with
+ * thomas_lu_boundary.hpp:114-233; double_sweep_solver.hpp:112-155 with double_sweep_boundary.hpp:88-190).
+ * a, b, c, f have n entries; bc types are LSS_OR_BC_*; Robin uses (lin, val), others use val only. */
+int lss_or_thomas_lu_f64(int64_t n, const double *a, const double *b, const double *c, const double *f,
+                         int32_t lo_type, double lo_lin, double lo_val, int32_t hi_type, double hi_lin,
+                         double hi_val, double *x);
+int lss_or_thomas_lu_f32(int64_t n, const float *a, const float *b, const float *c, const float *f,
+                         int32_t lo_type, float lo_lin, float lo_val, int32_t hi_type, float hi_lin, float hi_val,
+                         float *x);
+int lss_or_double_sweep_f64(int64_t n, const double *a, const double *b, const double *c, const double *f,
+                            int32_t lo_type, double lo_lin, double lo_val, int32_t hi_type, double hi_lin,
+                            double hi_val, double *x);
+int lss_or_double_sweep_f32(int64_t n, const float *a, const float *b, const float *c, const float *f,
+                            int32_t lo_type, float lo_lin, float lo_val, int32_t hi_type, float hi_lin,
+                            float hi_val, float *x);
+
+/* Batched plain systems (configs "B systems of length n"): contiguous layout x[k*n+i] when
+ * strided == 0, interleaved x[i*B+k] when strided == 1.  Each system is solved by the Thomas LU
+ * restatement with Neumann(0)/Neumann(0) ends and a[0] = c[n-1] = 0, which reduces the folding to
+ * the plain algorithm (thomas_lu_boundary.hpp:136-144, 184-191). */
+int lss_or_thomas_batched_f64(int64_t B, int64_t n, int32_t strided, const double *a, const double *b,
+                              const double *c, const double *d, double *x);
+int lss_or_thomas_batched_f32(int64_t B, int64_t n, int32_t strided, const float *a, const float *b,
+                              const float *c, const float *d, float *x);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,87 @@
+"""The reference's committed ADI surfaces and the exact configuration of the test that wrote each.
+
+Each entry gives the file stem under tests/golden/reference_outputs/, the writing test
+(file:line in the reference), the PDE data and the discretization/scheme, in the reference-API
+meaning used by oracle.Cfg and by the drop-in (linear_system_solvers_amd.heston.Problem).
+"""
+import os
+
+import numpy as np
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_outputs")
+
+DR, CS, MCS, HV = 0, 1, 2, 3
+EULER, THETA_30, CN, THETA_80 = 0, 1, 2, 3
+DOUBLE_SWEEP, THOMAS_LU = 1, 3
+
+# model: ("heston", dict(strike, rate, sig_sig, sig_kappa, sig_theta, rho)) or ("sabr", dict(...))
+CASES = [
+    dict(stem="ImplHestonEquationThomasLUSolverDouglasRachfordCrankNicolson",
+         where="unit_tests/common/lss_print_t.hpp:2104-2210",
+         model=("heston", dict(strike=100.0, rate=0.03, sig_sig=0.3, sig_kappa=2.0, sig_theta=0.2, rho=0.2)),
+         cfg=dict(x_lo=50.0, x_hi=150.0, y_lo=0.0, y_hi=1.2, nx=150, ny=50, nt=200, nonuniform=False,
+                  accumulation_point=0.0, alpha_scale=3.0, beta_scale=50.0, splitting=DR, weighting=0.5,
+                  implicit_scheme=CN, tridiagonal=THOMAS_LU),
+         floats=True),
+    dict(stem="ImplHestonEquationThomasLUSolverCrankNicolson",
+         where="unit_tests/common/lss_print_t.hpp:1850-1962",
+         model=("heston", dict(strike=100.0, rate=0.03, sig_sig=0.3, sig_kappa=2.0, sig_theta=0.2, rho=0.2)),
+         cfg=dict(x_lo=30.0, x_hi=200.0, y_lo=0.0, y_hi=1.2, nx=150, ny=50, nt=200, nonuniform=True,
+                  accumulation_point=100.0, alpha_scale=3.0, beta_scale=50.0, splitting=DR, weighting=0.5,
+                  implicit_scheme=CN, tridiagonal=THOMAS_LU),
+         floats=True),
+    dict(stem="ImplHestonEquationThomasLUSolverCraigSneydCrankNicolson",
+         where="unit_tests/common/lss_print_t.hpp:2224-2333",
+         model=("heston", dict(strike=100.0, rate=0.03, sig_sig=0.3, sig_kappa=2.0, sig_theta=0.2, rho=0.2)),
+         cfg=dict(x_lo=50.0, x_hi=200.0, y_lo=0.0, y_hi=1.2, nx=150, ny=50, nt=200, nonuniform=True,
+                  accumulation_point=100.0, alpha_scale=3.0, beta_scale=50.0, splitting=CS, weighting=0.8,
+                  implicit_scheme=THETA_80, tridiagonal=THOMAS_LU),
+         floats=True),
+    dict(stem="ImplHestonEquationThomasLUSolverModCraigSneydCrankNicolson",
+         where="unit_tests/common/lss_print_t.hpp:2347-2456",
+         model=("heston", dict(strike=100.0, rate=0.03, sig_sig=0.3, sig_kappa=2.0, sig_theta=0.2, rho=0.2)),
+         cfg=dict(x_lo=50.0, x_hi=180.0, y_lo=0.0, y_hi=1.2, nx=150, ny=50, nt=200, nonuniform=True,
+                  accumulation_point=100.0, alpha_scale=3.0, beta_scale=50.0, splitting=MCS, weighting=0.5,
+                  implicit_scheme=CN, tridiagonal=THOMAS_LU),
+         floats=True),
+    dict(stem="ImplHestonEquationThomasLUSolverHundsdorferVerwerCrankNicolson",
+         where="unit_tests/common/lss_print_t.hpp:2470-2579",
+         model=("heston", dict(strike=100.0, rate=0.03, sig_sig=0.3, sig_kappa=2.0, sig_theta=0.2, rho=0.2)),
+         cfg=dict(x_lo=50.0, x_hi=200.0, y_lo=0.0, y_hi=1.2, nx=150, ny=50, nt=200, nonuniform=True,
+                  accumulation_point=100.0, alpha_scale=3.0, beta_scale=50.0, splitting=HV, weighting=0.5,
+                  implicit_scheme=CN, tridiagonal=THOMAS_LU),
+         floats=False),
+    dict(stem="ImplHestonEquationCUDAQRSolverCrankNicolson",
+         where="unit_tests/common/lss_print_t.hpp:1724-1836 (Device cuSOLVER QR; LU-equivalent at 6 digits)",
+         model=("heston", dict(strike=10.0, rate=0.03, sig_sig=0.3, sig_kappa=2.0, sig_theta=0.2, rho=0.2)),
+         cfg=dict(x_lo=0.0, x_hi=20.0, y_lo=0.0, y_hi=1.0, nx=100, ny=50, nt=150, nonuniform=True,
+                  accumulation_point=10.0, alpha_scale=10.0 / 3.0, beta_scale=1.0, splitting=DR, weighting=0.5,
+                  implicit_scheme=CN, tridiagonal=THOMAS_LU),
+         floats=True),
+    dict(stem="ImplHestonEquationDSSolverCrankNicolsonFromBuilder",
+         where="unit_tests/builders/lss_2d_general_heston_equation_builder_t.hpp:19-163",
+         model=("heston", dict(strike=10.0, rate=0.03, sig_sig=0.3, sig_kappa=2.0, sig_theta=0.2, rho=0.2)),
+         cfg=dict(x_lo=0.0, x_hi=20.0, y_lo=0.0, y_hi=1.5, nx=100, ny=50, nt=150, nonuniform=True,
+                  accumulation_point=10.0, alpha_scale=3.0, beta_scale=60.0, splitting=DR, weighting=0.5,
+                  implicit_scheme=CN, tridiagonal=DOUBLE_SWEEP),
+         floats=True),
+    dict(stem="ImplSABREquationDoubleSweepSolverCrankNicolson",
+         where="unit_tests/common/lss_print_t.hpp:1976-2090",
+         model=("sabr", dict(strike=100.0, maturity=1.0, rate=0.03, sig_sig=0.081, rho=0.6, beta=0.7)),
+         cfg=dict(x_lo=50.0, x_hi=200.0, y_lo=0.0, y_hi=1.2, nx=100, ny=50, nt=200, nonuniform=True,
+                  accumulation_point=100.0, alpha_scale=3.0, beta_scale=50.0, splitting=DR, weighting=0.5,
+                  implicit_scheme=CN, tridiagonal=DOUBLE_SWEEP),
+         floats=True),
+]
+
+
+def load(stem: str, fp: str = "f64"):
+    """(xs, ys, values) parsed from the lss_print text format."""
+    path = os.path.join(HERE, f"{stem}_{'double' if fp == 'f64' else 'float'}.txt")
+    with open(path) as fh:
+        lines = fh.read().split("\n")
+    assert lines[0] == "SPACE_POINTS_X" and lines[2] == "SPACE_POINTS_Y" and lines[4] == "VALUES"
+    xs = np.array([float(v) for v in lines[1].split(",")])
+    ys = np.array([float(v) for v in lines[3].split(",")])
+    vals = np.array([[float(v) for v in row.split(",")] for row in lines[5:] if row.strip()])
+    return xs, ys, vals
