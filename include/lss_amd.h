@@ -1,0 +1,160 @@
+/*
+ * lss_amd.h -- C-ABI of the MI355X Heston/SABR ADI library (liblss_amd.so).
+ *
+ * Plain C types only: sizes, enums, host or device pointers and a hipStream_t passed as void*.
+ * The C++ drop-in headers (include/lss/...) and the Python bindings (linear_system_solvers_amd)
+ * both sit on top of these entry points.  Enum values are those of the reference's
+ * common/lss_enumerations.hpp:15-129.
+ *
+ * Status: every entry point returns LSS_OK (0) or a negative lss_status; no C++ exception
+ * crosses this boundary.  lss_last_error() returns a message for the calling thread's last error.
+ */
+#ifndef LSS_AMD_H
+#define LSS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum lss_status {
+    LSS_OK = 0,
+    LSS_ERR_INVALID = -1,     /* bad argument, size or combination                  */
+    LSS_ERR_UNSUPPORTED = -2, /* valid in the reference API but not offered here    */
+    LSS_ERR_DEVICE = -3,      /* HIP runtime error                                  */
+    LSS_ERR_ALLOC = -4        /* host or device allocation failed                   */
+} lss_status;
+
+enum { LSS_FP64 = 0, LSS_FP32 = 1 };
+/* splitting_method_enum */
+enum { LSS_DOUGLAS_RACHFORD = 0, LSS_CRAIG_SNEYD = 1, LSS_MODIFIED_CRAIG_SNEYD = 2, LSS_HUNDSDORFER_VERWER = 3 };
+/* implicit_pde_schemes_enum */
+enum { LSS_EULER = 0, LSS_THETA_30 = 1, LSS_CRANK_NICOLSON = 2, LSS_THETA_80 = 3 };
+/* tridiagonal_method_enum: selects the line solver's operation order */
+enum { LSS_CUDA_SOLVER = 0, LSS_DOUBLE_SWEEP = 1, LSS_SOR_SOLVER = 2, LSS_THOMAS_LU = 3 };
+/* traverse_direction_enum */
+enum { LSS_FORWARD = 0, LSS_BACKWARD = 1 };
+/* grid_enum */
+enum { LSS_GRID_UNIFORM = 0, LSS_GRID_NONUNIFORM = 1 };
+/* coefficient representation of a plan */
+enum {
+    LSS_COEFF_FIELDS = 0,    /* per point M, M~, P, P~, Z, W, C + D, E, F on the v = 0 row        */
+    LSS_COEFF_SEPARABLE = 1  /* q(i,j) = X0q[i] Y0q[j] + X1q[i] Y1q[j] for q = a, b, c, d, e, f     */
+};
+/* batched tridiagonal layouts */
+enum { LSS_LAYOUT_CONTIGUOUS = 0, LSS_LAYOUT_STRIDED = 1 };
+
+/*
+ * Sampled description of one ADI problem, in the precision `fp` (arrays are double* when fp is
+ * LSS_FP64 and float* when LSS_FP32).  It carries what general_heston_equation::solve evaluates
+ * through its closures (pde_solvers/two_dimensional/heat_type/lss_2d_general_heston_equation.hpp:184-307):
+ *   coeff      coeff_sets sets; FIELDS set = 7 nx*ny fields (M, M~, P, P~, Z, W, C at (t, zeta_i, eta_j),
+ *              coefficients.hpp:89-95) then D, E, F on the v = 0 row (3 nx); SEPARABLE set = 12 nx
+ *              vectors X[2q+r] then 12 ny vectors Y[2q+r] with transformed q = a..f
+ *              (lss_heat_data_transform.hpp:174-214).  coeff_sets is 1 (time independent) or steps.
+ *   bnd        bnd_sets sets of nx + 2 ny values: v_upper(t, X(zeta_i)), h_lower(t, Y(eta_j)),
+ *              h_upper(t, Y(eta_j)) * X'(1) (lss_heston_boundary_transform.hpp:48-62).
+ *   idx1/idx2  index_of maps (lss_grid_config.hpp:110-118).
+ *   theta..h1  scheme scalars, already rounded to fp (implicit_kernel.hpp:978-995, coefficients.hpp:62-73).
+ * All pointers are host pointers; the plan copies them to device memory.
+ */
+typedef struct lss_adi_desc {
+    int32_t fp;
+    int32_t splitting;
+    int32_t tridiagonal;     /* LSS_THOMAS_LU or LSS_DOUBLE_SWEEP operation order; LSS_CUDA_SOLVER = Thomas */
+    int32_t coeff_mode;
+    int64_t nx, ny, steps;
+    double theta, weighting;
+    double alpha, beta, gamma, delta, ni, rho, h1;
+    const uint32_t *idx1, *idx2;
+    int64_t coeff_sets;
+    const void *coeff;
+    int64_t bnd_sets;
+    const void *bnd;
+} lss_adi_desc;
+
+typedef struct lss_adi_plan lss_adi_plan;
+
+const char *lss_last_error(void);
+int32_t lss_version(void);
+
+/*
+ * Plans.  A plan holds `count` independent problems (count == 1: the single-problem stepper that
+ * general_heston_equation::solve's device branch uses, heston_equation.hpp:221-233; count > 1: a
+ * batched book stepping all problems in the same launches).  All problems share fp.
+ */
+int lss_adi_plan_create(const lss_adi_desc *descs, int64_t count, lss_adi_plan **plan);
+int lss_adi_plan_destroy(lss_adi_plan *plan);
+
+/*
+ * Time loop (implicit_heston_time_loop::run, implicit_kernel.hpp:150-207) for every problem of
+ * the plan.  u_in[p] holds problem p's initial (terminal) surface, u_out[p] receives the final
+ * surface; nx*ny row-major each.  _dev: device pointers, work enqueued on `stream` (a hipStream_t,
+ * NULL = default stream), returns without synchronising.  _host: host pointers, synchronous.
+ * use_graph != 0 replays the step launches from a hipGraph captured on first use.
+ */
+int lss_adi_run_dev(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, void *stream, int32_t use_graph);
+int lss_adi_run_host(lss_adi_plan *plan, const void *const *u_in, void *const *u_out);
+
+/*
+ * Reference-model convenience: sample the Heston or SABR data of the reference's tests
+ * (unit_tests/common/lss_print_t.hpp:2156-2186 Heston; :2039-2050 SABR) on the reference's grid and
+ * build a descriptor-equivalent problem inside the library.  One lss_model per problem.
+ */
+enum { LSS_MODEL_HESTON = 0, LSS_MODEL_SABR = 1 };
+typedef struct lss_model {
+    int32_t kind;              /* LSS_MODEL_HESTON | LSS_MODEL_SABR */
+    int32_t pad;
+    /* Heston: a = 0.5 v S^2, b = 0.5 sig^2 v, c = rho sig v S, d = r S, e = kappa (theta_v - v), f = -r
+     * SABR:   D = exp(-r (T - t)); a = 0.5 al^2 F^(2 beta) D^(2(1-beta)), b = 0.5 sig^2 al^2,
+     *         c = rho sig al^2 F^beta D^(1-beta), d = r F, e = 0, f = -r                        */
+    double strike, rate, sig_sig, sig_kappa, sig_theta, rho, beta, maturity;
+    double bnd_low;            /* Dirichlet value at x_lo (reference tests: 0)  */
+    double bnd_high;           /* Neumann value at x_hi (reference tests: -1)   */
+    /* pde_discretization_config_2d + grid_config_hints_2d + splitting/solver config */
+    double x_lo, x_hi, y_lo, y_hi, t_lo, t_hi;
+    int64_t nx, ny, nt;
+    double accumulation_point, alpha_scale, beta_scale;
+    int32_t grid;              /* LSS_GRID_UNIFORM | LSS_GRID_NONUNIFORM */
+    int32_t splitting;
+    double weighting;
+    int32_t implicit_scheme;
+    int32_t tridiagonal;
+    int32_t direction;
+    int32_t coeff_mode;        /* LSS_COEFF_FIELDS (reference rounding) or LSS_COEFF_SEPARABLE */
+} lss_model;
+
+int lss_model_plan_create(const lss_model *models, int64_t count, int32_t fp, lss_adi_plan **plan);
+/* sampled initial (terminal) surface of problem p of a model plan, host pointer, nx*ny */
+int lss_model_initial(const lss_adi_plan *plan, int64_t p, void *u0_host);
+/* sizes of problem p */
+int lss_plan_problem_shape(const lss_adi_plan *plan, int64_t p, int64_t *nx, int64_t *ny, int64_t *steps);
+
+/*
+ * Batched plain tridiagonal systems (replaces one thomas_lu_solver::solve per line,
+ * sparse_solvers/tridiagonal/thomas_lu_solver/lss_thomas_lu_solver.hpp:125-191, with Neumann(0) ends
+ * and a[0] = c[n-1] = 0, i.e. the plain algorithm).  B systems of length n; layout CONTIGUOUS
+ * (x[k*n+i]) or STRIDED (x[i*B+k]); device pointers; enqueued on `stream`.  The library keeps a
+ * grow-only device workspace per device for the solver's scratch.
+ */
+int lss_tridiag_batched(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b,
+                        const void *c, const void *d, void *x, void *stream);
+
+/*
+ * Same time loop as lss_adi_run_dev (device pointers, no graph) with a hipEvent pair recorded on
+ * `stream` around every launch; synchronous.  For each kernel kind k < LSS_K_COUNT it returns the
+ * summed launch time ms[k], the summed algorithmic HBM bytes bytes[k] (compulsory reads and writes
+ * of the 2-D state arrays of the problems active in each launch) and the number of launches.
+ */
+enum {
+    LSS_K_BND_A = 0, LSS_K_XSWEEP1 = 1, LSS_K_VSWEEP1 = 2, LSS_K_HV_Y3 = 3, LSS_K_XSWEEP2 = 4, LSS_K_VSWEEP2 = 5,
+    LSS_K_BND_B = 6, LSS_K_COUNT = 7
+};
+int lss_adi_run_profiled(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, void *stream, double *ms,
+                         double *bytes, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,232 @@
+"""linear_system_solvers_amd -- MI355X (gfx950) Heston/SABR ADI time-stepper.
+
+Python front end of liblss_amd.so through its C-ABI (include/lss_amd.h).  Every compute call goes
+to the HIP library; there is no CPU path here.  If the library has not been built, importing works
+but every entry point raises LssUnavailable.
+
+Reference-API meaning of the model fields: see ``Model`` (pde_discretization_config_2d,
+grid_config_hints_2d, splitting_method_config and heat_implicit_solver_config of
+MichalSara99/linear_system_solvers).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "liblss_amd.so")
+
+FP64, FP32 = 0, 1
+DOUGLAS_RACHFORD, CRAIG_SNEYD, MODIFIED_CRAIG_SNEYD, HUNDSDORFER_VERWER = 0, 1, 2, 3
+EULER, THETA_30, CRANK_NICOLSON, THETA_80 = 0, 1, 2, 3
+CUDA_SOLVER, DOUBLE_SWEEP, SOR_SOLVER, THOMAS_LU = 0, 1, 2, 3
+FORWARD, BACKWARD = 0, 1
+UNIFORM, NONUNIFORM = 0, 1
+COEFF_FIELDS, COEFF_SEPARABLE = 0, 1
+LAYOUT_CONTIGUOUS, LAYOUT_STRIDED = 0, 1
+MODEL_HESTON, MODEL_SABR = 0, 1
+
+# every symbol include/lss_amd.h declares
+EXPORTS = ("lss_last_error", "lss_version", "lss_adi_plan_create", "lss_adi_plan_destroy", "lss_adi_run_dev",
+           "lss_adi_run_host", "lss_model_plan_create", "lss_model_initial", "lss_plan_problem_shape",
+           "lss_tridiag_batched", "lss_adi_run_profiled")
+KERNEL_KINDS = ("bnd_a", "xsweep1", "vsweep1", "hv_y3", "xsweep2", "vsweep2", "bnd_b")
+
+
+class LssUnavailable(RuntimeError):
+    pass
+
+
+class LssError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"lss error {code}: {msg}")
+        self.code = code
+
+
+class _Model(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("pad", C.c_int32),
+                *[(n, C.c_double) for n in ("strike", "rate", "sig_sig", "sig_kappa", "sig_theta", "rho", "beta",
+                                            "maturity", "bnd_low", "bnd_high", "x_lo", "x_hi", "y_lo", "y_hi",
+                                            "t_lo", "t_hi")],
+                ("nx", C.c_int64), ("ny", C.c_int64), ("nt", C.c_int64),
+                ("accumulation_point", C.c_double), ("alpha_scale", C.c_double), ("beta_scale", C.c_double),
+                ("grid", C.c_int32), ("splitting", C.c_int32), ("weighting", C.c_double),
+                ("implicit_scheme", C.c_int32), ("tridiagonal", C.c_int32), ("direction", C.c_int32),
+                ("coeff_mode", C.c_int32)]
+
+
+class _Desc(C.Structure):
+    _fields_ = [("fp", C.c_int32), ("splitting", C.c_int32), ("tridiagonal", C.c_int32), ("coeff_mode", C.c_int32),
+                ("nx", C.c_int64), ("ny", C.c_int64), ("steps", C.c_int64),
+                *[(n, C.c_double) for n in ("theta", "weighting", "alpha", "beta", "gamma", "delta", "ni", "rho",
+                                            "h1")],
+                ("idx1", C.c_void_p), ("idx2", C.c_void_p), ("coeff_sets", C.c_int64), ("coeff", C.c_void_p),
+                ("bnd_sets", C.c_int64), ("bnd", C.c_void_p)]
+
+
+@dataclasses.dataclass
+class Model:
+    """One Heston or SABR problem with the reference tests' data (see include/lss_amd.h, lss_model)."""
+    kind: int = MODEL_HESTON
+    strike: float = 100.0
+    rate: float = 0.03
+    sig_sig: float = 0.3
+    sig_kappa: float = 2.0
+    sig_theta: float = 0.2
+    rho: float = 0.2
+    beta: float = 0.7
+    maturity: float = 1.0
+    bnd_low: float = 0.0
+    bnd_high: float = -1.0
+    x_lo: float = 50.0
+    x_hi: float = 200.0
+    y_lo: float = 0.0
+    y_hi: float = 1.2
+    t_lo: float = 0.0
+    t_hi: float = 1.0
+    nx: int = 100
+    ny: int = 50
+    nt: int = 101
+    accumulation_point: float = 0.0
+    alpha_scale: float = 3.0
+    beta_scale: float = 50.0
+    grid: int = UNIFORM
+    splitting: int = DOUGLAS_RACHFORD
+    weighting: float = 0.5
+    implicit_scheme: int = CRANK_NICOLSON
+    tridiagonal: int = THOMAS_LU
+    direction: int = BACKWARD
+    coeff_mode: int = COEFF_FIELDS
+
+    def c(self) -> _Model:
+        m = _Model()
+        for f in dataclasses.fields(self):
+            setattr(m, f.name, getattr(self, f.name))
+        return m
+
+    @property
+    def steps(self) -> int:
+        return self.nt - 1
+
+
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LssUnavailable(f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        # The library and PyTorch both need libamdhip64.so.7; whichever is loaded first serves the
+        # whole process.  Load torch's first when torch is present so both share one HIP runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = C.CDLL(LIB_PATH)
+        L.lss_last_error.restype = C.c_char_p
+        L.lss_adi_plan_create.argtypes = [C.POINTER(_Desc), C.c_int64, C.POINTER(C.c_void_p)]
+        L.lss_adi_plan_destroy.argtypes = [C.c_void_p]
+        L.lss_adi_run_dev.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p,
+                                      C.c_int32]
+        L.lss_adi_run_host.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
+        L.lss_model_plan_create.argtypes = [C.POINTER(_Model), C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]
+        L.lss_model_initial.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+        L.lss_plan_problem_shape.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                             C.POINTER(C.c_int64)]
+        L.lss_adi_run_profiled.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p,
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.lss_tridiag_batched.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise LssError(rc, lib().lss_last_error().decode())
+
+
+def np_dtype(fp: int):
+    return np.float64 if fp == FP64 else np.float32
+
+
+class Plan:
+    """A set of ADI problems stepped together on the GPU (one problem: the single-problem stepper)."""
+
+    def __init__(self, handle: int, fp: int, count: int):
+        self._h = C.c_void_p(handle)
+        self.fp = fp
+        self.count = count
+
+    @classmethod
+    def from_models(cls, models: Sequence[Model], fp: int = FP64) -> "Plan":
+        arr = (_Model * len(models))(*[m.c() for m in models])
+        h = C.c_void_p()
+        _check(lib().lss_model_plan_create(arr, len(models), fp, C.byref(h)))
+        return cls(h.value, fp, len(models))
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value:
+            lib().lss_adi_plan_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, p: int):
+        nx, ny, st = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().lss_plan_problem_shape(self._h, p, C.byref(nx), C.byref(ny), C.byref(st)))
+        return nx.value, ny.value, st.value
+
+    def initial(self, p: int) -> np.ndarray:
+        nx, ny, _ = self.shape(p)
+        out = np.zeros((nx, ny), dtype=np_dtype(self.fp))
+        _check(lib().lss_model_initial(self._h, p, out.ctypes.data))
+        return out
+
+    def run_host(self, inits: Optional[Sequence[np.ndarray]] = None) -> list:
+        """Synchronous solve from host arrays (default: the model plan's sampled initial surfaces)."""
+        dt = np_dtype(self.fp)
+        if inits is None:
+            inits = [self.initial(p) for p in range(self.count)]
+        ins = [np.ascontiguousarray(a, dtype=dt) for a in inits]
+        outs = [np.zeros_like(a) for a in ins]
+        pin = (C.c_void_p * self.count)(*[a.ctypes.data for a in ins])
+        pout = (C.c_void_p * self.count)(*[a.ctypes.data for a in outs])
+        _check(lib().lss_adi_run_host(self._h, pin, pout))
+        return outs
+
+    def run_dev(self, in_ptrs: Sequence[int], out_ptrs: Sequence[int], stream: int = 0, graph: bool = True) -> None:
+        """Enqueue a solve on `stream` from/to device pointers (e.g. torch tensors' data_ptr())."""
+        pin = (C.c_void_p * self.count)(*in_ptrs)
+        pout = (C.c_void_p * self.count)(*out_ptrs)
+        _check(lib().lss_adi_run_dev(self._h, pin, pout, C.c_void_p(stream), int(graph)))
+
+
+def run_profiled(plan: "Plan", in_ptrs: Sequence[int], out_ptrs: Sequence[int], stream: int = 0) -> dict:
+    """Per-kernel-kind event timing of one solve: {kind: (ms, algorithmic_bytes, launches)}."""
+    n = len(KERNEL_KINDS)
+    ms, by, nl = (C.c_double * n)(), (C.c_double * n)(), (C.c_int64 * n)()
+    pin = (C.c_void_p * plan.count)(*in_ptrs)
+    pout = (C.c_void_p * plan.count)(*out_ptrs)
+    _check(lib().lss_adi_run_profiled(plan._h, pin, pout, C.c_void_p(stream), ms, by, nl))
+    return {k: (ms[i], by[i], nl[i]) for i, k in enumerate(KERNEL_KINDS)}
+
+
+def tridiag_batched(fp: int, layout: int, B: int, n: int, a: int, b: int, c: int, d: int, x: int,
+                    stream: int = 0) -> None:
+    """Solve B plain tridiagonal systems of length n (device pointers; a[0], c[n-1] ignored)."""
+    _check(lib().lss_tridiag_batched(fp, layout, B, n, C.c_void_p(a), C.c_void_p(b), C.c_void_p(c),
+                                     C.c_void_p(d), C.c_void_p(x), C.c_void_p(stream)))

@@ -1,0 +1,820 @@
+// lss_adi_capi.hip -- host side of liblss_amd.so: plans, time loop, hipGraph capture, C-ABI.
+//
+// A plan owns all device memory of its problems (state ping-pong, intermediates, line-solver
+// scratch, sampled coefficients and boundaries, index maps) and a problem table.  Problems are
+// ordered by step count, descending, so that at loop step s the problems still stepping form a
+// prefix and each launch covers exactly the active tiles.  The whole loop is recorded once into a
+// hipGraph (on a plan-owned stream) and replayed on the caller's stream.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lss_adi_device.hpp"
+#include "lss_amd.h"
+#include "lss/lss_adi_sampler.hpp"
+
+namespace lssamd {
+
+template <typename T> __global__ void k_bnd_a(const Prob<T> *, const Tile *, int);
+template <typename T> __global__ void k_bnd_b(const Prob<T> *, const Tile *, int);
+template <typename T> __global__ void k_xsweep(const Prob<T> *, const Tile *, int, int);
+template <typename T> __global__ void k_vsweep(const Prob<T> *, const Tile *, int, int);
+template <typename T> __global__ void k_hv_y3(const Prob<T> *, const Tile *, int);
+template <typename T>
+__global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *);
+
+}  // namespace lssamd
+
+using namespace lssamd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct lss_error : std::runtime_error {
+    int code;
+    lss_error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                                           \
+    do {                                                                                                       \
+        hipError_t e_ = (expr);                                                                                \
+        if (e_ != hipSuccess)                                                                                  \
+            throw lss_error(LSS_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));                \
+    } while (0)
+
+template <typename F> int guarded(F &&f)
+{
+    try {
+        f();
+        return LSS_OK;
+    } catch (const lss_error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host allocation failed";
+        return LSS_ERR_ALLOC;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return LSS_ERR_INVALID;
+    }
+}
+
+void require(bool ok, const std::string &msg)
+{
+    if (!ok)
+        throw lss_error(LSS_ERR_INVALID, msg);
+}
+
+struct dev_buffer {
+    void *p = nullptr;
+    dev_buffer() = default;
+    dev_buffer(const dev_buffer &) = delete;
+    dev_buffer &operator=(const dev_buffer &) = delete;
+    ~dev_buffer()
+    {
+        if (p)
+            (void)hipFree(p);
+    }
+    void alloc(size_t bytes)
+    {
+        if (bytes == 0)
+            bytes = 256;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw lss_error(LSS_ERR_ALLOC, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+    }
+};
+
+// Layout of one problem's device allocation, in elements of T (index maps appended as uint32).
+struct ProbLayout {
+    size_t off_a, off_b, off_y1, off_y2, off_y3, off_scr, off_colA0, off_colA1, off_coeff, off_bnd;
+    size_t n_T;          // number of T elements
+    size_t off_idx_bytes, bytes;
+};
+
+}  // namespace
+
+struct lss_adi_plan {
+    virtual ~lss_adi_plan() = default;
+    int fp = LSS_FP64;
+    int64_t count = 0;
+    std::vector<int64_t> order;      // internal position -> caller index
+    std::vector<int64_t> where;      // caller index -> internal position
+    std::vector<int64_t> nx, ny, steps;
+    std::vector<std::vector<char>> init_host;  // model plans: sampled initial surface per caller index
+    virtual void run(const void *const *u_in, void *const *u_out, hipStream_t stream, bool use_graph, bool host) = 0;
+    virtual void profile(const void *const *u_in, void *const *u_out, hipStream_t stream, double *ms, double *bytes,
+                         int64_t *launches) = 0;
+};
+
+namespace {
+
+// Per-launch event timing of a plan run (used by the benchmark for the roofline figure).
+struct Profiler {
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kind;
+    std::vector<double> bytes;
+    hipStream_t st = nullptr;
+    ~Profiler()
+    {
+        for (auto e : ev)
+            (void)hipEventDestroy(e);
+    }
+    void begin(int k, double b)
+    {
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+        ev.push_back(e0);
+        ev.push_back(e1);
+        kind.push_back(k);
+        bytes.push_back(b);
+    }
+    void end() { HIPCHK(hipEventRecord(ev.back(), st)); }
+};
+
+template <typename T> struct PlanImpl final : lss_adi_plan {
+    std::vector<Prob<T>> probs;        // host copy (device pointers inside)
+    std::vector<ProbLayout> layouts;
+    dev_buffer mem;                    // all problems
+    dev_buffer probs_dev, tiles_dev;
+    // tile tables: rows [0,nx), cols [0,ny), S-lines [1,ny-2], v-lines [1,nx-2], interior rows
+    enum { TB_I = 0, TB_J, TB_X, TB_V, TB_R, TB_N };
+    std::vector<Tile> tiles[TB_N];
+    size_t tile_off[TB_N] = {};
+    std::vector<int> prefix[TB_N];     // prefix[tb][q] = tiles of internal problems [0, q)
+    int max_steps = 0;
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+
+    ~PlanImpl() override
+    {
+        if (graph_exec)
+            (void)hipGraphExecDestroy(graph_exec);
+        if (graph)
+            (void)hipGraphDestroy(graph);
+        if (cap_stream)
+            (void)hipStreamDestroy(cap_stream);
+    }
+
+    static void validate(const lss_adi_desc &d)
+    {
+        require(d.nx >= 4 && d.ny >= 4, "nx and ny must be >= 4");
+        require(d.nx <= (1 << 20) && d.ny <= (1 << 20) && d.nx * d.ny <= (int64_t)1 << 31,
+                "grid too large for one problem");
+        require(d.steps >= 1 && d.steps < (1 << 30), "steps must be >= 1");
+        require(d.splitting >= 0 && d.splitting <= 3, "unknown splitting method");
+        require(d.tridiagonal == LSS_THOMAS_LU || d.tridiagonal == LSS_DOUBLE_SWEEP ||
+                    d.tridiagonal == LSS_CUDA_SOLVER,
+                "tridiagonal method must be ThomasLU, DoubleSweep or CUDASolver");
+        require(d.coeff_mode == LSS_COEFF_FIELDS || d.coeff_mode == LSS_COEFF_SEPARABLE, "unknown coeff_mode");
+        require(d.coeff_sets == 1 || d.coeff_sets == d.steps, "coeff_sets must be 1 or steps");
+        require(d.bnd_sets == 1 || d.bnd_sets == d.steps, "bnd_sets must be 1 or steps");
+        require(d.idx1 && d.idx2 && d.coeff && d.bnd, "null array in descriptor");
+        for (int64_t i = 0; i < d.nx; ++i)
+            require(d.idx1[i] < (uint32_t)d.nx, "idx1 out of range");
+        for (int64_t j = 0; j < d.ny; ++j)
+            require(d.idx2[j] < (uint32_t)d.ny, "idx2 out of range");
+    }
+
+    static size_t coeff_stride(const lss_adi_desc &d)
+    {
+        return d.coeff_mode == LSS_COEFF_FIELDS ? (size_t)(7 * d.nx * d.ny + 3 * d.nx) : (size_t)(12 * (d.nx + d.ny));
+    }
+
+    void build(const lss_adi_desc *descs, int64_t n)
+    {
+        count = n;
+        require(n >= 1, "count must be >= 1");
+        for (int64_t p = 0; p < n; ++p) {
+            require(descs[p].fp == (sizeof(T) == 8 ? LSS_FP64 : LSS_FP32), "all problems of a plan share fp");
+            validate(descs[p]);
+        }
+        order.resize(n);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int64_t x, int64_t y) { return descs[x].steps > descs[y].steps; });
+        where.resize(n);
+        for (int64_t q = 0; q < n; ++q)
+            where[order[q]] = q;
+        nx.resize(n);
+        ny.resize(n);
+        steps.resize(n);
+        // layouts
+        layouts.resize(n);
+        size_t total = 0;
+        for (int64_t q = 0; q < n; ++q) {
+            const lss_adi_desc &d = descs[order[q]];
+            const size_t nxy = (size_t)(d.nx * d.ny);
+            ProbLayout L;
+            size_t o = 0;
+            auto take = [&](size_t k) {
+                const size_t r = o;
+                o += (k + 63) / 64 * 64;  // keep every array 256/512-byte aligned
+                return r;
+            };
+            L.off_a = take(nxy);
+            L.off_b = take(nxy);
+            L.off_y1 = take(nxy);
+            L.off_y2 = d.splitting == LSS_DOUGLAS_RACHFORD ? 0 : take(nxy);
+            L.off_y3 = d.splitting == LSS_DOUGLAS_RACHFORD ? 0 : take(nxy);
+            L.off_scr = take(nxy);
+            L.off_colA0 = take((size_t)d.nx);
+            L.off_colA1 = take((size_t)d.nx);
+            L.off_coeff = take(coeff_stride(d) * (size_t)d.coeff_sets);
+            L.off_bnd = take((size_t)(d.nx + 2 * d.ny) * (size_t)d.bnd_sets);
+            L.n_T = o;
+            L.off_idx_bytes = o * sizeof(T);
+            L.bytes = L.off_idx_bytes + ((size_t)(d.nx + d.ny) * 4 + 255) / 256 * 256;
+            layouts[q] = L;
+            total += L.bytes;
+        }
+        mem.alloc(total);
+        HIPCHK(hipMemset(mem.p, 0, total));
+        probs.resize(n);
+        char *base = (char *)mem.p;
+        for (int64_t q = 0; q < n; ++q) {
+            const lss_adi_desc &d = descs[order[q]];
+            const ProbLayout &L = layouts[q];
+            T *t = (T *)base;
+            Prob<T> &p = probs[q];
+            std::memset(&p, 0, sizeof(p));
+            p.nx = (int32_t)d.nx;
+            p.ny = (int32_t)d.ny;
+            p.steps = (int32_t)d.steps;
+            p.scheme = d.splitting;
+            p.tri = d.tridiagonal == LSS_DOUBLE_SWEEP ? TRI_DSWEEP : TRI_THOMAS;
+            p.coeff_mode = d.coeff_mode == LSS_COEFF_FIELDS ? COEFF_FIELDS : COEFF_SEPARABLE;
+            p.coeff_sets = (int32_t)d.coeff_sets;
+            p.bnd_sets = (int32_t)d.bnd_sets;
+            p.theta = (T)d.theta;
+            p.zeta = (T)d.weighting;
+            p.alpha = (T)d.alpha;
+            p.beta = (T)d.beta;
+            p.gamma = (T)d.gamma;
+            p.delta = (T)d.delta;
+            p.ni = (T)d.ni;
+            p.rho = (T)d.rho;
+            p.h1 = (T)d.h1;
+            p.a = t + L.off_a;
+            p.b = t + L.off_b;
+            p.y1 = t + L.off_y1;
+            p.y2 = d.splitting == LSS_DOUGLAS_RACHFORD ? nullptr : t + L.off_y2;
+            p.y3 = d.splitting == LSS_DOUGLAS_RACHFORD ? nullptr : t + L.off_y3;
+            p.scr = t + L.off_scr;
+            p.colA0 = t + L.off_colA0;
+            p.colA1 = t + L.off_colA1;
+            p.coeff = t + L.off_coeff;
+            p.bnd = t + L.off_bnd;
+            p.coeff_stride = (int64_t)coeff_stride(d);
+            p.bnd_stride = d.nx + 2 * d.ny;
+            uint32_t *ix = (uint32_t *)(base + L.off_idx_bytes);
+            p.idx1 = ix;
+            p.idx2 = ix + d.nx;
+            HIPCHK(hipMemcpy((void *)p.coeff, d.coeff, sizeof(T) * coeff_stride(d) * (size_t)d.coeff_sets,
+                             hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy((void *)p.bnd, d.bnd, sizeof(T) * (size_t)(d.nx + 2 * d.ny) * (size_t)d.bnd_sets,
+                             hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(ix, d.idx1, sizeof(uint32_t) * (size_t)d.nx, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(ix + d.nx, d.idx2, sizeof(uint32_t) * (size_t)d.ny, hipMemcpyHostToDevice));
+            nx[order[q]] = d.nx;
+            ny[order[q]] = d.ny;
+            steps[order[q]] = d.steps;
+            max_steps = std::max(max_steps, (int)d.steps);
+            base += L.bytes;
+        }
+        // tile tables
+        for (int tb = 0; tb < TB_N; ++tb) {
+            prefix[tb].assign(n + 1, 0);
+            for (int64_t q = 0; q < n; ++q) {
+                const Prob<T> &p = probs[q];
+                int lo = 0, hi = 0, step = TILE;
+                switch (tb) {
+                case TB_I: lo = 0, hi = p.nx; break;
+                case TB_J: lo = 0, hi = p.ny; break;
+                case TB_X: lo = 1, hi = p.ny - 1; break;
+                case TB_V: lo = 1, hi = p.nx - 1; break;
+                default: lo = 1, hi = p.nx - 1, step = 1; break;
+                }
+                for (int s = lo; s < hi; s += step) {
+                    Tile t;
+                    t.prob = (int32_t)q;
+                    t.first = s;
+                    t.count = std::min(step, hi - s);
+                    t.pad = 0;
+                    tiles[tb].push_back(t);
+                }
+                prefix[tb][q + 1] = (int)tiles[tb].size();
+            }
+        }
+        size_t ntiles = 0;
+        for (int tb = 0; tb < TB_N; ++tb) {
+            tile_off[tb] = ntiles;
+            ntiles += tiles[tb].size();
+        }
+        tiles_dev.alloc(ntiles * sizeof(Tile));
+        for (int tb = 0; tb < TB_N; ++tb)
+            HIPCHK(hipMemcpy((Tile *)tiles_dev.p + tile_off[tb], tiles[tb].data(), tiles[tb].size() * sizeof(Tile),
+                             hipMemcpyHostToDevice));
+        probs_dev.alloc(n * sizeof(Prob<T>));
+        HIPCHK(hipMemcpy(probs_dev.p, probs.data(), n * sizeof(Prob<T>), hipMemcpyHostToDevice));
+    }
+
+    // number of internal problems still stepping at loop step s (problems sorted by steps desc)
+    int64_t active(int s) const
+    {
+        int64_t lo = 0, hi = count;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) / 2;
+            if (probs[mid].steps > s)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        return lo;
+    }
+
+    // algorithmic HBM bytes of one launch of kernel kind k at step s: compulsory reads and writes of
+    // the 2-D state arrays (coefficients and boundary data are O(nx + ny) and not counted)
+    double algo_bytes(int k, int64_t na) const
+    {
+        double b = 0;
+        for (int64_t q = 0; q < na; ++q) {
+            const Prob<T> &p = probs[q];
+            const double nxy = (double)p.nx * p.ny;
+            const bool dr = p.scheme == SCHEME_DR, hv = p.scheme == SCHEME_HV;
+            switch (k) {
+            case LSS_K_BND_A: b += 5.0 * p.nx; break;
+            case LSS_K_XSWEEP1: b += 2 * nxy; break;
+            case LSS_K_VSWEEP1: b += 3 * nxy; break;
+            case LSS_K_HV_Y3: b += hv ? 4 * nxy : 0; break;
+            case LSS_K_XSWEEP2: b += dr ? 0 : (hv ? 3 * nxy : 4 * nxy); break;
+            case LSS_K_VSWEEP2: b += dr ? 0 : 3 * nxy; break;
+            default: b += 4.0 * p.ny; break;
+            }
+        }
+        return b * sizeof(T);
+    }
+
+    void launch_step(int s, hipStream_t st, Profiler *prof = nullptr)
+    {
+        const int64_t na = active(s);
+        const Prob<T> *pd = (const Prob<T> *)probs_dev.p;
+        const Tile *td = (const Tile *)tiles_dev.p;
+        auto grid = [&](int tb) { return (unsigned)prefix[tb][na]; };
+        bool any_split2 = false, any_hv = false;
+        for (int64_t q = 0; q < na; ++q) {
+            any_split2 |= probs[q].scheme != SCHEME_DR;
+            any_hv |= probs[q].scheme == SCHEME_HV;
+        }
+        auto pre = [&](int k) {
+            if (prof)
+                prof->begin(k, algo_bytes(k, na));
+        };
+        auto post = [&]() {
+            if (prof)
+                prof->end();
+        };
+        pre(LSS_K_BND_A);
+        hipLaunchKernelGGL(k_bnd_a<T>, dim3(grid(TB_I)), dim3(TILE), 0, st, pd, td + tile_off[TB_I], s);
+        post();
+        pre(LSS_K_XSWEEP1);
+        hipLaunchKernelGGL(k_xsweep<T>, dim3(grid(TB_X)), dim3(TILE), 0, st, pd, td + tile_off[TB_X], s, 1);
+        post();
+        pre(LSS_K_VSWEEP1);
+        hipLaunchKernelGGL(k_vsweep<T>, dim3(grid(TB_V)), dim3(TILE), 0, st, pd, td + tile_off[TB_V], s, 1);
+        post();
+        if (any_hv) {
+            pre(LSS_K_HV_Y3);
+            hipLaunchKernelGGL(k_hv_y3<T>, dim3(grid(TB_R)), dim3(TILE), 0, st, pd, td + tile_off[TB_R], s);
+            post();
+        }
+        if (any_split2) {
+            pre(LSS_K_XSWEEP2);
+            hipLaunchKernelGGL(k_xsweep<T>, dim3(grid(TB_X)), dim3(TILE), 0, st, pd, td + tile_off[TB_X], s, 2);
+            post();
+            pre(LSS_K_VSWEEP2);
+            hipLaunchKernelGGL(k_vsweep<T>, dim3(grid(TB_V)), dim3(TILE), 0, st, pd, td + tile_off[TB_V], s, 2);
+            post();
+        }
+        pre(LSS_K_BND_B);
+        hipLaunchKernelGGL(k_bnd_b<T>, dim3(grid(TB_J)), dim3(TILE), 0, st, pd, td + tile_off[TB_J], s);
+        post();
+    }
+
+    void launch_loop(hipStream_t st)
+    {
+        for (int s = 0; s < max_steps; ++s)
+            launch_step(s, st);
+        HIPCHK(hipGetLastError());
+    }
+
+    void profile(const void *const *u_in, void *const *u_out, hipStream_t stream, double *ms, double *bytes,
+                 int64_t *launches) override
+    {
+        Profiler prof;
+        prof.st = stream;
+        for (int64_t q = 0; q < count; ++q) {
+            const Prob<T> &p = probs[q];
+            HIPCHK(hipMemcpyAsync(p.a, u_in[order[q]], sizeof(T) * (size_t)p.nx * p.ny, hipMemcpyDeviceToDevice,
+                                  stream));
+        }
+        for (int s = 0; s < max_steps; ++s)
+            launch_step(s, stream, &prof);
+        HIPCHK(hipGetLastError());
+        for (int64_t q = 0; q < count; ++q) {
+            const Prob<T> &p = probs[q];
+            const T *res = (p.steps & 1) ? p.b : p.a;
+            HIPCHK(hipMemcpyAsync(u_out[order[q]], res, sizeof(T) * (size_t)p.nx * p.ny, hipMemcpyDeviceToDevice,
+                                  stream));
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+        for (int k = 0; k < LSS_K_COUNT; ++k) {
+            ms[k] = 0;
+            bytes[k] = 0;
+            launches[k] = 0;
+        }
+        for (size_t l = 0; l < prof.kind.size(); ++l) {
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, prof.ev[2 * l], prof.ev[2 * l + 1]));
+            ms[prof.kind[l]] += t;
+            bytes[prof.kind[l]] += prof.bytes[l];
+            launches[prof.kind[l]] += 1;
+        }
+    }
+
+    void run(const void *const *u_in, void *const *u_out, hipStream_t stream, bool use_graph, bool host) override
+    {
+        require(u_in && u_out, "null pointer array");
+        for (int64_t c = 0; c < count; ++c)
+            require(u_in[c] && u_out[c], "null surface pointer");
+        const hipMemcpyKind kin = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+        const hipMemcpyKind kout = host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        for (int64_t q = 0; q < count; ++q) {
+            const Prob<T> &p = probs[q];
+            HIPCHK(hipMemcpyAsync(p.a, u_in[order[q]], sizeof(T) * (size_t)p.nx * p.ny, kin, stream));
+        }
+        if (use_graph) {
+            if (!graph_exec) {
+                HIPCHK(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+                HIPCHK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
+                for (int s = 0; s < max_steps; ++s)
+                    launch_step(s, cap_stream);
+                HIPCHK(hipStreamEndCapture(cap_stream, &graph));
+                HIPCHK(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+            }
+            HIPCHK(hipGraphLaunch(graph_exec, stream));
+        } else {
+            launch_loop(stream);
+        }
+        for (int64_t q = 0; q < count; ++q) {
+            const Prob<T> &p = probs[q];
+            const T *res = (p.steps & 1) ? p.b : p.a;
+            HIPCHK(hipMemcpyAsync(u_out[order[q]], res, sizeof(T) * (size_t)p.nx * p.ny, kout, stream));
+        }
+        if (host)
+            HIPCHK(hipStreamSynchronize(stream));
+    }
+};
+
+lss_adi_plan *make_plan(int fp, const lss_adi_desc *descs, int64_t n)
+{
+    if (fp == LSS_FP64) {
+        auto p = std::make_unique<PlanImpl<double>>();
+        p->fp = fp;
+        p->build(descs, n);
+        return p.release();
+    }
+    if (fp == LSS_FP32) {
+        auto p = std::make_unique<PlanImpl<float>>();
+        p->fp = fp;
+        p->build(descs, n);
+        return p.release();
+    }
+    throw lss_error(LSS_ERR_INVALID, "fp must be LSS_FP64 or LSS_FP32");
+}
+
+// ------------------------------------------------------------------------------------------------
+// Reference models (the data of the reference's Heston and SABR tests)
+// ------------------------------------------------------------------------------------------------
+template <typename T> lss_amd_detail::grid_spec<T> spec_of(const lss_model &m)
+{
+    lss_amd_detail::grid_spec<T> g;
+    g.x_lo = (T)m.x_lo;
+    g.x_hi = (T)m.x_hi;
+    g.y_lo = (T)m.y_lo;
+    g.y_hi = (T)m.y_hi;
+    g.t_lo = (T)m.t_lo;
+    g.t_hi = (T)m.t_hi;
+    g.nx = (size_t)m.nx;
+    g.ny = (size_t)m.ny;
+    g.nt = (size_t)m.nt;
+    g.accumulation_point = (T)m.accumulation_point;
+    g.alpha_scale = (T)m.alpha_scale;
+    g.beta_scale = (T)m.beta_scale;
+    g.nonuniform = m.grid == LSS_GRID_NONUNIFORM;
+    g.splitting = m.splitting;
+    g.weighting = (T)m.weighting;
+    g.implicit_scheme = m.implicit_scheme;
+    g.tridiagonal = m.tridiagonal;
+    g.direction = m.direction;
+    return g;
+}
+
+// The reference's test lambdas capture double constants, so each expression is evaluated in
+// double and rounded to T on return (lss_print_t.hpp:2156-2186, 2039-2050).
+template <typename T> lss_amd_detail::user_data<T> closures_of(const lss_model &m)
+{
+    lss_amd_detail::user_data<T> u;
+    const double K = m.strike, r = m.rate, sig = m.sig_sig, kap = m.sig_kappa, th = m.sig_theta, rho = m.rho;
+    const double be = m.beta, mat = m.maturity, lo = m.bnd_low, hi = m.bnd_high;
+    if (m.kind == LSS_MODEL_HESTON) {
+        u.A = [](T, T s, T v) { return (T)(0.5 * v * s * s); };
+        u.B = [=](T, T, T v) { return (T)(0.5 * sig * sig * v); };
+        u.C = [=](T, T s, T v) { return (T)(rho * sig * v * s); };
+        u.D = [=](T, T s, T) { return (T)(r * s); };
+        u.E = [=](T, T, T v) { return (T)(kap * (th - v)); };
+        u.F = [=](T, T, T) { return (T)(-r); };
+    } else {
+        auto Df = [=](T t) { return std::exp(-r * (mat - t)); };
+        u.A = [=](T t, T s, T al) {
+            return (T)(0.5 * al * al * std::pow((double)s, 2.0 * be) * std::pow(Df(t), 2.0 * (1.0 - be)));
+        };
+        u.B = [=](T, T, T al) { return (T)(0.5 * sig * sig * al * al); };
+        u.C = [=](T t, T s, T al) {
+            return (T)(rho * sig * al * al * std::pow((double)s, be) * std::pow(Df(t), (1.0 - be)));
+        };
+        u.D = [=](T, T s, T) { return (T)(r * s); };
+        u.E = [](T, T, T) { return (T)0.0; };
+        u.F = [=](T, T, T) { return (T)(-r); };
+    }
+    u.init = [=](T s, T) { return std::max<T>((T)0.0, (T)(s - K)); };
+    u.h_lower = [=](T, T) { return (T)lo; };
+    u.h_upper = [=](T, T) { return (T)hi; };
+    u.v_upper = [](T, T s) { return s; };
+    return u;
+}
+
+// Separable factors of the transformed coefficients for the two models (coefficients evaluated
+// in double from the T-rounded grid data, then rounded to T).
+template <typename T>
+void separable_set(const lss_model &m, const lss_amd_detail::grid_spec<T> &g,
+                   const lss_amd_detail::sampled<T> &s, T t, T *out)
+{
+    const lss_amd_detail::grid_map<T> gm(g);
+    const size_t nx = g.nx, ny = g.ny;
+    T *X = out, *Y = out + 12 * nx;
+    std::fill(out, out + 12 * (nx + ny), (T)0);
+    auto x = [&](int q, int r, size_t i) -> T & { return X[(2 * q + r) * nx + i]; };
+    auto y = [&](int q, int r, size_t j) -> T & { return Y[(2 * q + r) * ny + j]; };
+    const T h_1 = T(1) / static_cast<T>(nx - 1), h_2 = T(1) / static_cast<T>(ny - 1);
+    const double r = m.rate, sig = m.sig_sig, rho = m.rho;
+    const double Dt = m.kind == LSS_MODEL_SABR ? std::exp(-r * (m.maturity - (double)t)) : 1.0;
+    for (size_t i = 0; i < nx; ++i) {
+        const T z = h_1 * static_cast<T>(i);
+        const double S = s.xs[i], dX = gm.dx(z), ddX = gm.ddx(z);
+        double aX, cX;
+        if (m.kind == LSS_MODEL_HESTON) {
+            aX = 0.5 * S * S;
+            cX = S;
+        } else {
+            aX = 0.5 * std::pow(S, 2.0 * m.beta) * std::pow(Dt, 2.0 * (1.0 - m.beta));
+            cX = std::pow(S, m.beta) * std::pow(Dt, 1.0 - m.beta);
+        }
+        x(0, 0, i) = (T)(aX / (dX * dX));
+        x(1, 0, i) = (T)1;
+        x(2, 0, i) = (T)(cX / dX);
+        x(3, 0, i) = (T)(r * S / dX);
+        x(3, 1, i) = (T)(-aX * ddX / (dX * dX * dX));
+        x(4, 0, i) = (T)1;
+        x(5, 0, i) = (T)1;
+    }
+    for (size_t j = 0; j < ny; ++j) {
+        const T e = h_2 * static_cast<T>(j);
+        const double V = s.ys[j], dY = gm.dy(e), ddY = gm.ddy(e);
+        double aY, bY, cY, eY;
+        if (m.kind == LSS_MODEL_HESTON) {
+            aY = V;
+            bY = 0.5 * sig * sig * V;
+            cY = rho * sig * V;
+            eY = m.sig_kappa * (m.sig_theta - V);
+        } else {
+            aY = V * V;
+            bY = 0.5 * sig * sig * V * V;
+            cY = rho * sig * V * V;
+            eY = 0.0;
+        }
+        y(0, 0, j) = (T)aY;
+        y(1, 0, j) = (T)(bY / (dY * dY));
+        y(2, 0, j) = (T)(cY / dY);
+        y(3, 0, j) = (T)1;
+        y(3, 1, j) = (T)aY;
+        y(4, 0, j) = (T)(eY / dY - bY * ddY / (dY * dY * dY));
+        y(5, 0, j) = (T)(-r);
+    }
+}
+
+template <typename T> struct model_samples {
+    std::vector<lss_amd_detail::sampled<T>> s;
+    std::vector<lss_adi_desc> d;
+};
+
+template <typename T> lss_adi_plan *model_plan(const lss_model *models, int64_t n)
+{
+    model_samples<T> ms;
+    ms.s.resize(n);
+    ms.d.resize(n);
+    for (int64_t q = 0; q < n; ++q) {
+        const lss_model &m = models[q];
+        require(m.kind == LSS_MODEL_HESTON || m.kind == LSS_MODEL_SABR, "unknown model kind");
+        require(m.nx >= 4 && m.ny >= 4 && m.nt >= 2, "model grid too small");
+        require(m.coeff_mode == LSS_COEFF_FIELDS || m.coeff_mode == LSS_COEFF_SEPARABLE, "unknown coeff_mode");
+        const auto g = spec_of<T>(m);
+        const auto u = closures_of<T>(m);
+        auto &s = ms.s[q];
+        const bool tdep = m.kind == LSS_MODEL_SABR && m.rate != 0.0;
+        const auto cmode = tdep ? lss_amd_detail::TIME_DEPENDENT : lss_amd_detail::TIME_INDEPENDENT;
+        if (m.coeff_mode == LSS_COEFF_FIELDS) {
+            lss_amd_detail::sample_fields(u, g, s, cmode, lss_amd_detail::TIME_INDEPENDENT);
+        } else {
+            // grid, initial data and boundaries as for fields; coefficients as separable factors
+            lss_amd_detail::sample_grid(g, s);
+            s.init.resize(g.nx * g.ny);
+            for (size_t i = 0; i < g.nx; ++i)
+                for (size_t j = 0; j < g.ny; ++j)
+                    s.init[i * g.ny + j] = u.init(s.xs[i], s.ys[j]);
+            s.bnd_sets = 1;
+            s.bnd.assign(g.nx + 2 * g.ny, (T)0);
+            lss_amd_detail::sample_bnd_set(u, g, s, s.times[0], s.bnd.data());
+            const size_t stride = 12 * (g.nx + g.ny);
+            s.coeff_sets = tdep ? s.steps : 1;
+            s.coeff.assign(stride * s.coeff_sets, (T)0);
+            for (size_t k = 0; k < s.coeff_sets; ++k)
+                separable_set(m, g, s, s.times[k], s.coeff.data() + k * stride);
+        }
+        ms.d[q] = s.desc(m.coeff_mode);
+    }
+    lss_adi_plan *plan = make_plan(sizeof(T) == 8 ? LSS_FP64 : LSS_FP32, ms.d.data(), n);
+    plan->init_host.resize(n);
+    for (int64_t q = 0; q < n; ++q) {
+        const auto &v = ms.s[q].init;
+        plan->init_host[q].resize(v.size() * sizeof(T));
+        std::memcpy(plan->init_host[q].data(), v.data(), v.size() * sizeof(T));
+    }
+    return plan;
+}
+
+// grow-only per-device workspace of lss_tridiag_batched
+std::mutex g_ws_mutex;
+std::vector<std::pair<void *, size_t>> g_ws;
+
+void *workspace(size_t bytes)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_ws_mutex);
+    if ((int)g_ws.size() <= dev)
+        g_ws.resize(dev + 1, {nullptr, 0});
+    auto &w = g_ws[dev];
+    if (w.second < bytes) {
+        if (w.first) {
+            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipFree(w.first));
+            w.first = nullptr;
+            w.second = 0;
+        }
+        hipError_t e = hipMalloc(&w.first, bytes);
+        if (e != hipSuccess)
+            throw lss_error(LSS_ERR_ALLOC, std::string("workspace hipMalloc failed: ") + hipGetErrorString(e));
+        w.second = bytes;
+    }
+    return w.first;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lss_last_error(void) { return g_last_error.c_str(); }
+
+int32_t lss_version(void) { return 1; }
+
+int lss_adi_plan_create(const lss_adi_desc *descs, int64_t count, lss_adi_plan **plan)
+{
+    return guarded([&] {
+        require(descs && plan && count >= 1, "null descriptor/plan pointer or count < 1");
+        *plan = nullptr;
+        *plan = make_plan(descs[0].fp, descs, count);
+    });
+}
+
+int lss_adi_plan_destroy(lss_adi_plan *plan)
+{
+    return guarded([&] { delete plan; });
+}
+
+int lss_adi_run_dev(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, void *stream, int32_t use_graph)
+{
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        plan->run(u_in, u_out, (hipStream_t)stream, use_graph != 0, false);
+    });
+}
+
+int lss_adi_run_profiled(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, void *stream, double *ms,
+                         double *bytes, int64_t *launches)
+{
+    return guarded([&] {
+        require(plan && u_in && u_out && ms && bytes && launches, "null pointer");
+        plan->profile(u_in, u_out, (hipStream_t)stream, ms, bytes, launches);
+    });
+}
+
+int lss_adi_run_host(lss_adi_plan *plan, const void *const *u_in, void *const *u_out)
+{
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        plan->run(u_in, u_out, nullptr, false, true);
+    });
+}
+
+int lss_model_plan_create(const lss_model *models, int64_t count, int32_t fp, lss_adi_plan **plan)
+{
+    return guarded([&] {
+        require(models && plan && count >= 1, "null model/plan pointer or count < 1");
+        *plan = nullptr;
+        if (fp == LSS_FP64)
+            *plan = model_plan<double>(models, count);
+        else if (fp == LSS_FP32)
+            *plan = model_plan<float>(models, count);
+        else
+            throw lss_error(LSS_ERR_INVALID, "fp must be LSS_FP64 or LSS_FP32");
+    });
+}
+
+int lss_model_initial(const lss_adi_plan *plan, int64_t p, void *u0_host)
+{
+    return guarded([&] {
+        require(plan && u0_host, "null pointer");
+        require(p >= 0 && p < plan->count, "problem index out of range");
+        require((int64_t)plan->init_host.size() == plan->count, "plan was not created from models");
+        std::memcpy(u0_host, plan->init_host[p].data(), plan->init_host[p].size());
+    });
+}
+
+int lss_plan_problem_shape(const lss_adi_plan *plan, int64_t p, int64_t *nx, int64_t *ny, int64_t *steps)
+{
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        require(p >= 0 && p < plan->count, "problem index out of range");
+        if (nx)
+            *nx = plan->nx[p];
+        if (ny)
+            *ny = plan->ny[p];
+        if (steps)
+            *steps = plan->steps[p];
+    });
+}
+
+int lss_tridiag_batched(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b,
+                        const void *c, const void *d, void *x, void *stream)
+{
+    return guarded([&] {
+        require(fp == LSS_FP64 || fp == LSS_FP32, "fp must be LSS_FP64 or LSS_FP32");
+        require(layout == LSS_LAYOUT_CONTIGUOUS || layout == LSS_LAYOUT_STRIDED, "unknown layout");
+        require(B >= 0 && n >= 4 && n < (1 << 30), "need B >= 0 and n >= 4");
+        require(a && b && c && d && x, "null array");
+        if (B == 0)
+            return;
+        const size_t es = fp == LSS_FP64 ? 8 : 4;
+        void *scr = workspace((size_t)B * (size_t)n * es);
+        const unsigned blocks = (unsigned)((B + 255) / 256);
+        hipStream_t st = (hipStream_t)stream;
+        if (fp == LSS_FP64)
+            hipLaunchKernelGGL(k_tridiag_lane<double>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
+                               (const double *)a, (const double *)b, (const double *)c, (const double *)d,
+                               (double *)x, (double *)scr);
+        else
+            hipLaunchKernelGGL(k_tridiag_lane<float>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
+                               (const float *)a, (const float *)b, (const float *)c, (const float *)d,
+                               (float *)x, (float *)scr);
+        HIPCHK(hipGetLastError());
+    });
+}
+
+}  // extern "C"

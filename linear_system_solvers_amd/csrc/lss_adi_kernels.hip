@@ -1,0 +1,455 @@
+// lss_adi_kernels.hip -- HIP kernels of the ADI time step for gfx950 (MI355X).
+//
+// One time step of every scheme is the same program of launches (the reference's loop body,
+// pde_solvers/two_dimensional/heat_type/lss_2d_general_heston_equation_implicit_kernel.hpp:177-184):
+//   bnd_a   boundary stage A: v = 0 column (explicit degenerate row) and v = v_max column
+//           (lss_2d_general_heston_equation_explicit_boundary.hpp:181-200, :39-84)
+//   xsweep  stage 1: S-lines j = 1..ny-2, rhs_intermed_1 of u, implicit solve along S  -> Y1
+//   vsweep  stage 1: v-lines i = 1..nx-2, v-direction rhs, implicit solve along v     -> Y2 (DR: next)
+//   hv_y3   HV only: full-grid explicit stage                                         -> Y3
+//   xsweep  stage 2: CS/MCS rhs_intermed_3 -> Y3 ; HV rhs_intermed_4 -> Y4 (stored in Y1's buffer)
+//   vsweep  stage 2: v-direction rhs of (u, Y3) or (Y2, Y4)                          -> next
+//   bnd_b   boundary stage B: S_min row and S_max Neumann extrapolation (explicit_boundary.hpp:202-238)
+// Kernels are compiled with -ffp-contract=off and keep the reference's operand order, so with
+// FIELDS coefficients every value is rounded exactly as the reference's host code rounds it.
+//
+// Line solves (v1): one lane per line, the forward sweep keeps (gamma_t, sol_t) in HBM
+// (sol in the output array, gamma in a scratch array), the backward sweep overwrites the output.
+// Lanes of a wavefront hold adjacent S-lines, so S-line sweeps read and write whole rows.
+
+#include "lss_adi_device.hpp"
+
+namespace lssamd {
+
+template <typename T> struct Stencil {
+    T M, Mt, P, Pt, Z, W, C;
+};
+
+template <typename T>
+__device__ __forceinline__ T sep(const T *X, const T *Y, int nx, int ny, int k, int i, int j)
+{
+    return X[(2 * k) * nx + i] * Y[(2 * k) * ny + j] + X[(2 * k + 1) * nx + i] * Y[(2 * k + 1) * ny + j];
+}
+
+// Stencil coefficients at (zeta_i, eta_j): general_heston_equation_coefficients (coefficients.hpp:89-95).
+template <typename T>
+__device__ __forceinline__ Stencil<T> stencil_at(const Prob<T> &p, const T *__restrict__ cs, int i, int j)
+{
+    Stencil<T> s;
+    if (p.coeff_mode == COEFF_FIELDS) {
+        const int64_t nxy = (int64_t)p.nx * p.ny, q = (int64_t)i * p.ny + j;
+        s.M = cs[q];
+        s.Mt = cs[nxy + q];
+        s.P = cs[2 * nxy + q];
+        s.Pt = cs[3 * nxy + q];
+        s.Z = cs[4 * nxy + q];
+        s.W = cs[5 * nxy + q];
+        s.C = cs[6 * nxy + q];
+    } else {
+        const T *X = cs, *Y = cs + 12 * (int64_t)p.nx;
+        const T a = sep(X, Y, p.nx, p.ny, 0, i, j), b = sep(X, Y, p.nx, p.ny, 1, i, j);
+        const T c = sep(X, Y, p.nx, p.ny, 2, i, j), d = sep(X, Y, p.nx, p.ny, 3, i, j);
+        const T e = sep(X, Y, p.nx, p.ny, 4, i, j), f = sep(X, Y, p.nx, p.ny, 5, i, j);
+        const T two = (T)2.0, half = (T)0.5;
+        s.M = (p.alpha * a - p.delta * d);
+        s.Mt = (p.beta * b - p.ni * e);
+        s.P = (p.alpha * a + p.delta * d);
+        s.Pt = (p.beta * b + p.ni * e);
+        s.Z = (two * p.alpha * a - half * p.rho * f);
+        s.W = (two * p.beta * b - half * p.rho * f);
+        s.C = c;
+    }
+    return s;
+}
+
+// D, E, F on the v = 0 row at zeta_i (used by boundary stage A).
+template <typename T>
+__device__ __forceinline__ void def_row0(const Prob<T> &p, const T *__restrict__ cs, int i, T &D, T &E, T &F)
+{
+    if (p.coeff_mode == COEFF_FIELDS) {
+        const T *r = cs + 7 * (int64_t)p.nx * p.ny;
+        D = r[i];
+        E = r[p.nx + i];
+        F = r[2 * p.nx + i];
+    } else {
+        const T *X = cs, *Y = cs + 12 * (int64_t)p.nx;
+        D = sep(X, Y, p.nx, p.ny, 3, i, 0);
+        E = sep(X, Y, p.nx, p.ny, 4, i, 0);
+        F = sep(X, Y, p.nx, p.ny, 5, i, 0);
+    }
+}
+
+template <typename T> __device__ __forceinline__ const T *coeff_set(const Prob<T> &p, int step)
+{
+    return p.coeff + (p.coeff_sets > 1 ? (int64_t)step : 0) * p.coeff_stride;
+}
+template <typename T> __device__ __forceinline__ const T *bnd_set(const Prob<T> &p, int step)
+{
+    return p.bnd + (p.bnd_sets > 1 ? (int64_t)step : 0) * p.bnd_stride;
+}
+
+// -------------------------------------------------------------------------------------------------
+// Boundary stage A: colA0 = new v = 0 column, colA1 = v_max column (explicit_boundary.hpp:39-84, :181-200)
+// -------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(TILE) k_bnd_a(const Prob<T> *__restrict__ probs, const Tile *__restrict__ tiles,
+                                                int step)
+{
+    const Tile tl = tiles[blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    if (step >= p.steps || (int)threadIdx.x >= tl.count)
+        return;
+    const int i = tl.first + threadIdx.x;
+    const int nx = p.nx, ny = p.ny, N = nx - 1;
+    const T *u = (step & 1) ? p.b : p.a;
+    const T *cs = coeff_set(p, step);
+    const T *bs = bnd_set(p, step);
+    const T four = (T)4.0, three = (T)3.0, two = (T)2.0, one = (T)1.0;
+    const T delta = p.delta, ni = p.ni, rho = p.rho;
+    T v;
+    if (i == 0) {
+        v = bs[nx];  // h_lower(t, Y(0))
+    } else {
+        T D, E, F;
+        def_row0(p, cs, i, D, E, F);
+        const T *ui = u + (int64_t)i * ny;
+        if (i == N) {
+            const T delta_ = two * p.h1 * bs[nx + ny];  // 2 h_1 * h_upper(t, Y(0)) X'(1)
+            v = ((one - three * ni * E + rho * F) * ui[0]) + (four * ni * E * ui[1]) - (ni * E * ui[2]) -
+                (delta * delta_ * D);
+        } else {
+            v = (-delta * D * ui[-ny]) + ((one - three * ni * E + rho * F) * ui[0]) + (delta * D * ui[ny]) -
+                (ni * E * ui[2]) + (four * ni * E * ui[1]);
+        }
+    }
+    p.colA0[i] = v;
+    p.colA1[i] = bs[i];  // v_upper(t, X(zeta_i))
+}
+
+// -------------------------------------------------------------------------------------------------
+// Boundary stage B: row 0 Dirichlet, row nx-1 Neumann extrapolation through idx2 (explicit_boundary.hpp:202-238)
+// -------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(TILE) k_bnd_b(const Prob<T> *__restrict__ probs, const Tile *__restrict__ tiles,
+                                                int step)
+{
+    const Tile tl = tiles[blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    if (step >= p.steps || (int)threadIdx.x >= tl.count)
+        return;
+    const int j = tl.first + threadIdx.x;
+    const int nx = p.nx, ny = p.ny;
+    T *nxt = (step & 1) ? p.a : p.b;
+    const T *bs = bnd_set(p, step);
+    const T two = (T)2.0, three = (T)3.0, four = (T)4.0;
+    const uint32_t jj = p.idx2[j];
+    const T e = (((four * nxt[(int64_t)(nx - 2) * ny + jj]) - nxt[(int64_t)(nx - 3) * ny + jj] -
+                  (two * p.h1 * bs[nx + ny + j])) /
+                 three);
+    nxt[j] = bs[nx + j];
+    nxt[(int64_t)(nx - 1) * ny + j] = e;
+}
+
+// -------------------------------------------------------------------------------------------------
+// Right-hand sides (operand order of the reference's rhs functions)
+// -------------------------------------------------------------------------------------------------
+// rhs_intermed_1 (lss_heat_douglas_rachford_method.hpp:37-68); `in` points at (t, j), row stride ny
+template <typename T>
+__device__ __forceinline__ T rhs_intermed_1(const Prob<T> &p, const Stencil<T> &s, const T *in, int ny)
+{
+    const T one = (T)1.0, gamma = p.gamma, theta = p.theta;
+    return (gamma * s.C * in[-ny - 1]) + ((one - theta) * s.M * in[-ny]) - (gamma * s.C * in[-ny + 1]) +
+           (s.Mt * in[-1]) + ((one - s.W - (one - theta) * s.Z) * in[0]) + (s.Pt * in[1]) -
+           (gamma * s.C * in[ny - 1]) + ((one - theta) * s.P * in[ny]) + (gamma * s.C * in[ny + 1]);
+}
+
+// CS rhs_intermed_3 (lss_heat_craig_sneyd_method.hpp:128-156), MCS adds the (zeta - theta) term
+// of lss_heat_modified_craig_sneyd_method.hpp:158-163 (with its "- P")
+template <typename T, bool MCS>
+__device__ __forceinline__ T rhs_intermed_3(const Prob<T> &p, const Stencil<T> &s, const T *in, const T *inh,
+                                            const T *inhn, int ny)
+{
+    const T gamma = p.gamma, theta = p.theta, zeta = p.zeta;
+    T v = (-theta * s.M * in[-ny]) + (theta * s.Z * in[0]) - (theta * s.P * in[ny]) + (inh[0]) +
+          (zeta * gamma * s.C *
+           ((inhn[ny + 1] - in[ny + 1]) - (inhn[ny - 1] - in[ny - 1]) - (inhn[-ny + 1] - in[-ny + 1]) +
+            (inhn[-ny - 1] - in[-ny - 1])));
+    if (MCS) {
+        v = v + ((zeta - theta) * (s.M * (inhn[-ny] - in[-ny]) - (s.Z + s.W) * (inhn[0] - in[0]) -
+                                   s.P * (inhn[ny] - in[ny]) + s.Mt * (inhn[-1] - in[-1]) +
+                                   s.Pt * (inhn[1] - in[1])));
+    }
+    return v;
+}
+
+// HV rhs_intermed_4 (lss_heat_hundsdorfer_verwer_method.hpp:174-194)
+template <typename T>
+__device__ __forceinline__ T rhs_intermed_4(const Prob<T> &p, const Stencil<T> &s, const T *in, const T *inh, int ny)
+{
+    const T theta = p.theta;
+    return (-theta * s.M * in[-ny]) + (theta * s.Z * in[0]) - (theta * s.P * in[ny]) + inh[0];
+}
+
+// v-direction rhs shared by every scheme (e.g. lss_heat_douglas_rachford_method.hpp:106-126)
+template <typename T>
+__device__ __forceinline__ T rhs_v(const Prob<T> &p, const Stencil<T> &s, const T *in, const T *inh)
+{
+    const T theta = p.theta;
+    return (-theta * s.Mt * in[-1]) + (theta * s.W * in[0]) - (theta * s.Pt * in[1]) + inh[0];
+}
+
+// -------------------------------------------------------------------------------------------------
+// Line solve step of the forward sweep.  Thomas LU keeps (gamma_t, sol_t); double sweep keeps
+// (L_t, K_t).  first/last fold the Dirichlet ends exactly as thomas_lu_boundary.hpp:127-135,
+// 176-183 and double_sweep_boundary.hpp:100-106 do.
+// -------------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void fwd_step(int tri, bool first, bool last, T low, T diag, T high, T r, T lo, T hi,
+                                         T gprev, T sprev, T &g, T &s)
+{
+    if (tri == TRI_THOMAS) {
+        if (first) {
+            const T beta = diag;
+            g = high / beta;
+            const T rr = r - low * lo;
+            s = rr / beta;
+        } else if (!last) {
+            const T beta = diag - (low * gprev);
+            g = high / beta;
+            s = (r - (low * sprev)) / beta;
+        } else {
+            const T beta = diag - (low * gprev);
+            const T rn = r - high * hi;
+            s = (rn - (low * sprev)) / beta;
+            g = (T)0;
+        }
+    } else {
+        const T mone = (T)-1.0;
+        const T Lp = first ? (T)0 : gprev;
+        const T Kp = first ? lo : sprev;
+        const T tmp = diag + (low * Lp);
+        g = mone * high / tmp;
+        s = (r - (low * Kp)) / tmp;
+    }
+}
+
+template <typename T> __device__ __forceinline__ T bwd_step(int tri, T g, T s, T xnext)
+{
+    return tri == TRI_THOMAS ? (s - (g * xnext)) : ((g * xnext) + s);
+}
+
+// -------------------------------------------------------------------------------------------------
+// S-line sweep (one lane per line j).  stage 1: rhs_intermed_1(u) -> Y1.
+// stage 2: CS/MCS rhs_intermed_3(u, Y1, Y2) -> Y3; HV rhs_intermed_4(Y2, Y3) -> Y4 (in Y1's buffer).
+// Dirichlet ends: u(0, idx2[j]), u(nx-1, idx2[j]) (implicit_kernel.hpp:86-101, old time level).
+// -------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(TILE) k_xsweep(const Prob<T> *__restrict__ probs, const Tile *__restrict__ tiles,
+                                                 int step, int stage)
+{
+    const Tile tl = tiles[blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    if (step >= p.steps || (stage == 2 && p.scheme == SCHEME_DR) || (int)threadIdx.x >= tl.count)
+        return;
+    const int j = tl.first + threadIdx.x;
+    const int nx = p.nx, ny = p.ny, end = nx - 2, tri = p.tri;
+    const T *u = (step & 1) ? p.b : p.a;
+    const T *cs = coeff_set(p, step);
+    const T one = (T)1.0, theta = p.theta;
+    const int kind = (stage == 1) ? 0 : (p.scheme == SCHEME_CS ? 1 : (p.scheme == SCHEME_MCS ? 2 : 3));
+    const T *in = (kind == 3) ? p.y2 : u;
+    const T *inh = (kind == 3) ? p.y3 : p.y1;
+    const T *inhn = p.y2;
+    T *out = (kind == 0) ? p.y1 : (kind == 3 ? p.y1 : p.y3);
+    T *scr = p.scr;
+    const uint32_t jj = p.idx2[j];
+    const T lo = u[jj], hi = u[(int64_t)(nx - 1) * ny + jj];
+    T gprev = (T)0, sprev = (T)0;
+    for (int t = 1; t <= end; ++t) {
+        const int64_t q = (int64_t)t * ny + j;
+        const Stencil<T> s = stencil_at(p, cs, t, j);
+        const T low = (-theta * s.M), diag = (one + theta * s.Z), high = (-theta * s.P);
+        T r;
+        if (kind == 0)
+            r = rhs_intermed_1(p, s, in + q, ny);
+        else if (kind == 1)
+            r = rhs_intermed_3<T, false>(p, s, in + q, inh + q, inhn + q, ny);
+        else if (kind == 2)
+            r = rhs_intermed_3<T, true>(p, s, in + q, inh + q, inhn + q, ny);
+        else
+            r = rhs_intermed_4(p, s, in + q, inh + q, ny);
+        T g, sv;
+        fwd_step(tri, t == 1, t == end, low, diag, high, r, lo, hi, gprev, sprev, g, sv);
+        scr[q] = g;
+        out[q] = sv;
+        gprev = g;
+        sprev = sv;
+    }
+    T x = (tri == TRI_THOMAS) ? sprev : bwd_step(tri, gprev, sprev, hi);
+    out[(int64_t)end * ny + j] = x;
+    for (int t = end - 1; t >= 1; --t) {
+        const int64_t q = (int64_t)t * ny + j;
+        x = bwd_step(tri, scr[q], out[q], x);
+        out[q] = x;
+    }
+    out[j] = lo;
+    out[(int64_t)(nx - 1) * ny + j] = hi;
+}
+
+// -------------------------------------------------------------------------------------------------
+// v-line sweep (one lane per line i).  stage 1: rhs_v(u, Y1) -> Y2 (DR: next).
+// stage 2: CS/MCS rhs_v(u, Y3) -> next; HV rhs_v(Y2, Y4) -> next.
+// Dirichlet ends colA0/colA1[idx1[i]]: the copy of `next` the vertical lambdas captured after
+// boundary stage A (implicit_kernel.hpp:69-84).
+// -------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(TILE) k_vsweep(const Prob<T> *__restrict__ probs, const Tile *__restrict__ tiles,
+                                                 int step, int stage)
+{
+    const Tile tl = tiles[blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    if (step >= p.steps || (stage == 2 && p.scheme == SCHEME_DR) || (int)threadIdx.x >= tl.count)
+        return;
+    const int i = tl.first + threadIdx.x;
+    const int ny = p.ny, end = ny - 2, tri = p.tri;
+    const T *u = (step & 1) ? p.b : p.a;
+    T *nxt = (step & 1) ? p.a : p.b;
+    const T *cs = coeff_set(p, step);
+    const T one = (T)1.0, theta = p.theta;
+    const T *in, *inh;
+    T *out;
+    if (stage == 1) {
+        in = u;
+        inh = p.y1;
+        out = (p.scheme == SCHEME_DR) ? nxt : p.y2;
+    } else if (p.scheme == SCHEME_HV) {
+        in = p.y2;
+        inh = p.y1;
+        out = nxt;
+    } else {
+        in = u;
+        inh = p.y3;
+        out = nxt;
+    }
+    const int64_t row = (int64_t)i * ny;
+    const uint32_t ii = p.idx1[i];
+    const T lo = p.colA0[ii], hi = p.colA1[ii];
+    T *scr = p.scr + row;
+    in += row;
+    inh += row;
+    out += row;
+    T gprev = (T)0, sprev = (T)0;
+    for (int t = 1; t <= end; ++t) {
+        const Stencil<T> s = stencil_at(p, cs, i, t);
+        const T low = (-theta * s.Mt), diag = (one + theta * s.W), high = (-theta * s.Pt);
+        const T r = rhs_v(p, s, in + t, inh + t);
+        T g, sv;
+        fwd_step(tri, t == 1, t == end, low, diag, high, r, lo, hi, gprev, sprev, g, sv);
+        scr[t] = g;
+        out[t] = sv;
+        gprev = g;
+        sprev = sv;
+    }
+    T x = (tri == TRI_THOMAS) ? sprev : bwd_step(tri, gprev, sprev, hi);
+    out[end] = x;
+    for (int t = end - 1; t >= 1; --t) {
+        x = bwd_step(tri, scr[t], out[t], x);
+        out[t] = x;
+    }
+    out[0] = lo;
+    out[ny - 1] = hi;
+}
+
+// -------------------------------------------------------------------------------------------------
+// HV full-grid explicit stage Y3 (lss_heat_hundsdorfer_verwer_method.hpp:128-172), interior only.
+// One workgroup per row r; lanes over columns.
+// -------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(TILE) k_hv_y3(const Prob<T> *__restrict__ probs, const Tile *__restrict__ tiles,
+                                                int step)
+{
+    const Tile tl = tiles[blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    if (step >= p.steps || p.scheme != SCHEME_HV)
+        return;
+    const int r = tl.first;
+    const int ny = p.ny;
+    const T *u = (step & 1) ? p.b : p.a;
+    const T *cs = coeff_set(p, step);
+    const T one = (T)1.0, gamma = p.gamma, theta = p.theta, zeta = p.zeta;
+    for (int c = 1 + threadIdx.x; c < ny - 1; c += blockDim.x) {
+        const int64_t q = (int64_t)r * ny + c;
+        const Stencil<T> s = stencil_at(p, cs, r, c);
+        const T *in = u + q, *inh = p.y1 + q, *inhn = p.y2 + q;
+        const T val =
+            (-theta * s.M * inh[-ny]) + ((one + theta * s.Z) * inh[0]) - (theta * s.P * inh[ny]) +
+            (theta * s.M * in[-ny]) - (theta * s.Z * in[0]) + (theta * s.P * in[ny]) +
+            (zeta * gamma * s.C *
+             ((inhn[ny + 1] - in[ny + 1]) - (inhn[ny - 1] - in[ny - 1]) - (inhn[-ny + 1] - in[-ny + 1]) +
+              (inhn[-ny - 1] - in[-ny - 1]))) +
+            (zeta * (s.M * (inhn[-ny] - in[-ny]) - (s.W + s.Z) * (inhn[0] - in[0]) + s.P * (inhn[ny] - in[ny]) +
+                     s.Mt * (inhn[-1] - in[-1]) + s.Pt * (inhn[1] - in[1])));
+        p.y3[q] = val;
+    }
+}
+
+// -------------------------------------------------------------------------------------------------
+// Batched plain tridiagonal systems (one lane per system).  a[0] and c[n-1] are ignored, which is
+// the reference's Thomas LU with Neumann(0)/Neumann(0) ends (thomas_lu_boundary.hpp:136-144, 184-191).
+// Layout: contiguous x[k*n + i] (strided == 0) or interleaved x[i*B + k] (strided == 1).
+// -------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_tridiag_lane(int64_t B, int n, int strided, const T *__restrict__ a,
+                                                      const T *__restrict__ b, const T *__restrict__ c,
+                                                      const T *__restrict__ d, T *__restrict__ x, T *__restrict__ scr)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= B)
+        return;
+    const int64_t base = strided ? k : k * n, st = strided ? B : 1;
+    // t = 0: beta = b0, gamma = (0 + c0) / beta, r = d0 - 0 * 0
+    T beta = b[base];
+    T g = c[base] / beta;
+    T s = d[base] / beta;
+    scr[base] = g;
+    x[base] = s;
+    for (int t = 1; t < n - 1; ++t) {
+        const int64_t q = base + t * st;
+        const T at = a[q];
+        beta = b[q] - (at * g);
+        g = c[q] / beta;
+        s = (d[q] - (at * s)) / beta;
+        scr[q] = g;
+        x[q] = s;
+    }
+    {
+        const int64_t q = base + (int64_t)(n - 1) * st;
+        const T at = a[q];
+        beta = b[q] - (at * g);
+        s = (d[q] - (at * s)) / beta;
+        x[q] = s;
+    }
+    T xv = s;
+    for (int t = n - 2; t >= 0; --t) {
+        const int64_t q = base + t * st;
+        xv = x[q] - (scr[q] * xv);
+        x[q] = xv;
+    }
+}
+
+// -------------------------------------------------------------------------------------------------
+// explicit instantiations
+// -------------------------------------------------------------------------------------------------
+#define LSS_INST(T)                                                                                           \
+    template __global__ void k_bnd_a<T>(const Prob<T> *, const Tile *, int);                                  \
+    template __global__ void k_bnd_b<T>(const Prob<T> *, const Tile *, int);                                  \
+    template __global__ void k_xsweep<T>(const Prob<T> *, const Tile *, int, int);                            \
+    template __global__ void k_vsweep<T>(const Prob<T> *, const Tile *, int, int);                            \
+    template __global__ void k_hv_y3<T>(const Prob<T> *, const Tile *, int);                                  \
+    template __global__ void k_tridiag_lane<T>(int64_t, int, int, const T *, const T *, const T *, const T *, \
+                                               T *, T *);
+LSS_INST(double)
+LSS_INST(float)
+#undef LSS_INST
+
+}  // namespace lssamd
