@@ -88,7 +88,7 @@ template <typename T> struct sampled {
     std::size_t bnd_sets = 0;
     std::vector<T> init;    // nx*ny initial surface
 
-    lss_adi_desc desc(int coeff_mode) const
+    lss_adi_desc desc(int coeff_mode, int line_solver = LSS_LINE_EXACT) const
     {
         lss_adi_desc d;
         std::memset(&d, 0, sizeof(d));
@@ -96,6 +96,7 @@ template <typename T> struct sampled {
         d.splitting = splitting;
         d.tridiagonal = tridiagonal;
         d.coeff_mode = coeff_mode;
+        d.line_solver = line_solver;
         d.nx = (int64_t)nx;
         d.ny = (int64_t)ny;
         d.steps = (int64_t)steps;

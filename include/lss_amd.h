@@ -42,6 +42,11 @@ enum {
     LSS_COEFF_FIELDS = 0,    /* per point M, M~, P, P~, Z, W, C + D, E, F on the v = 0 row        */
     LSS_COEFF_SEPARABLE = 1  /* q(i,j) = X0q[i] Y0q[j] + X1q[i] Y1q[j] for q = a, b, c, d, e, f     */
 };
+/* line solver of the implicit sweeps */
+enum {
+    LSS_LINE_EXACT = 0,       /* one lane per line in the reference's Thomas / double-sweep order   */
+    LSS_LINE_PARTITIONED = 1  /* register-resident chunks + cyclic reduction of the chunk interfaces */
+};
 /* batched tridiagonal layouts */
 enum { LSS_LAYOUT_CONTIGUOUS = 0, LSS_LAYOUT_STRIDED = 1 };
 
@@ -57,6 +62,8 @@ enum { LSS_LAYOUT_CONTIGUOUS = 0, LSS_LAYOUT_STRIDED = 1 };
  *              h_upper(t, Y(eta_j)) * X'(1) (lss_heston_boundary_transform.hpp:48-62).
  *   idx1/idx2  index_of maps (lss_grid_config.hpp:110-118).
  *   theta..h1  scheme scalars, already rounded to fp (implicit_kernel.hpp:978-995, coefficients.hpp:62-73).
+ * FIELDS with LSS_LINE_EXACT reproduces the reference's host result bit for bit.  The partitioned
+ * solver needs (nx - 2) and (ny - 2) <= 64 * 16.
  * All pointers are host pointers; the plan copies them to device memory.
  */
 typedef struct lss_adi_desc {
@@ -64,6 +71,8 @@ typedef struct lss_adi_desc {
     int32_t splitting;
     int32_t tridiagonal;     /* LSS_THOMAS_LU or LSS_DOUBLE_SWEEP operation order; LSS_CUDA_SOLVER = Thomas */
     int32_t coeff_mode;
+    int32_t line_solver;     /* LSS_LINE_EXACT | LSS_LINE_PARTITIONED; equal for all problems of a plan */
+    int32_t pad;
     int64_t nx, ny, steps;
     double theta, weighting;
     double alpha, beta, gamma, delta, ni, rho, h1;
@@ -123,6 +132,7 @@ typedef struct lss_model {
     int32_t tridiagonal;
     int32_t direction;
     int32_t coeff_mode;        /* LSS_COEFF_FIELDS (reference rounding) or LSS_COEFF_SEPARABLE */
+    int32_t line_solver;       /* LSS_LINE_EXACT | LSS_LINE_PARTITIONED */
 } lss_model;
 
 int lss_model_plan_create(const lss_model *models, int64_t count, int32_t fp, lss_adi_plan **plan);

@@ -27,6 +27,7 @@ CUDA_SOLVER, DOUBLE_SWEEP, SOR_SOLVER, THOMAS_LU = 0, 1, 2, 3
 FORWARD, BACKWARD = 0, 1
 UNIFORM, NONUNIFORM = 0, 1
 COEFF_FIELDS, COEFF_SEPARABLE = 0, 1
+LINE_EXACT, LINE_PARTITIONED = 0, 1
 LAYOUT_CONTIGUOUS, LAYOUT_STRIDED = 0, 1
 MODEL_HESTON, MODEL_SABR = 0, 1
 
@@ -56,12 +57,12 @@ class _Model(C.Structure):
                 ("accumulation_point", C.c_double), ("alpha_scale", C.c_double), ("beta_scale", C.c_double),
                 ("grid", C.c_int32), ("splitting", C.c_int32), ("weighting", C.c_double),
                 ("implicit_scheme", C.c_int32), ("tridiagonal", C.c_int32), ("direction", C.c_int32),
-                ("coeff_mode", C.c_int32)]
+                ("coeff_mode", C.c_int32), ("line_solver", C.c_int32)]
 
 
 class _Desc(C.Structure):
     _fields_ = [("fp", C.c_int32), ("splitting", C.c_int32), ("tridiagonal", C.c_int32), ("coeff_mode", C.c_int32),
-                ("nx", C.c_int64), ("ny", C.c_int64), ("steps", C.c_int64),
+                ("line_solver", C.c_int32), ("pad", C.c_int32), ("nx", C.c_int64), ("ny", C.c_int64), ("steps", C.c_int64),
                 *[(n, C.c_double) for n in ("theta", "weighting", "alpha", "beta", "gamma", "delta", "ni", "rho",
                                             "h1")],
                 ("idx1", C.c_void_p), ("idx2", C.c_void_p), ("coeff_sets", C.c_int64), ("coeff", C.c_void_p),
@@ -101,6 +102,7 @@ class Model:
     tridiagonal: int = THOMAS_LU
     direction: int = BACKWARD
     coeff_mode: int = COEFF_FIELDS
+    line_solver: int = LINE_EXACT
 
     def c(self) -> _Model:
         m = _Model()

@@ -32,6 +32,9 @@ template <typename T> __global__ void k_vsweep(const Prob<T> *, const Tile *, in
 template <typename T> __global__ void k_hv_y3(const Prob<T> *, const Tile *, int);
 template <typename T>
 __global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *);
+template <typename T, int M> __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int, int);
+template <typename T, int M> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int);
+constexpr int P_NCH = 64, P_XTJ = 16, P_VROWS = 4;
 
 }  // namespace lssamd
 
@@ -158,6 +161,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     size_t tile_off[TB_N] = {};
     std::vector<int> prefix[TB_N];     // prefix[tb][q] = tiles of internal problems [0, q)
     int max_steps = 0;
+    int line_solver = LINE_EXACT;
+    int pm = 1;                        // partitioned: points per chunk bound (template M)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -183,6 +188,10 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                     d.tridiagonal == LSS_CUDA_SOLVER,
                 "tridiagonal method must be ThomasLU, DoubleSweep or CUDASolver");
         require(d.coeff_mode == LSS_COEFF_FIELDS || d.coeff_mode == LSS_COEFF_SEPARABLE, "unknown coeff_mode");
+        require(d.line_solver == LSS_LINE_EXACT || d.line_solver == LSS_LINE_PARTITIONED, "unknown line_solver");
+        if (d.line_solver == LSS_LINE_PARTITIONED)
+            require(d.nx - 2 <= P_NCH * 16 && d.ny - 2 <= P_NCH * 16,
+                    "partitioned line solver needs nx - 2 and ny - 2 <= 1024");
         require(d.coeff_sets == 1 || d.coeff_sets == d.steps, "coeff_sets must be 1 or steps");
         require(d.bnd_sets == 1 || d.bnd_sets == d.steps, "bnd_sets must be 1 or steps");
         require(d.idx1 && d.idx2 && d.coeff && d.bnd, "null array in descriptor");
@@ -204,6 +213,16 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         for (int64_t p = 0; p < n; ++p) {
             require(descs[p].fp == (sizeof(T) == 8 ? LSS_FP64 : LSS_FP32), "all problems of a plan share fp");
             validate(descs[p]);
+            require(descs[p].line_solver == descs[0].line_solver, "all problems of a plan share line_solver");
+        }
+        line_solver = descs[0].line_solver == LSS_LINE_PARTITIONED ? LINE_PARTITIONED : LINE_EXACT;
+        if (line_solver == LINE_PARTITIONED) {
+            int64_t mmax = 1;
+            for (int64_t q = 0; q < n; ++q)
+                mmax = std::max({mmax, (descs[q].nx - 2 + P_NCH - 1) / P_NCH, (descs[q].ny - 2 + P_NCH - 1) / P_NCH});
+            pm = 1;
+            while (pm < mmax)
+                pm *= 2;
         }
         order.resize(n);
         std::iota(order.begin(), order.end(), 0);
@@ -276,6 +295,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             p.y1 = t + L.off_y1;
             p.y2 = d.splitting == LSS_DOUGLAS_RACHFORD ? nullptr : t + L.off_y2;
             p.y3 = d.splitting == LSS_DOUGLAS_RACHFORD ? nullptr : t + L.off_y3;
+            p.y4 = line_solver == LINE_PARTITIONED ? p.y3 : p.y1;
             p.scr = t + L.off_scr;
             p.colA0 = t + L.off_colA0;
             p.colA1 = t + L.off_colA1;
@@ -307,8 +327,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 switch (tb) {
                 case TB_I: lo = 0, hi = p.nx; break;
                 case TB_J: lo = 0, hi = p.ny; break;
-                case TB_X: lo = 1, hi = p.ny - 1; break;
-                case TB_V: lo = 1, hi = p.nx - 1; break;
+                case TB_X: lo = 1, hi = p.ny - 1, step = line_solver == LINE_PARTITIONED ? P_XTJ : TILE; break;
+                case TB_V: lo = 1, hi = p.nx - 1, step = line_solver == LINE_PARTITIONED ? P_VROWS : TILE; break;
                 default: lo = 1, hi = p.nx - 1, step = 1; break;
                 }
                 for (int s = lo; s < hi; s += step) {
@@ -363,7 +383,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             case LSS_K_XSWEEP1: b += 2 * nxy; break;
             case LSS_K_VSWEEP1: b += 3 * nxy; break;
             case LSS_K_HV_Y3: b += hv ? 4 * nxy : 0; break;
-            case LSS_K_XSWEEP2: b += dr ? 0 : (hv ? 3 * nxy : 4 * nxy); break;
+            case LSS_K_XSWEEP2: b += dr ? 0 : ((hv && line_solver != LINE_PARTITIONED) ? 3 * nxy : 4 * nxy); break;
             case LSS_K_VSWEEP2: b += dr ? 0 : 3 * nxy; break;
             default: b += 4.0 * p.ny; break;
             }
@@ -393,23 +413,51 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         pre(LSS_K_BND_A);
         hipLaunchKernelGGL(k_bnd_a<T>, dim3(grid(TB_I)), dim3(TILE), 0, st, pd, td + tile_off[TB_I], s);
         post();
+        auto xs = [&](int stage) {
+            if (line_solver == LINE_EXACT) {
+                hipLaunchKernelGGL(k_xsweep<T>, dim3(grid(TB_X)), dim3(TILE), 0, st, pd, td + tile_off[TB_X], s, stage);
+                return;
+            }
+            const dim3 g(grid(TB_X)), b(P_XTJ * P_NCH);
+            switch (pm) {
+            case 1: hipLaunchKernelGGL((k_xsweep_p<T, 1>), g, b, 0, st, pd, td + tile_off[TB_X], s, stage); break;
+            case 2: hipLaunchKernelGGL((k_xsweep_p<T, 2>), g, b, 0, st, pd, td + tile_off[TB_X], s, stage); break;
+            case 4: hipLaunchKernelGGL((k_xsweep_p<T, 4>), g, b, 0, st, pd, td + tile_off[TB_X], s, stage); break;
+            case 8: hipLaunchKernelGGL((k_xsweep_p<T, 8>), g, b, 0, st, pd, td + tile_off[TB_X], s, stage); break;
+            default: hipLaunchKernelGGL((k_xsweep_p<T, 16>), g, b, 0, st, pd, td + tile_off[TB_X], s, stage); break;
+            }
+        };
+        auto vs = [&](int stage) {
+            if (line_solver == LINE_EXACT) {
+                hipLaunchKernelGGL(k_vsweep<T>, dim3(grid(TB_V)), dim3(TILE), 0, st, pd, td + tile_off[TB_V], s, stage);
+                return;
+            }
+            const dim3 g(grid(TB_V)), b(64 * P_VROWS);
+            switch (pm) {
+            case 1: hipLaunchKernelGGL((k_vsweep_p<T, 1>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
+            case 2: hipLaunchKernelGGL((k_vsweep_p<T, 2>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
+            case 4: hipLaunchKernelGGL((k_vsweep_p<T, 4>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
+            case 8: hipLaunchKernelGGL((k_vsweep_p<T, 8>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
+            default: hipLaunchKernelGGL((k_vsweep_p<T, 16>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
+            }
+        };
         pre(LSS_K_XSWEEP1);
-        hipLaunchKernelGGL(k_xsweep<T>, dim3(grid(TB_X)), dim3(TILE), 0, st, pd, td + tile_off[TB_X], s, 1);
+        xs(1);
         post();
         pre(LSS_K_VSWEEP1);
-        hipLaunchKernelGGL(k_vsweep<T>, dim3(grid(TB_V)), dim3(TILE), 0, st, pd, td + tile_off[TB_V], s, 1);
+        vs(1);
         post();
-        if (any_hv) {
+        if (any_hv && line_solver == LINE_EXACT) {
             pre(LSS_K_HV_Y3);
             hipLaunchKernelGGL(k_hv_y3<T>, dim3(grid(TB_R)), dim3(TILE), 0, st, pd, td + tile_off[TB_R], s);
             post();
         }
         if (any_split2) {
             pre(LSS_K_XSWEEP2);
-            hipLaunchKernelGGL(k_xsweep<T>, dim3(grid(TB_X)), dim3(TILE), 0, st, pd, td + tile_off[TB_X], s, 2);
+            xs(2);
             post();
             pre(LSS_K_VSWEEP2);
-            hipLaunchKernelGGL(k_vsweep<T>, dim3(grid(TB_V)), dim3(TILE), 0, st, pd, td + tile_off[TB_V], s, 2);
+            vs(2);
             post();
         }
         pre(LSS_K_BND_B);
@@ -667,7 +715,7 @@ template <typename T> lss_adi_plan *model_plan(const lss_model *models, int64_t 
             for (size_t k = 0; k < s.coeff_sets; ++k)
                 separable_set(m, g, s, s.times[k], s.coeff.data() + k * stride);
         }
-        ms.d[q] = s.desc(m.coeff_mode);
+        ms.d[q] = s.desc(m.coeff_mode, m.line_solver);
     }
     lss_adi_plan *plan = make_plan(sizeof(T) == 8 ? LSS_FP64 : LSS_FP32, ms.d.data(), n);
     plan->init_host.resize(n);

@@ -23,6 +23,8 @@ enum : int32_t { TRI_THOMAS = 0, TRI_DSWEEP = 1 };
 //   SEPARABLE: each transformed coefficient q in {a,b,c,d,e,f} is q(i,j) = X0q[i]*Y0q[j] + X1q[i]*Y1q[j]
 //              (12 nx vectors, 12 ny vectors); M..W are formed in the kernel.
 enum : int32_t { COEFF_FIELDS = 0, COEFF_SEPARABLE = 1 };
+// line solver of the sweeps: the reference's sequential order (one lane per line) or partitioned
+enum : int32_t { LINE_EXACT = 0, LINE_PARTITIONED = 1 };
 
 constexpr int TILE = 64;  // lines (or points) per workgroup tile: one wavefront
 
@@ -33,6 +35,7 @@ template <typename T> struct Prob {
     T theta, zeta, alpha, beta, gamma, delta, ni, rho, h1;
     T *a, *b;                  // ping-pong state: step s reads (s even ? a : b), writes the other
     T *y1, *y2, *y3, *scr;     // intermediates and line-solver scratch (nx*ny each)
+    T *y4;                     // HV's fourth stage: Y1's buffer (exact sweeps) or Y3's (partitioned)
     T *colA0, *colA1;          // v = 0 and v = v_max columns produced by boundary stage A (nx each)
     const uint32_t *idx1;      // index_of maps (lss_grid_config.hpp:110-118)
     const uint32_t *idx2;

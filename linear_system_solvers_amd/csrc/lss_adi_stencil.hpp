@@ -1,0 +1,140 @@
+// lss_adi_stencil.hpp -- device functions shared by the ADI kernels: stencil coefficients at a grid
+// point (FIELDS or SEPARABLE) and the stage right-hand sides, each in the reference's operand order.
+#pragma once
+
+#include "lss_adi_device.hpp"
+
+namespace lssamd {
+
+template <typename T> struct Stencil {
+    T M, Mt, P, Pt, Z, W, C;
+};
+
+template <typename T>
+__device__ __forceinline__ T sep(const T *X, const T *Y, int nx, int ny, int k, int i, int j)
+{
+    return X[(2 * k) * nx + i] * Y[(2 * k) * ny + j] + X[(2 * k + 1) * nx + i] * Y[(2 * k + 1) * ny + j];
+}
+
+// Stencil coefficients at (zeta_i, eta_j): general_heston_equation_coefficients (coefficients.hpp:89-95).
+template <typename T>
+__device__ __forceinline__ Stencil<T> stencil_at(const Prob<T> &p, const T *__restrict__ cs, int i, int j)
+{
+    Stencil<T> s;
+    if (p.coeff_mode == COEFF_FIELDS) {
+        const int64_t nxy = (int64_t)p.nx * p.ny, q = (int64_t)i * p.ny + j;
+        s.M = cs[q];
+        s.Mt = cs[nxy + q];
+        s.P = cs[2 * nxy + q];
+        s.Pt = cs[3 * nxy + q];
+        s.Z = cs[4 * nxy + q];
+        s.W = cs[5 * nxy + q];
+        s.C = cs[6 * nxy + q];
+    } else {
+        const T *X = cs, *Y = cs + 12 * (int64_t)p.nx;
+        const T a = sep(X, Y, p.nx, p.ny, 0, i, j), b = sep(X, Y, p.nx, p.ny, 1, i, j);
+        const T c = sep(X, Y, p.nx, p.ny, 2, i, j), d = sep(X, Y, p.nx, p.ny, 3, i, j);
+        const T e = sep(X, Y, p.nx, p.ny, 4, i, j), f = sep(X, Y, p.nx, p.ny, 5, i, j);
+        const T two = (T)2.0, half = (T)0.5;
+        s.M = (p.alpha * a - p.delta * d);
+        s.Mt = (p.beta * b - p.ni * e);
+        s.P = (p.alpha * a + p.delta * d);
+        s.Pt = (p.beta * b + p.ni * e);
+        s.Z = (two * p.alpha * a - half * p.rho * f);
+        s.W = (two * p.beta * b - half * p.rho * f);
+        s.C = c;
+    }
+    return s;
+}
+
+// D, E, F on the v = 0 row at zeta_i (used by boundary stage A).
+template <typename T>
+__device__ __forceinline__ void def_row0(const Prob<T> &p, const T *__restrict__ cs, int i, T &D, T &E, T &F)
+{
+    if (p.coeff_mode == COEFF_FIELDS) {
+        const T *r = cs + 7 * (int64_t)p.nx * p.ny;
+        D = r[i];
+        E = r[p.nx + i];
+        F = r[2 * p.nx + i];
+    } else {
+        const T *X = cs, *Y = cs + 12 * (int64_t)p.nx;
+        D = sep(X, Y, p.nx, p.ny, 3, i, 0);
+        E = sep(X, Y, p.nx, p.ny, 4, i, 0);
+        F = sep(X, Y, p.nx, p.ny, 5, i, 0);
+    }
+}
+
+template <typename T> __device__ __forceinline__ const T *coeff_set(const Prob<T> &p, int step)
+{
+    return p.coeff + (p.coeff_sets > 1 ? (int64_t)step : 0) * p.coeff_stride;
+}
+template <typename T> __device__ __forceinline__ const T *bnd_set(const Prob<T> &p, int step)
+{
+    return p.bnd + (p.bnd_sets > 1 ? (int64_t)step : 0) * p.bnd_stride;
+}
+
+// -------------------------------------------------------------------------------------------------
+// Right-hand sides (operand order of the reference's rhs functions)
+// -------------------------------------------------------------------------------------------------
+// rhs_intermed_1 (lss_heat_douglas_rachford_method.hpp:37-68); `in` points at (t, j), row stride ny
+template <typename T>
+__device__ __forceinline__ T rhs_intermed_1(const Prob<T> &p, const Stencil<T> &s, const T *in, int ny)
+{
+    const T one = (T)1.0, gamma = p.gamma, theta = p.theta;
+    return (gamma * s.C * in[-ny - 1]) + ((one - theta) * s.M * in[-ny]) - (gamma * s.C * in[-ny + 1]) +
+           (s.Mt * in[-1]) + ((one - s.W - (one - theta) * s.Z) * in[0]) + (s.Pt * in[1]) -
+           (gamma * s.C * in[ny - 1]) + ((one - theta) * s.P * in[ny]) + (gamma * s.C * in[ny + 1]);
+}
+
+// CS rhs_intermed_3 (lss_heat_craig_sneyd_method.hpp:128-156), MCS adds the (zeta - theta) term
+// of lss_heat_modified_craig_sneyd_method.hpp:158-163 (with its "- P")
+template <typename T, bool MCS>
+__device__ __forceinline__ T rhs_intermed_3(const Prob<T> &p, const Stencil<T> &s, const T *in, const T *inh,
+                                            const T *inhn, int ny)
+{
+    const T gamma = p.gamma, theta = p.theta, zeta = p.zeta;
+    T v = (-theta * s.M * in[-ny]) + (theta * s.Z * in[0]) - (theta * s.P * in[ny]) + (inh[0]) +
+          (zeta * gamma * s.C *
+           ((inhn[ny + 1] - in[ny + 1]) - (inhn[ny - 1] - in[ny - 1]) - (inhn[-ny + 1] - in[-ny + 1]) +
+            (inhn[-ny - 1] - in[-ny - 1])));
+    if (MCS) {
+        v = v + ((zeta - theta) * (s.M * (inhn[-ny] - in[-ny]) - (s.Z + s.W) * (inhn[0] - in[0]) -
+                                   s.P * (inhn[ny] - in[ny]) + s.Mt * (inhn[-1] - in[-1]) +
+                                   s.Pt * (inhn[1] - in[1])));
+    }
+    return v;
+}
+
+// HV rhs_intermed_4 (lss_heat_hundsdorfer_verwer_method.hpp:174-194)
+template <typename T>
+__device__ __forceinline__ T rhs_intermed_4(const Prob<T> &p, const Stencil<T> &s, const T *in, const T *inh, int ny)
+{
+    const T theta = p.theta;
+    return (-theta * s.M * in[-ny]) + (theta * s.Z * in[0]) - (theta * s.P * in[ny]) + inh[0];
+}
+
+// v-direction rhs shared by every scheme (e.g. lss_heat_douglas_rachford_method.hpp:106-126)
+template <typename T>
+__device__ __forceinline__ T rhs_v(const Prob<T> &p, const Stencil<T> &s, const T *in, const T *inh)
+{
+    const T theta = p.theta;
+    return (-theta * s.Mt * in[-1]) + (theta * s.W * in[0]) - (theta * s.Pt * in[1]) + inh[0];
+}
+
+// HV explicit stage value Y3 at a point (lss_heat_hundsdorfer_verwer_method.hpp:146-168);
+// in = u, inh = Y1, inhn = Y2, each pointing at the point.
+template <typename T>
+__device__ __forceinline__ T hv_y3_at(const Prob<T> &p, const Stencil<T> &s, const T *in, const T *inh,
+                                      const T *inhn, int ny)
+{
+    const T one = (T)1.0, gamma = p.gamma, theta = p.theta, zeta = p.zeta;
+    return (-theta * s.M * inh[-ny]) + ((one + theta * s.Z) * inh[0]) - (theta * s.P * inh[ny]) +
+           (theta * s.M * in[-ny]) - (theta * s.Z * in[0]) + (theta * s.P * in[ny]) +
+           (zeta * gamma * s.C *
+            ((inhn[ny + 1] - in[ny + 1]) - (inhn[ny - 1] - in[ny - 1]) - (inhn[-ny + 1] - in[-ny + 1]) +
+             (inhn[-ny - 1] - in[-ny - 1]))) +
+           (zeta * (s.M * (inhn[-ny] - in[-ny]) - (s.W + s.Z) * (inhn[0] - in[0]) + s.P * (inhn[ny] - in[ny]) +
+                    s.Mt * (inhn[-1] - in[-1]) + s.Pt * (inhn[1] - in[1])));
+}
+
+}  // namespace lssamd

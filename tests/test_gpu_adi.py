@@ -129,3 +129,69 @@ def test_graph_replay_equals_direct_launches():
     for k in range(1, 4):
         assert np.array_equal(outs[k].cpu().numpy(), ref)
     plan.close()
+
+
+# ---------------------------------------------------------------------------------------------
+# Partitioned line solver: the same per-point arithmetic, a different elimination order for the
+# line solves, so it is held to tolerances, normwise per problem (max |g - c| / ||c||_inf).
+# ---------------------------------------------------------------------------------------------
+def _normwise(g, c):
+    return float(np.abs(np.asarray(g, np.float64) - c).max() / np.abs(c).max())
+
+
+@pytest.mark.parametrize("case", R.CASES, ids=[c["stem"] for c in R.CASES])
+@pytest.mark.parametrize("mode", [L.COEFF_FIELDS, L.COEFF_SEPARABLE])
+def test_partitioned_matches_oracle(oracle, case, mode):
+    m = model_from_case(case, coeff_mode=mode, line_solver=L.LINE_PARTITIONED)
+    g = _solve(m)
+    c = oracle_solve(oracle, m)
+    print(f"{case['stem']} partitioned mode {mode}: normwise {_normwise(g, c):.2e} per-value {rel_err(g, c):.2e}",
+          flush=True)
+    assert _normwise(g, c) <= 1e-12
+
+
+def _stable(nx, ny, steps=10, **kw):
+    """Heston DR/CS/MCS/HV problem whose time step keeps the reference's explicit v = 0 row stable
+    (3 nu E <= 2 with the uniform-hints v map: k <= 1.5 h2 for kappa = 3, theta_v = 0.2)."""
+    k = min(0.01, 1.5 / (ny - 1))
+    return config0(nx=nx, ny=ny, nt=steps + 1, t_hi=k * steps, **kw)
+
+
+def _assert_stable(c, m):
+    assert np.isfinite(c).all() and np.abs(c).max() <= 2.0 * m.x_hi, "test problem is not stable"
+
+
+@pytest.mark.parametrize("nx,ny", [(4, 4), (6, 5), (9, 7), (66, 66), (67, 130), (131, 200), (300, 140), (514, 258),
+                                   (1026, 70), (70, 1026)])
+@pytest.mark.parametrize("splitting", [0, 1, 2, 3])
+def test_partitioned_sizes_and_schemes(oracle, nx, ny, splitting):
+    m = _stable(nx, ny, splitting=splitting, weighting=0.3, coeff_mode=L.COEFF_SEPARABLE,
+                line_solver=L.LINE_PARTITIONED)
+    g = _solve(m)
+    c = oracle_solve(oracle, m)
+    _assert_stable(c, m)
+    assert _normwise(g, c) <= 1e-12
+
+
+@pytest.mark.parametrize("fp,tol", [(L.FP64, 1e-13), (L.FP32, 1e-5)])
+def test_partitioned_agrees_with_exact_sweeps_on_gpu(oracle, fp, tol):
+    for splitting in range(4):
+        m = _stable(150, 60, steps=50, splitting=splitting, weighting=0.4, coeff_mode=L.COEFF_FIELDS)
+        ex = _solve(m, fp)
+        _assert_stable(ex.astype(np.float64), m)
+        m.line_solver = L.LINE_PARTITIONED
+        pa = _solve(m, fp)
+        assert _normwise(pa, ex.astype(np.float64)) <= tol
+
+
+def test_partitioned_book_matches_oracle(oracle):
+    rng = np.random.default_rng(11)
+    models = [_stable(int(rng.integers(5, 300)), int(rng.integers(5, 150)), steps=int(rng.integers(1, 30)),
+                      splitting=q % 4, strike=float(rng.uniform(60, 140)), coeff_mode=L.COEFF_SEPARABLE,
+                      line_solver=L.LINE_PARTITIONED)
+              for q in range(16)]
+    outs = L.Plan.from_models(models).run_host()
+    for m, g in zip(models, outs):
+        c = oracle_solve(oracle, m)
+        _assert_stable(c, m)
+        assert _normwise(g, c) <= 1e-12
