@@ -138,6 +138,9 @@ typedef struct lss_model {
 int lss_model_plan_create(const lss_model *models, int64_t count, int32_t fp, lss_adi_plan **plan);
 /* sampled initial (terminal) surface of problem p of a model plan, host pointer, nx*ny */
 int lss_model_initial(const lss_adi_plan *plan, int64_t p, void *u0_host);
+/* physical grid of problem p of a model plan: S (or F) values X(zeta_i), nx, and v (or alpha)
+ * values Y(eta_j), ny (lss_grid_transform_config.hpp:118-149), host pointers in the plan's precision */
+int lss_model_grid(const lss_adi_plan *plan, int64_t p, void *xs_host, void *ys_host);
 /* sizes of problem p */
 int lss_plan_problem_shape(const lss_adi_plan *plan, int64_t p, int64_t *nx, int64_t *ny, int64_t *steps);
 
@@ -163,6 +166,33 @@ enum {
 };
 int lss_adi_run_profiled(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, void *stream, double *ms,
                          double *bytes, int64_t *launches);
+
+/*
+ * Reverse mode (no counterpart in the reference, whose solvers have no derivatives): the adjoint of
+ * the time loop, shaped as an autograd-style forward/backward pair.
+ *   lss_adi_forward   the time loop of lss_adi_run_dev (device pointers, enqueued on `stream`) that also
+ *                     keeps every ckpt_every-th time level (ckpt_every even, >= 2) in plan memory.
+ *   lss_adi_backward  given lam_out[p] = dJ/du_out[p] (device, nx*ny), recomputes each segment of
+ *                     ckpt_every steps from its checkpoint and runs the adjoint steps backwards.  Writes
+ *                     lam_in[p] = dJ/du_in[p] (device; a NULL entry or array skips it) and, on the host,
+ *                     dparams[p * LSS_MAX_PARAMS + k] = dJ/dparam_k of problem p.  Synchronous.
+ *                     Needs nx - 2 <= 1024 and ny - 2 <= 1024.
+ * Parameters enter through the transformed coefficients: a derivative term says
+ *   d q(i, j) / d param_k = X_t[i] * Y_t[j]   for q one of a..f (LSS_Q_*),
+ * i.e. the derivative of heat_data_transform's coefficient at grid point (i, j)
+ * (lss_heat_data_transform.hpp:174-214).  Model plans carry their terms (Heston: kappa, theta_v, sigma,
+ * rho, r; SABR: nu, rho, beta); other plans set them with lss_adi_set_param_terms: term t has
+ * parameter term_param[t] and coefficient term_coeff[t]; factors holds `sets` (1 or steps) blocks of
+ * nterm * (nx X-values then ny Y-values) in the plan's precision.
+ */
+enum { LSS_MAX_PARAMS = 6, LSS_MAX_TERMS = 12 };
+enum { LSS_Q_A = 0, LSS_Q_B = 1, LSS_Q_C = 2, LSS_Q_D = 3, LSS_Q_E = 4, LSS_Q_F = 5 };
+int lss_adi_set_param_terms(lss_adi_plan *plan, int64_t p, int32_t nparam, int32_t nterm, const int32_t *term_param,
+                            const int32_t *term_coeff, int64_t sets, const void *factors);
+int lss_plan_param_count(const lss_adi_plan *plan, int64_t p, int32_t *nparam);
+int lss_adi_forward(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, int32_t ckpt_every, void *stream);
+int lss_adi_backward(lss_adi_plan *plan, const void *const *lam_out, void *const *lam_in, double *dparams,
+                     void *stream);
 
 #ifdef __cplusplus
 }

@@ -33,8 +33,13 @@ MODEL_HESTON, MODEL_SABR = 0, 1
 
 # every symbol include/lss_amd.h declares
 EXPORTS = ("lss_last_error", "lss_version", "lss_adi_plan_create", "lss_adi_plan_destroy", "lss_adi_run_dev",
-           "lss_adi_run_host", "lss_model_plan_create", "lss_model_initial", "lss_plan_problem_shape",
-           "lss_tridiag_batched", "lss_adi_run_profiled")
+           "lss_adi_run_host", "lss_model_plan_create", "lss_model_initial", "lss_model_grid", "lss_plan_problem_shape",
+           "lss_tridiag_batched", "lss_adi_run_profiled", "lss_adi_set_param_terms", "lss_plan_param_count",
+           "lss_adi_forward", "lss_adi_backward")
+MAX_PARAMS = 6
+MAX_TERMS = 12
+HESTON_PARAMS = ("kappa", "theta_v", "sigma", "rho", "r")
+SABR_PARAMS = ("nu", "rho", "beta")
 KERNEL_KINDS = ("bnd_a", "xsweep1", "vsweep1", "hv_y3", "xsweep2", "vsweep2", "bnd_b")
 
 
@@ -142,12 +147,20 @@ def lib():
         L.lss_adi_run_host.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
         L.lss_model_plan_create.argtypes = [C.POINTER(_Model), C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]
         L.lss_model_initial.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+        L.lss_model_grid.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]
         L.lss_plan_problem_shape.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
                                              C.POINTER(C.c_int64)]
         L.lss_adi_run_profiled.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p,
                                            C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.lss_tridiag_batched.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.lss_adi_set_param_terms.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                              C.POINTER(C.c_int32), C.c_int64, C.c_void_p]
+        L.lss_plan_param_count.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]
+        L.lss_adi_forward.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32,
+                                      C.c_void_p]
+        L.lss_adi_backward.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                       C.POINTER(C.c_double), C.c_void_p]
         _lib = L
     return _lib
 
@@ -198,6 +211,14 @@ class Plan:
         _check(lib().lss_model_initial(self._h, p, out.ctypes.data))
         return out
 
+    def grid(self, p: int):
+        """Physical coordinates (xs, ys) of problem p of a model plan."""
+        nx, ny, _ = self.shape(p)
+        xs = np.zeros(nx, dtype=np_dtype(self.fp))
+        ys = np.zeros(ny, dtype=np_dtype(self.fp))
+        _check(lib().lss_model_grid(self._h, p, xs.ctypes.data, ys.ctypes.data))
+        return xs, ys
+
     def run_host(self, inits: Optional[Sequence[np.ndarray]] = None) -> list:
         """Synchronous solve from host arrays (default: the model plan's sampled initial surfaces)."""
         dt = np_dtype(self.fp)
@@ -215,6 +236,56 @@ class Plan:
         pin = (C.c_void_p * self.count)(*in_ptrs)
         pout = (C.c_void_p * self.count)(*out_ptrs)
         _check(lib().lss_adi_run_dev(self._h, pin, pout, C.c_void_p(stream), int(graph)))
+
+
+    # ---- reverse mode -------------------------------------------------------------------------
+    def param_count(self, p: int) -> int:
+        n = C.c_int32()
+        _check(lib().lss_plan_param_count(self._h, p, C.byref(n)))
+        return n.value
+
+    def set_param_terms(self, p: int, nparam: int, term_param: Sequence[int], term_coeff: Sequence[int],
+                        factors: np.ndarray) -> None:
+        """Derivative terms of problem p: d q_{term_coeff[t]}(i, j) / d param_{term_param[t]} = X_t[i] Y_t[j];
+        factors has shape (sets, nterm, nx + ny)."""
+        nt = len(term_param)
+        tp = (C.c_int32 * max(nt, 1))(*term_param)
+        tq = (C.c_int32 * max(nt, 1))(*term_coeff)
+        f = np.ascontiguousarray(factors, dtype=np_dtype(self.fp))
+        sets = f.shape[0] if nt else 1
+        _check(lib().lss_adi_set_param_terms(self._h, p, nparam, nt, tp, tq, sets, f.ctypes.data))
+
+    def forward(self, in_ptrs: Sequence[int], out_ptrs: Sequence[int], ckpt_every: int = 32,
+                stream: int = 0) -> None:
+        """Time loop from/to device pointers that keeps every ckpt_every-th level for backward()."""
+        pin = (C.c_void_p * self.count)(*in_ptrs)
+        pout = (C.c_void_p * self.count)(*out_ptrs)
+        _check(lib().lss_adi_forward(self._h, pin, pout, int(ckpt_every), C.c_void_p(stream)))
+
+    def backward(self, lam_out_ptrs: Sequence[int], lam_in_ptrs: Optional[Sequence[int]] = None,
+                 stream: int = 0) -> np.ndarray:
+        """Adjoint of the last forward(): lam_out = dJ/du_out (device), writes dJ/du_in into lam_in_ptrs
+        (device, entries may be 0 to skip) and returns dJ/dparams, shape (count, MAX_PARAMS)."""
+        pl = (C.c_void_p * self.count)(*lam_out_ptrs)
+        pi = (C.c_void_p * self.count)(*lam_in_ptrs) if lam_in_ptrs is not None else None
+        dp = (C.c_double * (self.count * MAX_PARAMS))()
+        _check(lib().lss_adi_backward(self._h, pl, pi, dp, C.c_void_p(stream)))
+        return np.frombuffer(dp, dtype=np.float64).reshape(self.count, MAX_PARAMS).copy()
+
+
+def bilinear_weights(xs: np.ndarray, ys: np.ndarray, x0: float, y0: float) -> np.ndarray:
+    """Weights w with sum(w * U) = the bilinear interpolant of the surface U (rows xs, columns ys) at
+    (x0, y0); also the adjoint seed dJ/dU of that price."""
+    i = int(np.clip(np.searchsorted(xs, x0, side="right") - 1, 0, len(xs) - 2))
+    j = int(np.clip(np.searchsorted(ys, y0, side="right") - 1, 0, len(ys) - 2))
+    wx = (x0 - xs[i]) / (xs[i + 1] - xs[i])
+    wy = (y0 - ys[j]) / (ys[j + 1] - ys[j])
+    w = np.zeros((len(xs), len(ys)), dtype=np.float64)
+    w[i, j] = (1 - wx) * (1 - wy)
+    w[i + 1, j] = wx * (1 - wy)
+    w[i, j + 1] = (1 - wx) * wy
+    w[i + 1, j + 1] = wx * wy
+    return w
 
 
 def run_profiled(plan: "Plan", in_ptrs: Sequence[int], out_ptrs: Sequence[int], stream: int = 0) -> dict:

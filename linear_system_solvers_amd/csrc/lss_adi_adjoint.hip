@@ -1,0 +1,546 @@
+// lss_adi_adjoint.hip -- reverse mode of the ADI time step for gfx950 (hand-written adjoint).
+//
+// One forward step maps the level U_s to U_{s+1} through the stages of its scheme (boundary stage A,
+// S-sweep X1, v-sweep V1, [HV explicit H], S-sweep X2, v-sweep V2, boundary stage B; DR stops after
+// V1).  Every stage is linear in the state and in the transformed coefficients a..f of the grid
+// points it touches, so the adjoint step runs the stages backwards:
+//   k_adj_v (last)  B^T and the transposed solve of the last v-stage:   lambda(U_{s+1}) -> lv2, elo2
+//   k_adj_x (2)     transposed solve of X2 (CS, MCS, HV):               lv2 -> lx2
+//   k_adj_v (1)     lambda(Y2) gathered from lx2 (and lv2 for HV), transposed solve of V1 -> lv1, elo1
+//   k_adj_x (1)     lambda(Y1) gathered, transposed solve of X1:        -> lx1
+//   k_adj_u         lambda(U_s) gathered from every equation that reads U_s (the stencils of the
+//                   right-hand sides, the Dirichlet ends of the S-lines through idx2, boundary stage A)
+// A transposed solve of row equations (a_k, b_k, c_k) solves A^T l = r, whose row k is
+// (c_{k-1}, b_k, a_{k+1}); it reuses the register-resident partitioned elimination of the forward
+// sweeps.  Every right-hand side is written as a gather (each output point sums over the equation
+// points whose stencil contains it), so no kernel scatters or uses atomics.
+//
+// Parameter gradients: an equation F(y, in; q) = 0 with multiplier l contributes -l dF/dq for each
+// stencil coefficient (M, M~, P, P~, Z, W, C of coefficients.hpp:89-95); those map to the transformed
+// coefficients a..f and, through the problem's derivative terms, to the model parameters.  Each
+// workgroup reduces its contributions in double and adds them to its own slot, so the sums are
+// deterministic.
+
+#include "lss_adi_chunk.hpp"
+#include "lss_adi_stencil.hpp"
+
+namespace lssamd {
+
+// gradient with respect to the stencil coefficients at one point
+template <typename T> struct SG {
+    T M, Mt, P, Pt, Z, W, C;
+};
+
+template <typename T> __device__ __forceinline__ SG<T> sg_zero()
+{
+    SG<T> g;
+    g.M = g.Mt = g.P = g.Pt = g.Z = g.W = g.C = (T)0;
+    return g;
+}
+
+// acc[k] += sum over terms of (d J / d q_t(i, j)) * X_t[i] * Y_t[j] for parameter k = tp[t]
+template <typename T>
+__device__ __forceinline__ void add_terms_q(const Prob<T> &p, const Adj<T> &A, int step, int i, int j, T ga, T gb,
+                                            T gc, T gd, T ge, T gf, double *acc)
+{
+    const int nx = p.nx, ny = p.ny;
+    const T *f = A.dfac + (A.dfac_sets > 1 ? (int64_t)step : 0) * A.dfac_stride;
+    for (int t = 0; t < A.nterm; ++t) {
+        const int q = A.tq[t];
+        const T gq = q == 0 ? ga : (q == 1 ? gb : (q == 2 ? gc : (q == 3 ? gd : (q == 4 ? ge : gf))));
+        const T *ft = f + (int64_t)t * (nx + ny);
+        const double v = (double)gq * (double)ft[i] * (double)ft[nx + j];
+        const int k = A.tp[t];
+#pragma unroll
+        for (int z = 0; z < NPAR; ++z)
+            acc[z] += (k == z) ? v : 0.0;
+    }
+}
+
+// M = alpha a - delta d, M~ = beta b - ni e, P = alpha a + delta d, P~ = beta b + ni e,
+// Z = 2 alpha a - rho f / 2, W = 2 beta b - rho f / 2, C = c  (coefficients.hpp:89-95)
+template <typename T>
+__device__ __forceinline__ void add_terms(const Prob<T> &p, const Adj<T> &A, int step, int i, int j, const SG<T> &g,
+                                          double *acc)
+{
+    if (A.nterm == 0)
+        return;
+    const T two = (T)2, half = (T)0.5;
+    const T ga = p.alpha * (g.M + g.P) + two * p.alpha * g.Z;
+    const T gb = p.beta * (g.Mt + g.Pt) + two * p.beta * g.W;
+    const T gc = g.C;
+    const T gd = p.delta * (g.P - g.M);
+    const T ge = p.ni * (g.Pt - g.Mt);
+    const T gf = -half * p.rho * (g.Z + g.W);
+    add_terms_q(p, A, step, i, j, ga, gb, gc, gd, ge, gf, acc);
+}
+
+// sum acc over the workgroup and add it to slot[0..NPAR); every thread of the block must call it
+template <int NWAVES>
+__device__ __forceinline__ void block_acc_store(double *acc, double (*red)[NPAR], double *slot)
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int z = 0; z < NPAR; ++z) {
+        double v = acc[z];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1)
+            v += __shfl_xor(v, d);
+        if (lane == 0)
+            red[wave][z] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NPAR) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < NWAVES; ++w)
+            v += red[w][threadIdx.x];
+        slot[threadIdx.x] += v;
+    }
+}
+
+// -------------------------------------------------------------------------------------------------
+// Weights of the right-hand sides: the coefficient with which an equation at point e reads the
+// value at e + (oi, oj).  Each restates the corresponding forward right-hand side term by term.
+// -------------------------------------------------------------------------------------------------
+// rhs_intermed_1 (lss_heat_douglas_rachford_method.hpp:37-68), on U
+template <typename T> __device__ __forceinline__ T w_r1(const Prob<T> &p, const Stencil<T> &s, int oi, int oj)
+{
+    const T one = (T)1, th = p.theta, gC = p.gamma * s.C;
+    if (oi == -1)
+        return oj == -1 ? gC : (oj == 0 ? (one - th) * s.M : -gC);
+    if (oi == 1)
+        return oj == -1 ? -gC : (oj == 0 ? (one - th) * s.P : gC);
+    return oj == -1 ? s.Mt : (oj == 0 ? (one - s.W - (one - th) * s.Z) : s.Pt);
+}
+
+// v-direction rhs (e.g. lss_heat_douglas_rachford_method.hpp:106-126), on `in` (same row)
+template <typename T> __device__ __forceinline__ T w_rv(const Prob<T> &p, const Stencil<T> &s, int oj)
+{
+    const T th = p.theta;
+    return oj == -1 ? -th * s.Mt : (oj == 0 ? th * s.W : -th * s.Pt);
+}
+
+// CS / MCS rhs_intermed_3 (lss_heat_craig_sneyd_method.hpp:128-156,
+// lss_heat_modified_craig_sneyd_method.hpp:128-165), on U (onU) or on Y2 (!onU)
+template <typename T>
+__device__ __forceinline__ T w_r3(const Prob<T> &p, const Stencil<T> &s, int oi, int oj, bool mcs, bool onU)
+{
+    const T th = p.theta, zg = p.zeta * p.gamma * s.C, zt = mcs ? (p.zeta - th) : (T)0;
+    T base = (T)0, d = (T)0;  // base: U-only terms; d: coefficient of Delta = Y2 - U
+    if (oi != 0 && oj != 0) {
+        d = (oi == oj) ? zg : -zg;
+    } else if (oi == -1) {
+        base = -th * s.M;
+        d = zt * s.M;
+    } else if (oi == 1) {
+        base = -th * s.P;
+        d = -zt * s.P;
+    } else if (oj == 0) {
+        base = th * s.Z;
+        d = -zt * (s.Z + s.W);
+    } else {
+        d = zt * (oj == -1 ? s.Mt : s.Pt);
+    }
+    return onU ? base - d : d;
+}
+
+// HV explicit stage Y3 (lss_heat_hundsdorfer_verwer_method.hpp:128-172): which = 0 on Y1, 1 on U, 2 on Y2
+template <typename T>
+__device__ __forceinline__ T w_h(const Prob<T> &p, const Stencil<T> &s, int oi, int oj, int which)
+{
+    const T one = (T)1, th = p.theta, z = p.zeta, zg = p.zeta * p.gamma * s.C;
+    T y1 = (T)0, u = (T)0, d = (T)0;
+    if (oi != 0 && oj != 0) {
+        d = (oi == oj) ? zg : -zg;
+    } else if (oi == -1) {
+        y1 = -th * s.M;
+        u = th * s.M;
+        d = z * s.M;
+    } else if (oi == 1) {
+        y1 = -th * s.P;
+        u = th * s.P;
+        d = z * s.P;
+    } else if (oj == 0) {
+        y1 = one + th * s.Z;
+        u = -th * s.Z;
+        d = -z * (s.W + s.Z);
+    } else {
+        d = z * (oj == -1 ? s.Mt : s.Pt);
+    }
+    return which == 0 ? y1 : (which == 1 ? u - d : d);
+}
+
+// HV rhs_intermed_4 (lss_heat_hundsdorfer_verwer_method.hpp:174-194), on Y2 (same column)
+template <typename T> __device__ __forceinline__ T w_r4(const Prob<T> &p, const Stencil<T> &s, int oi)
+{
+    const T th = p.theta;
+    return oi == -1 ? -th * s.M : (oi == 0 ? th * s.Z : -th * s.P);
+}
+
+template <typename T> __device__ __forceinline__ bool inside(const Prob<T> &p, int i, int j)
+{
+    return i >= 1 && i <= p.nx - 2 && j >= 1 && j <= p.ny - 2;
+}
+
+// lambda(Y2) at (i, j) (interior row i, any column): gathered from the X2 equations (CS/MCS: Y2 in
+// the Delta terms; HV: rhs_intermed_4 and the explicit stage H, whose multiplier is also lx2) and,
+// for HV, from the V2 equations (in = Y2).
+template <typename T>
+__device__ __forceinline__ T lam_y2(const Prob<T> &p, const Adj<T> &A, const T *cs, int i, int j)
+{
+    const int ny = p.ny;
+    const bool hv = p.scheme == SCHEME_HV, mcs = p.scheme == SCHEME_MCS;
+    T v = (T)0;
+#pragma unroll
+    for (int di = -1; di <= 1; ++di) {
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj) {
+            const int ei = i + di, ej = j + dj;
+            if (!inside(p, ei, ej))
+                continue;
+            const Stencil<T> s = stencil_at(p, cs, ei, ej);
+            const int64_t q = (int64_t)ei * ny + ej;
+            if (hv) {
+                T w = w_h(p, s, -di, -dj, 2);
+                if (dj == 0)
+                    w += w_r4(p, s, -di);
+                v += w * A.lx2[q];
+                if (di == 0)
+                    v += w_rv(p, s, -dj) * A.lv2[q];
+            } else {
+                v += w_r3(p, s, -di, -dj, mcs, false) * A.lx2[q];
+            }
+        }
+    }
+    return v;
+}
+
+// -------------------------------------------------------------------------------------------------
+// Adjoint of a v-sweep (one wavefront per row i).  last: the step's last v-stage, whose output went
+// through boundary stage B into U_{s+1}; otherwise V1 of CS/MCS/HV.
+// -------------------------------------------------------------------------------------------------
+template <typename T, int M>
+__global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
+                                                      const Tile *__restrict__ tiles, int step, int last,
+                                                      double *__restrict__ slots)
+{
+    __shared__ double red[VROWS][NPAR];
+    const Tile tl = tiles[blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    const Adj<T> &A = adjs[tl.prob];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double acc[NPAR];
+#pragma unroll
+    for (int z = 0; z < NPAR; ++z)
+        acc[z] = 0.0;
+    const bool run = step < p.steps && (last || p.scheme != SCHEME_DR) && wave < tl.count;
+    if (run) {
+        const int i = tl.first + wave;
+        const int nx = p.nx, ny = p.ny, N = ny - 2, P = nx - 1;
+        const T *u = (step & 1) ? p.b : p.a;
+        const T *un = (step & 1) ? p.a : p.b;
+        const T *lamn = ((step + 1) & 1) ? A.lb : A.la;
+        const T *cs = coeff_set(p, step);
+        const T one = (T)1, theta = p.theta;
+        const bool hv = p.scheme == SCHEME_HV;
+        const T *yrow = (last ? un : p.y2) + (int64_t)i * ny;
+        const T *irow = ((last && hv) ? p.y2 : u) + (int64_t)i * ny;
+        T *L = (last ? A.lv2 : A.lv1) + (int64_t)i * ny;
+        // lambda of the stage output at (i, j): B^T (explicit_boundary.hpp:202-238) for the last stage
+        auto lam_out = [&](int j) -> T {
+            if (!last)
+                return lam_y2(p, A, cs, i, j);
+            T v = lamn[(int64_t)i * ny + j];
+            if (i == P - 1 || i == P - 2) {
+                T sB = (T)0;
+                for (uint32_t k = A.inv2o[j]; k < A.inv2o[j + 1]; ++k)
+                    sB += lamn[(int64_t)P * ny + A.inv2i[k]];
+                v += (i == P - 1) ? ((T)4 / (T)3) * sB : -(sB / (T)3);
+            }
+            return v;
+        };
+        auto eq = [&](int k) -> Eq<T> {
+            Eq<T> e;
+            e.b = one + theta * stencil_at(p, cs, i, k).W;
+            e.a = (k >= 2) ? -theta * stencil_at(p, cs, i, k - 1).Pt : (T)0;
+            e.c = (k <= N - 1) ? -theta * stencil_at(p, cs, i, k + 1).Mt : (T)0;
+            e.r = lam_out(k);
+            return e;
+        };
+        auto put = [&](int k, T x) {
+            L[k] = x;
+            SG<T> g = sg_zero<T>();
+            g.Mt = theta * x * (yrow[k - 1] - irow[k - 1]);
+            g.W = -theta * x * (yrow[k] - irow[k]);
+            g.Pt = theta * x * (yrow[k + 1] - irow[k + 1]);
+            add_terms(p, A, step, i, k, g, acc);
+            if (k == 1) {
+                const T a1 = -theta * stencil_at(p, cs, i, 1).Mt;
+                (last ? A.elo2 : A.elo1)[i] = lam_out(0) - a1 * x;
+            }
+        };
+        wave_solve<T, M>(N, lane, eq, put);
+    }
+    block_acc_store<VROWS>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
+}
+
+// -------------------------------------------------------------------------------------------------
+// Adjoint of an S-sweep (XTJ columns x NCH chunks per workgroup).  stage 2: X2 of CS/MCS/HV
+// (with HV's explicit stage H, whose multiplier equals X2's); stage 1: X1.
+// -------------------------------------------------------------------------------------------------
+template <typename T, int M>
+__global__ void __launch_bounds__(XTJ *NCH) k_adj_x(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
+                                                    const Tile *__restrict__ tiles, int step, int stage,
+                                                    double *__restrict__ slots)
+{
+    __shared__ T sh[2][4][NCH][XTJ];
+    __shared__ double red[XTJ * NCH / 64][NPAR];
+    const Tile tl = tiles[blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    const Adj<T> &A = adjs[tl.prob];
+    double acc[NPAR];
+#pragma unroll
+    for (int z = 0; z < NPAR; ++z)
+        acc[z] = 0.0;
+    if (step >= p.steps || (stage == 2 && p.scheme == SCHEME_DR))
+        return;  // uniform over the workgroup
+    const int col = threadIdx.x % XTJ, chn = threadIdx.x / XTJ;
+    const bool colok = col < tl.count;
+    const int j = tl.first + (colok ? col : 0);
+    const int nx = p.nx, ny = p.ny, N = nx - 2;
+    const T *u = (step & 1) ? p.b : p.a;
+    const T *cs = coeff_set(p, step);
+    const T one = (T)1, theta = p.theta, zeta = p.zeta, gamma = p.gamma;
+    const int sch = p.scheme;
+    const bool hv = sch == SCHEME_HV, mcs = sch == SCHEME_MCS, dr = sch == SCHEME_DR;
+    const T *y = (stage == 1) ? p.y1 : p.y4;  // stage output (p.y4 is Y3's buffer for CS/MCS)
+    if (stage == 2 && !hv)
+        y = p.y3;
+    T *L = (stage == 2) ? A.lx2 : A.lx1;
+    auto lam_out = [&](int k) -> T {
+        const int64_t q = (int64_t)k * ny + j;
+        if (stage == 2)
+            return A.lv2[q];
+        if (dr)
+            return A.lv2[q];
+        if (!hv)
+            return A.lx2[q] + A.lv1[q];
+        T v = A.lv1[q];
+#pragma unroll
+        for (int di = -1; di <= 1; ++di) {
+            const int ei = k + di;
+            if (ei >= 1 && ei <= N)
+                v += w_h(p, stencil_at(p, cs, ei, j), -di, 0, 0) * A.lx2[(int64_t)ei * ny + j];
+        }
+        return v;
+    };
+    auto eq = [&](int k) -> Eq<T> {
+        Eq<T> e;
+        e.b = one + theta * stencil_at(p, cs, k, j).Z;
+        e.a = (k >= 2) ? -theta * stencil_at(p, cs, k - 1, j).P : (T)0;
+        e.c = (k <= N - 1) ? -theta * stencil_at(p, cs, k + 1, j).M : (T)0;
+        e.r = lam_out(k);
+        return e;
+    };
+    auto put = [&](int k, T x) {
+        const int64_t q = (int64_t)k * ny + j;
+        L[q] = x;
+        if (A.nterm == 0)
+            return;
+        SG<T> g = sg_zero<T>();
+        // implicit rows [-theta M, 1 + theta Z, -theta P] applied to the stage output
+        g.M = theta * x * y[q - ny];
+        g.Z = -theta * x * y[q];
+        g.P = theta * x * y[q + ny];
+        if (stage == 1) {  // rhs_intermed_1 on U
+            g.M += (one - theta) * x * u[q - ny];
+            g.Z += -(one - theta) * x * u[q];
+            g.W += -x * u[q];
+            g.P += (one - theta) * x * u[q + ny];
+            g.Mt += x * u[q - 1];
+            g.Pt += x * u[q + 1];
+            g.C += x * gamma * (u[q - ny - 1] - u[q - ny + 1] - u[q + ny - 1] + u[q + ny + 1]);
+        } else {
+            const T *y2 = p.y2;
+            auto D = [&](int64_t o) { return y2[q + o] - u[q + o]; };
+            const T cornerD = D(ny + 1) - D(ny - 1) - D(-ny + 1) + D(-ny - 1);
+            if (!hv) {  // rhs_intermed_3
+                g.M += -theta * x * u[q - ny];
+                g.Z += theta * x * u[q];
+                g.P += -theta * x * u[q + ny];
+                g.C += x * zeta * gamma * cornerD;
+                if (mcs) {
+                    const T zt = zeta - theta;
+                    g.M += x * zt * D(-ny);
+                    g.Z += -x * zt * D(0);
+                    g.W += -x * zt * D(0);
+                    g.P += -x * zt * D(ny);
+                    g.Mt += x * zt * D(-1);
+                    g.Pt += x * zt * D(1);
+                }
+            } else {  // rhs_intermed_4 on Y2, then the explicit stage H with the same multiplier
+                const T *y1 = p.y1;
+                g.M += -theta * x * y2[q - ny];
+                g.Z += theta * x * y2[q];
+                g.P += -theta * x * y2[q + ny];
+                g.M += x * (-theta * y1[q - ny] + theta * u[q - ny] + zeta * D(-ny));
+                g.Z += x * (theta * y1[q] - theta * u[q] - zeta * D(0));
+                g.P += x * (-theta * y1[q + ny] + theta * u[q + ny] + zeta * D(ny));
+                g.W += -x * zeta * D(0);
+                g.Mt += x * zeta * D(-1);
+                g.Pt += x * zeta * D(1);
+                g.C += x * zeta * gamma * cornerD;
+            }
+        }
+        add_terms(p, A, step, k, j, g, acc);
+    };
+    block_solve<T, M>(sh, N, col, chn, colok, eq, put);
+    block_acc_store<XTJ * NCH / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
+}
+
+// -------------------------------------------------------------------------------------------------
+// lambda(U_s): one workgroup per row r, threads over the columns.  Also the parameter terms of
+// boundary stage A (its D, E, F on the v = 0 row).
+// -------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_u(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
+                                               const Tile *__restrict__ tiles, int step, double *__restrict__ slots)
+{
+    __shared__ double red[4][NPAR];
+    const Tile tl = tiles[blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    const Adj<T> &A = adjs[tl.prob];
+    double acc[NPAR];
+#pragma unroll
+    for (int z = 0; z < NPAR; ++z)
+        acc[z] = 0.0;
+    if (step < p.steps) {
+        const int r = tl.first;
+        const int nx = p.nx, ny = p.ny, P = nx - 1;
+        const T *u = (step & 1) ? p.b : p.a;
+        T *lam = (step & 1) ? A.lb : A.la;
+        const T *cs = coeff_set(p, step);
+        const T *bs = bnd_set(p, step);
+        const T theta = p.theta, delta = p.delta, ni = p.ni, rho = p.rho;
+        const int sch = p.scheme;
+        const bool dr = sch == SCHEME_DR, hv = sch == SCHEME_HV, mcs = sch == SCHEME_MCS;
+        // lambda of colA0[i]: every v-line whose lower end read it (through idx1)
+        auto lamA = [&](int i) -> T {
+            T v = (T)0;
+            for (uint32_t k = A.inv1o[i]; k < A.inv1o[i + 1]; ++k) {
+                const uint32_t rr = A.inv1i[k];
+                if (rr >= 1 && (int)rr <= P - 1)
+                    v += A.elo2[rr] + (dr ? (T)0 : A.elo1[rr]);
+            }
+            return v;
+        };
+        for (int c = threadIdx.x; c < ny; c += blockDim.x) {
+            T v = (T)0;
+#pragma unroll
+            for (int di = -1; di <= 1; ++di) {
+#pragma unroll
+                for (int dj = -1; dj <= 1; ++dj) {
+                    const int ei = r + di, ej = c + dj;
+                    if (!inside(p, ei, ej))
+                        continue;
+                    const Stencil<T> s = stencil_at(p, cs, ei, ej);
+                    const int64_t q = (int64_t)ei * ny + ej;
+                    if (di == 0 && !hv)
+                        v += w_rv(p, s, -dj) * A.lv2[q];
+                    if (di == 0 && !dr)
+                        v += w_rv(p, s, -dj) * A.lv1[q];
+                    if (!dr && !hv)
+                        v += w_r3(p, s, -di, -dj, mcs, true) * A.lx2[q];
+                    if (hv)
+                        v += w_h(p, s, -di, -dj, 1) * A.lx2[q];
+                    v += w_r1(p, s, -di, -dj) * A.lx1[q];
+                }
+            }
+            // Dirichlet ends of the S-lines j with idx2[j] == c (implicit_kernel.hpp:86-101)
+            if (r == 0 || r == P) {
+                const int ei = (r == 0) ? 1 : P - 1;
+                for (uint32_t k = A.inv2o[c]; k < A.inv2o[c + 1]; ++k) {
+                    const int j = (int)A.inv2i[k];
+                    if (j < 1 || j > ny - 2)
+                        continue;
+                    const Stencil<T> s = stencil_at(p, cs, ei, j);
+                    const T w = (r == 0) ? theta * s.M : theta * s.P;  // -(folded coefficient)
+                    const int64_t q = (int64_t)ei * ny + j;
+                    T l = A.lx1[q];
+                    if (!dr)
+                        l += A.lx2[q];
+                    v += w * l;
+                    if (hv)  // H reads Y1 at rows 0 and P; Y1's ends are these U values
+                        v += -w * A.lx2[q];
+                }
+            }
+            // boundary stage A (explicit_boundary.hpp:39-84): the v = 0 column of U_{s+1} read U_s
+            if (c <= 2) {
+                if (c == 0) {
+#pragma unroll
+                    for (int di = -1; di <= 1; ++di) {
+                        const int i = r + di;
+                        if (i < 1 || i > P)
+                            continue;
+                        T D, E, F;
+                        def_row0(p, cs, i, D, E, F);
+                        const T la = lamA(i);
+                        if (di == 0)
+                            v += ((T)1 - (T)3 * ni * E + rho * F) * la;
+                        else if (i < P)
+                            v += (di == 1 ? -delta * D : delta * D) * la;
+                    }
+                } else if (r >= 1) {
+                    T D, E, F;
+                    def_row0(p, cs, r, D, E, F);
+                    v += (c == 1 ? (T)4 * ni * E : -ni * E) * lamA(r);
+                }
+                if (c == 0 && r >= 1 && A.nterm > 0) {
+                    const T la = lamA(r);
+                    const T *ur = u + (int64_t)r * ny;
+                    const T gd = (r < P) ? la * delta * (u[(int64_t)(r + 1) * ny] - u[(int64_t)(r - 1) * ny])
+                                         : -la * delta * ((T)2 * p.h1 * bs[nx + ny]);
+                    const T ge = la * ni * (-(T)3 * ur[0] + (T)4 * ur[1] - ur[2]);
+                    const T gf = la * rho * ur[0];
+                    add_terms_q(p, A, step, r, 0, (T)0, (T)0, (T)0, gd, ge, gf, acc);
+                }
+            }
+            lam[(int64_t)r * ny + c] = v;
+        }
+    }
+    block_acc_store<4>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
+}
+
+// -------------------------------------------------------------------------------------------------
+// Batched device copies: job blockIdx.y, grid-stride over its elements.
+// -------------------------------------------------------------------------------------------------
+template <typename T> __global__ void __launch_bounds__(256) k_copy_jobs(const CopyJob *__restrict__ jobs)
+{
+    const CopyJob jb = jobs[blockIdx.y];
+    const T *src = (const T *)jb.src;
+    T *dst = (T *)jb.dst;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < jb.n; k += (int64_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
+#define LSS_INST_A(T, M)                                                                                       \
+    template __global__ void k_adj_v<T, M>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *); \
+    template __global__ void k_adj_x<T, M>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
+LSS_INST_A(double, 1)
+LSS_INST_A(double, 2)
+LSS_INST_A(double, 4)
+LSS_INST_A(double, 8)
+LSS_INST_A(double, 16)
+LSS_INST_A(float, 1)
+LSS_INST_A(float, 2)
+LSS_INST_A(float, 4)
+LSS_INST_A(float, 8)
+LSS_INST_A(float, 16)
+#undef LSS_INST_A
+template __global__ void k_adj_u<double>(const Prob<double> *, const Adj<double> *, const Tile *, int, double *);
+template __global__ void k_adj_u<float>(const Prob<float> *, const Adj<float> *, const Tile *, int, double *);
+template __global__ void k_copy_jobs<double>(const CopyJob *);
+template __global__ void k_copy_jobs<float>(const CopyJob *);
+
+}  // namespace lssamd

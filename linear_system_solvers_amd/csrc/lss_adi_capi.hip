@@ -14,6 +14,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -34,6 +35,12 @@ template <typename T>
 __global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *);
 template <typename T, int M> __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int, int);
 template <typename T, int M> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int);
+template <typename T, int M>
+__global__ void k_adj_v(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
+template <typename T, int M>
+__global__ void k_adj_x(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
+template <typename T> __global__ void k_adj_u(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
+template <typename T> __global__ void k_copy_jobs(const CopyJob *);
 constexpr int P_NCH = 64, P_XTJ = 16, P_VROWS = 4;
 
 }  // namespace lssamd
@@ -118,6 +125,18 @@ struct lss_adi_plan {
     std::vector<int64_t> where;      // caller index -> internal position
     std::vector<int64_t> nx, ny, steps;
     std::vector<std::vector<char>> init_host;  // model plans: sampled initial surface per caller index
+    std::vector<std::vector<char>> grid_host;  // model plans: physical S (nx) then v (ny) coordinates
+    // parameter derivative terms per caller index (see Adj in lss_adi_device.hpp)
+    struct Terms {
+        int32_t nparam = 0, nterm = 0;
+        int64_t sets = 0;
+        int8_t tp[NTERM] = {}, tq[NTERM] = {};
+        std::vector<char> fac;  // sets * nterm * (nx + ny) values of the plan's precision
+    };
+    std::vector<Terms> terms;
+    virtual void forward_ckpt(const void *const *u_in, void *const *u_out, int ckpt, hipStream_t stream) = 0;
+    virtual void backward(const void *const *lam_out, void *const *lam_in, double *dparams, hipStream_t stream) = 0;
+    virtual void invalidate_adjoint() = 0;
     virtual void run(const void *const *u_in, void *const *u_out, hipStream_t stream, bool use_graph, bool host) = 0;
     virtual void profile(const void *const *u_in, void *const *u_out, hipStream_t stream, double *ms, double *bytes,
                          int64_t *launches) = 0;
@@ -156,13 +175,16 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     dev_buffer mem;                    // all problems
     dev_buffer probs_dev, tiles_dev;
     // tile tables: rows [0,nx), cols [0,ny), S-lines [1,ny-2], v-lines [1,nx-2], interior rows
-    enum { TB_I = 0, TB_J, TB_X, TB_V, TB_R, TB_N };
+    // adjoint: S-line tiles of P_XTJ, v-line tiles of P_VROWS, one tile per row (all rows)
+    enum { TB_I = 0, TB_J, TB_X, TB_V, TB_R, TB_AX, TB_AV, TB_U, TB_N };
     std::vector<Tile> tiles[TB_N];
     size_t tile_off[TB_N] = {};
     std::vector<int> prefix[TB_N];     // prefix[tb][q] = tiles of internal problems [0, q)
     int max_steps = 0;
     int line_solver = LINE_EXACT;
     int pm = 1;                        // partitioned: points per chunk bound (template M)
+    int apm = 1;                       // the same bound for the adjoint's transposed solves
+    bool adj_ok = false;               // every line fits the partitioned solver (needed by the adjoint)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -216,13 +238,16 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             require(descs[p].line_solver == descs[0].line_solver, "all problems of a plan share line_solver");
         }
         line_solver = descs[0].line_solver == LSS_LINE_PARTITIONED ? LINE_PARTITIONED : LINE_EXACT;
-        if (line_solver == LINE_PARTITIONED) {
+        {
             int64_t mmax = 1;
             for (int64_t q = 0; q < n; ++q)
                 mmax = std::max({mmax, (descs[q].nx - 2 + P_NCH - 1) / P_NCH, (descs[q].ny - 2 + P_NCH - 1) / P_NCH});
-            pm = 1;
-            while (pm < mmax)
-                pm *= 2;
+            apm = 1;
+            while (apm < mmax)
+                apm *= 2;
+            adj_ok = apm <= 16;
+            if (line_solver == LINE_PARTITIONED)
+                pm = apm;
         }
         order.resize(n);
         std::iota(order.begin(), order.end(), 0);
@@ -329,6 +354,9 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 case TB_J: lo = 0, hi = p.ny; break;
                 case TB_X: lo = 1, hi = p.ny - 1, step = line_solver == LINE_PARTITIONED ? P_XTJ : TILE; break;
                 case TB_V: lo = 1, hi = p.nx - 1, step = line_solver == LINE_PARTITIONED ? P_VROWS : TILE; break;
+                case TB_AX: lo = 1, hi = p.ny - 1, step = P_XTJ; break;
+                case TB_AV: lo = 1, hi = p.nx - 1, step = P_VROWS; break;
+                case TB_U: lo = 0, hi = p.nx, step = 1; break;
                 default: lo = 1, hi = p.nx - 1, step = 1; break;
                 }
                 for (int s = lo; s < hi; s += step) {
@@ -391,10 +419,12 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         return b * sizeof(T);
     }
 
-    void launch_step(int s, hipStream_t st, Profiler *prof = nullptr)
+    // one loop step of every active problem; `table` replaces the plan's problem table (the adjoint's
+    // recomputation writes each step's stage outputs into their own buffers)
+    void launch_step(int s, hipStream_t st, Profiler *prof = nullptr, const Prob<T> *table = nullptr)
     {
         const int64_t na = active(s);
-        const Prob<T> *pd = (const Prob<T> *)probs_dev.p;
+        const Prob<T> *pd = table ? table : (const Prob<T> *)probs_dev.p;
         const Tile *td = (const Tile *)tiles_dev.p;
         auto grid = [&](int tb) { return (unsigned)prefix[tb][na]; };
         bool any_split2 = false, any_hv = false;
@@ -463,6 +493,334 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         pre(LSS_K_BND_B);
         hipLaunchKernelGGL(k_bnd_b<T>, dim3(grid(TB_J)), dim3(TILE), 0, st, pd, td + tile_off[TB_J], s);
         post();
+    }
+
+    // ---------------------------------------------------------------------------------------------
+    // Reverse mode: checkpointed forward, then segment by segment (last first) a recomputation that
+    // keeps every stage output of the segment, followed by the adjoint steps of the segment.
+    // ---------------------------------------------------------------------------------------------
+    int ck = 0;                                   // checkpoint interval of the adjoint allocation
+    bool fwd_done = false;
+    dev_buffer adj_mem, adj_dev, seg_dev, slots_dev, jobs_dev, lam_jobs_dev;
+    std::vector<Adj<T>> adjs;                     // host copy (device pointers inside)
+    std::vector<T *> ckpt_base, seg_lv_base;      // per internal problem
+    std::vector<size_t> fwd_job_off, bwd_job_off; // per checkpoint: first job in jobs_dev
+    std::vector<int> fwd_job_n, bwd_job_n;        // per checkpoint: number of jobs
+    int64_t max_nxy = 0;
+    size_t slot_off[5] = {};                      // per adjoint kernel kind, in units of NPAR doubles
+    size_t nslots = 0;
+    const int slot_tb[5] = {TB_AV, TB_AX, TB_AV, TB_AX, TB_U};
+
+    void invalidate_adjoint() override
+    {
+        ck = 0;
+        fwd_done = false;
+    }
+
+    const Prob<T> *seg_table(int k) const { return (const Prob<T> *)seg_dev.p + (size_t)k * count; }
+
+    void adj_setup(int K)
+    {
+        require(adj_ok, "the adjoint needs nx - 2 <= 1024 and ny - 2 <= 1024 for every problem");
+        require(K >= 2 && K % 2 == 0 && K <= 4096, "ckpt_every must be even, >= 2 and <= 4096");
+        ck = 0;
+        fwd_done = false;
+        const int64_t n = count;
+        // sizes
+        std::vector<size_t> off_T(n), n_T(n), off_U(n);
+        size_t total = 0;
+        max_nxy = 0;
+        for (int64_t q = 0; q < n; ++q) {
+            const Prob<T> &p = probs[q];
+            const size_t nxy = (size_t)p.nx * p.ny, nx_ = (size_t)p.nx, ny_ = (size_t)p.ny;
+            const bool dr = p.scheme == SCHEME_DR, hv = p.scheme == SCHEME_HV;
+            const size_t nys = (dr ? 1 : 3) + ((hv && line_solver == LINE_EXACT) ? 1 : 0);
+            const size_t nl = dr ? 4 : 6;
+            const size_t nck = ((size_t)p.steps + K - 1) / K;
+            const Terms *tm = terms.empty() ? nullptr : &terms[order[q]];
+            const size_t nfac = tm ? tm->fac.size() / sizeof(T) : 0;
+            auto al = [](size_t k) { return (k + 63) / 64 * 64; };
+            size_t t = nl * al(nxy) + 2 * al(nx_) + nck * al(nxy) + (K + 1) * al(nxy) + (size_t)K * nys * al(nxy) +
+                       al(nfac);
+            off_T[q] = total;
+            n_T[q] = t;
+            total += t * sizeof(T);
+            off_U[q] = total;
+            total += ((2 * (nx_ + ny_) + 2) * 4 + 255) / 256 * 256;
+            max_nxy = std::max<int64_t>(max_nxy, (int64_t)nxy);
+        }
+        adj_mem.~dev_buffer();
+        new (&adj_mem) dev_buffer();
+        adj_mem.alloc(total);
+        HIPCHK(hipMemset(adj_mem.p, 0, total));
+        char *base = (char *)adj_mem.p;
+        adjs.assign(n, Adj<T>{});
+        ckpt_base.assign(n, nullptr);
+        seg_lv_base.assign(n, nullptr);
+        std::vector<T *> seg_y_base(n, nullptr);
+        for (int64_t q = 0; q < n; ++q) {
+            const Prob<T> &p = probs[q];
+            const size_t nxy = (size_t)p.nx * p.ny, nx_ = (size_t)p.nx, ny_ = (size_t)p.ny;
+            const bool dr = p.scheme == SCHEME_DR, hv = p.scheme == SCHEME_HV;
+            const size_t nys = (dr ? 1 : 3) + ((hv && line_solver == LINE_EXACT) ? 1 : 0);
+            const size_t nck = ((size_t)p.steps + K - 1) / K;
+            auto al = [](size_t k) { return (k + 63) / 64 * 64; };
+            T *t = (T *)(base + off_T[q]);
+            auto take = [&](size_t k) {
+                T *r = t;
+                t += al(k);
+                return r;
+            };
+            Adj<T> &A = adjs[q];
+            A.la = take(nxy);
+            A.lb = take(nxy);
+            A.lv2 = take(nxy);
+            A.lx1 = take(nxy);
+            A.lx2 = dr ? nullptr : take(nxy);
+            A.lv1 = dr ? nullptr : take(nxy);
+            A.elo2 = take(nx_);
+            A.elo1 = take(nx_);
+            ckpt_base[q] = take(nck * nxy);
+            seg_lv_base[q] = take((K + 1) * nxy);
+            seg_y_base[q] = take((size_t)K * nys * nxy);
+            const Terms *tm = terms.empty() ? nullptr : &terms[order[q]];
+            const size_t nfac = tm ? tm->fac.size() / sizeof(T) : 0;
+            T *fac = take(nfac);
+            if (tm && tm->nterm > 0) {
+                HIPCHK(hipMemcpy(fac, tm->fac.data(), tm->fac.size(), hipMemcpyHostToDevice));
+                A.dfac = fac;
+                A.dfac_sets = (int32_t)tm->sets;
+                A.dfac_stride = (int64_t)tm->nterm * (int64_t)(nx_ + ny_);
+                A.nterm = tm->nterm;
+                for (int z = 0; z < NTERM; ++z) {
+                    A.tp[z] = tm->tp[z];
+                    A.tq[z] = tm->tq[z];
+                }
+            }
+            // inverse index maps
+            std::vector<uint32_t> h1, h2;
+            h1.resize(probs[q].nx);
+            h2.resize(probs[q].ny);
+            HIPCHK(hipMemcpy(h1.data(), p.idx1, nx_ * 4, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(h2.data(), p.idx2, ny_ * 4, hipMemcpyDeviceToHost));
+            auto inverse = [](const std::vector<uint32_t> &m, std::vector<uint32_t> &o, std::vector<uint32_t> &ix) {
+                const size_t k = m.size();
+                o.assign(k + 1, 0);
+                for (uint32_t v : m)
+                    o[v + 1] += 1;
+                for (size_t z = 0; z < k; ++z)
+                    o[z + 1] += o[z];
+                ix.assign(k, 0);
+                std::vector<uint32_t> fill(o.begin(), o.end() - 1);
+                for (size_t z = 0; z < k; ++z)
+                    ix[fill[m[z]]++] = (uint32_t)z;
+            };
+            std::vector<uint32_t> o1, i1, o2, i2;
+            inverse(h1, o1, i1);
+            inverse(h2, o2, i2);
+            uint32_t *u = (uint32_t *)(base + off_U[q]);
+            HIPCHK(hipMemcpy(u, o1.data(), o1.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(u + o1.size(), i1.data(), i1.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(u + o1.size() + i1.size(), o2.data(), o2.size() * 4, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(u + o1.size() + i1.size() + o2.size(), i2.data(), i2.size() * 4, hipMemcpyHostToDevice));
+            A.inv1o = u;
+            A.inv1i = u + o1.size();
+            A.inv2o = u + o1.size() + i1.size();
+            A.inv2i = u + o1.size() + i1.size() + o2.size();
+        }
+        adj_dev.~dev_buffer();
+        new (&adj_dev) dev_buffer();
+        adj_dev.alloc(n * sizeof(Adj<T>));
+        HIPCHK(hipMemcpy(adj_dev.p, adjs.data(), n * sizeof(Adj<T>), hipMemcpyHostToDevice));
+        // per-step problem tables of a segment: step s0 + k reads level slot k, writes slot k + 1
+        std::vector<Prob<T>> seg((size_t)K * n);
+        for (int k = 0; k < K; ++k)
+            for (int64_t q = 0; q < n; ++q) {
+                Prob<T> t = probs[q];
+                const size_t nxy = (size_t)t.nx * t.ny;
+                T *lk = seg_lv_base[q] + (size_t)k * nxy, *lk1 = lk + nxy;
+                if (k % 2 == 0) {
+                    t.a = lk;
+                    t.b = lk1;
+                } else {
+                    t.b = lk;
+                    t.a = lk1;
+                }
+                const bool dr = t.scheme == SCHEME_DR, hv = t.scheme == SCHEME_HV;
+                const size_t nys = (dr ? 1 : 3) + ((hv && line_solver == LINE_EXACT) ? 1 : 0);
+                T *yb = seg_y_base[q] + (size_t)k * nys * nxy;
+                t.y1 = yb;
+                t.y2 = dr ? nullptr : yb + nxy;
+                t.y3 = dr ? nullptr : yb + 2 * nxy;
+                t.y4 = line_solver == LINE_PARTITIONED ? t.y3 : (hv ? yb + 3 * nxy : t.y1);
+                seg[(size_t)k * n + q] = t;
+            }
+        seg_dev.~dev_buffer();
+        new (&seg_dev) dev_buffer();
+        seg_dev.alloc(seg.size() * sizeof(Prob<T>));
+        HIPCHK(hipMemcpy(seg_dev.p, seg.data(), seg.size() * sizeof(Prob<T>), hipMemcpyHostToDevice));
+        // checkpoint copy jobs
+        const int nck_max = (max_steps + K - 1) / K;
+        std::vector<CopyJob> jobs;
+        fwd_job_off.assign(nck_max, 0);
+        bwd_job_off.assign(nck_max, 0);
+        fwd_job_n.assign(nck_max, 0);
+        bwd_job_n.assign(nck_max, 0);
+        for (int c = 0; c < nck_max; ++c) {
+            fwd_job_off[c] = jobs.size();
+            for (int64_t q = 0; q < n; ++q)
+                if (probs[q].steps > c * K)
+                    jobs.push_back({probs[q].a, ckpt_base[q] + (size_t)c * probs[q].nx * probs[q].ny,
+                                    (int64_t)probs[q].nx * probs[q].ny});
+            fwd_job_n[c] = (int)(jobs.size() - fwd_job_off[c]);
+            bwd_job_off[c] = jobs.size();
+            for (int64_t q = 0; q < n; ++q)
+                if (probs[q].steps > c * K)
+                    jobs.push_back({ckpt_base[q] + (size_t)c * probs[q].nx * probs[q].ny, seg_lv_base[q],
+                                    (int64_t)probs[q].nx * probs[q].ny});
+            bwd_job_n[c] = (int)(jobs.size() - bwd_job_off[c]);
+        }
+        jobs_dev.~dev_buffer();
+        new (&jobs_dev) dev_buffer();
+        jobs_dev.alloc(jobs.size() * sizeof(CopyJob));
+        HIPCHK(hipMemcpy(jobs_dev.p, jobs.data(), jobs.size() * sizeof(CopyJob), hipMemcpyHostToDevice));
+        lam_jobs_dev.~dev_buffer();
+        new (&lam_jobs_dev) dev_buffer();
+        lam_jobs_dev.alloc(2 * n * sizeof(CopyJob));
+        // parameter slots
+        nslots = 0;
+        for (int k = 0; k < 5; ++k) {
+            slot_off[k] = nslots;
+            nslots += tiles[slot_tb[k]].size();
+        }
+        slots_dev.~dev_buffer();
+        new (&slots_dev) dev_buffer();
+        slots_dev.alloc(nslots * NPAR * sizeof(double));
+        ck = K;
+    }
+
+    void launch_jobs(const CopyJob *jd, int njobs, hipStream_t st)
+    {
+        if (njobs <= 0)
+            return;
+        const unsigned bx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (max_nxy + 255) / 256));
+        for (int first = 0; first < njobs; first += 65535) {
+            const int nj = std::min(65535, njobs - first);
+            hipLaunchKernelGGL(k_copy_jobs<T>, dim3(bx, (unsigned)nj), dim3(256), 0, st, jd + first);
+        }
+    }
+
+    template <int M> void adj_launch(int s, const Prob<T> *table, hipStream_t st)
+    {
+        const int64_t na = active(s);
+        auto grid = [&](int tb) { return (unsigned)prefix[tb][na]; };
+        bool any_split2 = false;
+        for (int64_t q = 0; q < na; ++q)
+            any_split2 |= probs[q].scheme != SCHEME_DR;
+        const Adj<T> *ad = (const Adj<T> *)adj_dev.p;
+        const Tile *td = (const Tile *)tiles_dev.p;
+        double *sl = (double *)slots_dev.p;
+        const dim3 bv(64 * P_VROWS), bx(P_XTJ * P_NCH), bu(256);
+        hipLaunchKernelGGL((k_adj_v<T, M>), dim3(grid(TB_AV)), bv, 0, st, table, ad, td + tile_off[TB_AV], s, 1,
+                           sl + slot_off[0] * NPAR);
+        if (any_split2) {
+            hipLaunchKernelGGL((k_adj_x<T, M>), dim3(grid(TB_AX)), bx, 0, st, table, ad, td + tile_off[TB_AX], s, 2,
+                               sl + slot_off[1] * NPAR);
+            hipLaunchKernelGGL((k_adj_v<T, M>), dim3(grid(TB_AV)), bv, 0, st, table, ad, td + tile_off[TB_AV], s, 0,
+                               sl + slot_off[2] * NPAR);
+        }
+        hipLaunchKernelGGL((k_adj_x<T, M>), dim3(grid(TB_AX)), bx, 0, st, table, ad, td + tile_off[TB_AX], s, 1,
+                           sl + slot_off[3] * NPAR);
+        hipLaunchKernelGGL(k_adj_u<T>, dim3(grid(TB_U)), bu, 0, st, table, ad, td + tile_off[TB_U], s,
+                           sl + slot_off[4] * NPAR);
+    }
+
+    void launch_adj_step(int s, const Prob<T> *table, hipStream_t st)
+    {
+        switch (apm) {
+        case 1: adj_launch<1>(s, table, st); break;
+        case 2: adj_launch<2>(s, table, st); break;
+        case 4: adj_launch<4>(s, table, st); break;
+        case 8: adj_launch<8>(s, table, st); break;
+        default: adj_launch<16>(s, table, st); break;
+        }
+    }
+
+    void forward_ckpt(const void *const *u_in, void *const *u_out, int K, hipStream_t stream) override
+    {
+        require(u_in && u_out, "null pointer array");
+        for (int64_t c = 0; c < count; ++c)
+            require(u_in[c] && u_out[c], "null surface pointer");
+        if (ck != K)
+            adj_setup(K);
+        fwd_done = false;
+        for (int64_t q = 0; q < count; ++q) {
+            const Prob<T> &p = probs[q];
+            HIPCHK(hipMemcpyAsync(p.a, u_in[order[q]], sizeof(T) * (size_t)p.nx * p.ny, hipMemcpyDeviceToDevice,
+                                  stream));
+        }
+        const CopyJob *jd = (const CopyJob *)jobs_dev.p;
+        for (int s = 0; s < max_steps; ++s) {
+            if (s % K == 0)
+                launch_jobs(jd + fwd_job_off[s / K], fwd_job_n[s / K], stream);
+            launch_step(s, stream);
+        }
+        HIPCHK(hipGetLastError());
+        for (int64_t q = 0; q < count; ++q) {
+            const Prob<T> &p = probs[q];
+            const T *res = (p.steps & 1) ? p.b : p.a;
+            HIPCHK(hipMemcpyAsync(u_out[order[q]], res, sizeof(T) * (size_t)p.nx * p.ny, hipMemcpyDeviceToDevice,
+                                  stream));
+        }
+        fwd_done = true;
+    }
+
+    void backward(const void *const *lam_out, void *const *lam_in, double *dparams, hipStream_t stream) override
+    {
+        require(fwd_done && ck > 0, "lss_adi_backward needs a preceding lss_adi_forward on this plan");
+        require(lam_out && dparams, "null lam_out or dparams");
+        for (int64_t c = 0; c < count; ++c)
+            require(lam_out[c], "null lambda pointer");
+        const int K = ck;
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipMemsetAsync(slots_dev.p, 0, nslots * NPAR * sizeof(double), stream));
+        std::vector<CopyJob> lj;
+        for (int64_t q = 0; q < count; ++q) {
+            const Prob<T> &p = probs[q];
+            lj.push_back({lam_out[order[q]], (p.steps & 1) ? adjs[q].lb : adjs[q].la, (int64_t)p.nx * p.ny});
+        }
+        const size_t nin = lj.size();
+        if (lam_in)
+            for (int64_t q = 0; q < count; ++q)
+                if (lam_in[order[q]])
+                    lj.push_back({adjs[q].la, lam_in[order[q]], (int64_t)probs[q].nx * probs[q].ny});
+        HIPCHK(hipMemcpy(lam_jobs_dev.p, lj.data(), lj.size() * sizeof(CopyJob), hipMemcpyHostToDevice));
+        const CopyJob *ljd = (const CopyJob *)lam_jobs_dev.p;
+        launch_jobs(ljd, (int)nin, stream);
+        const CopyJob *jd = (const CopyJob *)jobs_dev.p;
+        const int nseg = (max_steps + K - 1) / K;
+        for (int c = nseg - 1; c >= 0; --c) {
+            const int s0 = c * K, s1 = std::min(max_steps, s0 + K);
+            launch_jobs(jd + bwd_job_off[c], bwd_job_n[c], stream);
+            for (int s = s0; s < s1; ++s)
+                launch_step(s, stream, nullptr, seg_table(s - s0));
+            for (int s = s1 - 1; s >= s0; --s)
+                launch_adj_step(s, seg_table(s - s0), stream);
+        }
+        launch_jobs(ljd + nin, (int)(lj.size() - nin), stream);
+        HIPCHK(hipGetLastError());
+        std::vector<double> sl(nslots * NPAR);
+        HIPCHK(hipMemcpyAsync(sl.data(), slots_dev.p, sl.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        std::fill(dparams, dparams + count * NPAR, 0.0);
+        for (int k = 0; k < 5; ++k) {
+            const auto &tt = tiles[slot_tb[k]];
+            for (size_t t = 0; t < tt.size(); ++t) {
+                const int64_t c = order[tt[t].prob];
+                for (int z = 0; z < NPAR; ++z)
+                    dparams[c * NPAR + z] += sl[(slot_off[k] + t) * NPAR + z];
+            }
+        }
     }
 
     void launch_loop(hipStream_t st)
@@ -677,6 +1035,80 @@ void separable_set(const lss_model &m, const lss_amd_detail::grid_spec<T> &g,
     }
 }
 
+// Derivative terms of the transformed coefficients with respect to the model parameters
+// (transform of lss_heat_data_transform.hpp:174-214 applied to the derivatives of the closures):
+//   Heston (kappa, theta_v, sigma, rho, r):
+//     e' = (theta_v - v) / Y', kappa / Y';  b' = sigma v / Y'^2, c' = rho v S / (X' Y'),
+//     e' = -sigma v Y'' / Y'^3;  c' = sigma v S / (X' Y');  d' = S / X', f' = -1
+//   SABR (nu, rho, beta), D = exp(-r (T - t)):
+//     b' = nu al^2 / Y'^2, c' = rho al^2 cX / (X' Y'), e' = -nu al^2 Y'' / Y'^3;  c' = nu al^2 cX / (X' Y');
+//     a' = aX_b al^2 / X'^2, c' = rho nu al^2 cX_b / (X' Y'), d' = -aX_b al^2 X'' / X'^3
+//     with aX = F^(2 beta) D^(2 (1 - beta)) / 2, cX = F^beta D^(1 - beta), aX_b = 2 aX ln(F / D),
+//     cX_b = cX ln(F / D) (0 at F = 0).
+template <typename T>
+void param_terms(const lss_model &m, const lss_amd_detail::grid_spec<T> &g, const lss_amd_detail::sampled<T> &s,
+                 T t, lss_adi_plan::Terms &tm, T *out)
+{
+    const lss_amd_detail::grid_map<T> gm(g);
+    const size_t nx = g.nx, ny = g.ny, L = nx + ny;
+    const T h_1 = T(1) / static_cast<T>(nx - 1), h_2 = T(1) / static_cast<T>(ny - 1);
+    const double r = m.rate, sig = m.sig_sig, rho = m.rho;
+    std::vector<double> S(nx), dX(nx), ddX(nx), V(ny), dY(ny), ddY(ny);
+    for (size_t i = 0; i < nx; ++i) {
+        const T z = h_1 * static_cast<T>(i);
+        S[i] = s.xs[i];
+        dX[i] = gm.dx(z);
+        ddX[i] = gm.ddx(z);
+    }
+    for (size_t j = 0; j < ny; ++j) {
+        const T e = h_2 * static_cast<T>(j);
+        V[j] = s.ys[j];
+        dY[j] = gm.dy(e);
+        ddY[j] = gm.ddy(e);
+    }
+    int nt = 0;
+    auto term = [&](int par, int q, auto fx, auto fy) {
+        tm.tp[nt] = (int8_t)par;
+        tm.tq[nt] = (int8_t)q;
+        T *x = out + (size_t)nt * L, *y = x + nx;
+        for (size_t i = 0; i < nx; ++i)
+            x[i] = (T)fx(i);
+        for (size_t j = 0; j < ny; ++j)
+            y[j] = (T)fy(j);
+        ++nt;
+    };
+    auto one = [](size_t) { return 1.0; };
+    if (m.kind == LSS_MODEL_HESTON) {
+        const double kap = m.sig_kappa, th = m.sig_theta;
+        auto sx = [&](size_t i) { return S[i] / dX[i]; };
+        term(0, 4, one, [&](size_t j) { return (th - V[j]) / dY[j]; });
+        term(1, 4, one, [&](size_t j) { return kap / dY[j]; });
+        term(2, 1, one, [&](size_t j) { return sig * V[j] / (dY[j] * dY[j]); });
+        term(2, 2, sx, [&](size_t j) { return rho * V[j] / dY[j]; });
+        term(2, 4, one, [&](size_t j) { return -sig * V[j] * ddY[j] / (dY[j] * dY[j] * dY[j]); });
+        term(3, 2, sx, [&](size_t j) { return sig * V[j] / dY[j]; });
+        term(4, 3, sx, one);
+        term(4, 5, one, [](size_t) { return -1.0; });
+        tm.nparam = 5;
+    } else {
+        const double be = m.beta, lnD = -r * (m.maturity - (double)t);
+        const double Dt = std::exp(lnD);
+        auto aX = [&](size_t i) { return 0.5 * std::pow(S[i], 2.0 * be) * std::pow(Dt, 2.0 * (1.0 - be)); };
+        auto cX = [&](size_t i) { return std::pow(S[i], be) * std::pow(Dt, 1.0 - be); };
+        auto lnFD = [&](size_t i) { return S[i] > 0.0 ? std::log(S[i]) - lnD : 0.0; };
+        auto a2 = [&](size_t j) { return V[j] * V[j]; };
+        term(0, 1, one, [&](size_t j) { return sig * a2(j) / (dY[j] * dY[j]); });
+        term(0, 2, [&](size_t i) { return cX(i) / dX[i]; }, [&](size_t j) { return rho * a2(j) / dY[j]; });
+        term(0, 4, one, [&](size_t j) { return -sig * a2(j) * ddY[j] / (dY[j] * dY[j] * dY[j]); });
+        term(1, 2, [&](size_t i) { return cX(i) / dX[i]; }, [&](size_t j) { return sig * a2(j) / dY[j]; });
+        term(2, 0, [&](size_t i) { return 2.0 * aX(i) * lnFD(i) / (dX[i] * dX[i]); }, a2);
+        term(2, 2, [&](size_t i) { return cX(i) * lnFD(i) / dX[i]; }, [&](size_t j) { return rho * sig * a2(j) / dY[j]; });
+        term(2, 3, [&](size_t i) { return -2.0 * aX(i) * lnFD(i) * ddX[i] / (dX[i] * dX[i] * dX[i]); }, a2);
+        tm.nparam = 3;
+    }
+    tm.nterm = nt;
+}
+
 template <typename T> struct model_samples {
     std::vector<lss_amd_detail::sampled<T>> s;
     std::vector<lss_adi_desc> d;
@@ -718,11 +1150,36 @@ template <typename T> lss_adi_plan *model_plan(const lss_model *models, int64_t 
         ms.d[q] = s.desc(m.coeff_mode, m.line_solver);
     }
     lss_adi_plan *plan = make_plan(sizeof(T) == 8 ? LSS_FP64 : LSS_FP32, ms.d.data(), n);
+    plan->terms.resize(n);
+    for (int64_t q = 0; q < n; ++q) {
+        const lss_model &m = models[q];
+        const auto g = spec_of<T>(m);
+        const auto &sm = ms.s[q];
+        auto &tm = plan->terms[q];
+        const bool tdep = m.kind == LSS_MODEL_SABR && m.rate != 0.0;
+        tm.sets = tdep ? sm.steps : 1;
+        const size_t L = (size_t)(m.nx + m.ny), per = (size_t)NTERM * L;
+        std::vector<T> fac(per * tm.sets, (T)0);
+        for (int64_t k = 0; k < tm.sets; ++k)
+            param_terms(m, g, sm, sm.times[k], tm, fac.data() + k * per);
+        // keep only the used terms of each set
+        std::vector<T> used((size_t)tm.nterm * L * tm.sets);
+        for (int64_t k = 0; k < tm.sets; ++k)
+            std::copy(fac.begin() + k * per, fac.begin() + k * per + (size_t)tm.nterm * L,
+                      used.begin() + k * (size_t)tm.nterm * L);
+        tm.fac.resize(used.size() * sizeof(T));
+        std::memcpy(tm.fac.data(), used.data(), tm.fac.size());
+    }
     plan->init_host.resize(n);
+    plan->grid_host.resize(n);
     for (int64_t q = 0; q < n; ++q) {
         const auto &v = ms.s[q].init;
         plan->init_host[q].resize(v.size() * sizeof(T));
         std::memcpy(plan->init_host[q].data(), v.data(), v.size() * sizeof(T));
+        const auto &xs = ms.s[q].xs, &ys = ms.s[q].ys;
+        plan->grid_host[q].resize((xs.size() + ys.size()) * sizeof(T));
+        std::memcpy(plan->grid_host[q].data(), xs.data(), xs.size() * sizeof(T));
+        std::memcpy(plan->grid_host[q].data() + xs.size() * sizeof(T), ys.data(), ys.size() * sizeof(T));
     }
     return plan;
 }
@@ -825,6 +1282,18 @@ int lss_model_initial(const lss_adi_plan *plan, int64_t p, void *u0_host)
     });
 }
 
+int lss_model_grid(const lss_adi_plan *plan, int64_t p, void *xs_host, void *ys_host)
+{
+    return guarded([&] {
+        require(plan && xs_host && ys_host, "null pointer");
+        require(p >= 0 && p < plan->count, "problem index out of range");
+        require(!plan->grid_host.empty(), "not a model plan");
+        const size_t es = plan->fp == LSS_FP64 ? 8 : 4, nxb = (size_t)plan->nx[p] * es;
+        std::memcpy(xs_host, plan->grid_host[p].data(), nxb);
+        std::memcpy(ys_host, plan->grid_host[p].data() + nxb, (size_t)plan->ny[p] * es);
+    });
+}
+
 int lss_plan_problem_shape(const lss_adi_plan *plan, int64_t p, int64_t *nx, int64_t *ny, int64_t *steps)
 {
     return guarded([&] {
@@ -836,6 +1305,63 @@ int lss_plan_problem_shape(const lss_adi_plan *plan, int64_t p, int64_t *nx, int
             *ny = plan->ny[p];
         if (steps)
             *steps = plan->steps[p];
+    });
+}
+
+int lss_adi_set_param_terms(lss_adi_plan *plan, int64_t p, int32_t nparam, int32_t nterm, const int32_t *term_param,
+                            const int32_t *term_coeff, int64_t sets, const void *factors)
+{
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        require(p >= 0 && p < plan->count, "problem index out of range");
+        require(nparam >= 0 && nparam <= LSS_MAX_PARAMS, "nparam must be in [0, LSS_MAX_PARAMS]");
+        require(nterm >= 0 && nterm <= LSS_MAX_TERMS, "nterm must be in [0, LSS_MAX_TERMS]");
+        require(sets == 1 || sets == plan->steps[p], "sets must be 1 or the problem's step count");
+        require(nterm == 0 || (term_param && term_coeff && factors), "null term array");
+        if (plan->terms.empty())
+            plan->terms.resize(plan->count);
+        lss_adi_plan::Terms tm;
+        tm.nparam = nparam;
+        tm.nterm = nterm;
+        tm.sets = sets;
+        for (int t = 0; t < nterm; ++t) {
+            require(term_param[t] >= 0 && term_param[t] < nparam, "term parameter out of range");
+            require(term_coeff[t] >= LSS_Q_A && term_coeff[t] <= LSS_Q_F, "term coefficient out of range");
+            tm.tp[t] = (int8_t)term_param[t];
+            tm.tq[t] = (int8_t)term_coeff[t];
+        }
+        const size_t es = plan->fp == LSS_FP64 ? 8 : 4;
+        tm.fac.resize((size_t)sets * nterm * (size_t)(plan->nx[p] + plan->ny[p]) * es);
+        if (!tm.fac.empty())
+            std::memcpy(tm.fac.data(), factors, tm.fac.size());
+        plan->terms[p] = std::move(tm);
+        plan->invalidate_adjoint();
+    });
+}
+
+int lss_plan_param_count(const lss_adi_plan *plan, int64_t p, int32_t *nparam)
+{
+    return guarded([&] {
+        require(plan && nparam, "null pointer");
+        require(p >= 0 && p < plan->count, "problem index out of range");
+        *nparam = plan->terms.empty() ? 0 : plan->terms[p].nparam;
+    });
+}
+
+int lss_adi_forward(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, int32_t ckpt_every, void *stream)
+{
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        plan->forward_ckpt(u_in, u_out, ckpt_every, (hipStream_t)stream);
+    });
+}
+
+int lss_adi_backward(lss_adi_plan *plan, const void *const *lam_out, void *const *lam_in, double *dparams,
+                     void *stream)
+{
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        plan->backward(lam_out, lam_in, dparams, (hipStream_t)stream);
     });
 }
 

@@ -1,0 +1,285 @@
+// lss_adi_chunk.hpp -- register-resident partitioned tridiagonal elimination shared by the forward
+// line sweeps and their adjoints.
+//
+// A line with interior unknowns 1..N is cut into NCH = 64 chunks of at most M consecutive points.
+// Chunk l owns points [s, e]; its last point e is the chunk's interface unknown xi_l.  With xi_{l-1}
+// (left) and xi_l (right) as Dirichlet values, the chunk's interior points s..e-1 are eliminated by
+// Thomas with three right-hand sides (the data and the two spikes):
+//     x_k = p_k + q_k xi_{l-1} + r_k xi_l            (k = s..e-1, held in registers)
+// Substituting the last interior point of chunk l and the first interior point of chunk l+1 into the
+// equation of point e gives one tridiagonal equation per chunk in xi_{l-1}, xi_l, xi_{l+1}; that
+// 64-unknown interface system is solved by parallel cyclic reduction (wave shuffles or LDS), after
+// which every chunk finishes its points.
+#pragma once
+
+#include "lss_adi_device.hpp"
+
+namespace lssamd {
+
+constexpr int NCH = 64;   // chunks per line
+constexpr int XTJ = 16;   // S-lines per x-sweep workgroup
+constexpr int VROWS = 4;  // v-lines (wavefronts) per v-sweep workgroup
+
+template <typename T> struct Eq {
+    T a, b, c, r;
+};
+
+// Register-resident chunk elimination.  After fwd()/bwd(): g holds r_k, zp holds p_k, zq holds q_k
+// for the cnt - 1 interior points, and (ae, be, ce, re) is the equation of the chunk's last point.
+template <typename T, int M> struct Chunk {
+    T g[M], zp[M], zq[M];
+    T ae, be, ce, re;
+    int cnt;
+
+    // Forward elimination of point t (a compile-time index after unrolling).  The array entries are
+    // written unconditionally, so no two stores share an address computed at run time (which would
+    // move the arrays to scratch memory); the entries of the chunk's last point are never read.
+    __device__ __forceinline__ void put(int t, const Eq<T> &q)
+    {
+#pragma unroll
+        for (int u = 0; u < M; ++u) {
+            if (u == t) {
+                const T rb = (T)1 / (u == 0 ? q.b : q.b - q.a * g[u > 0 ? u - 1 : 0]);
+                g[u] = q.c * rb;
+                zp[u] = (u == 0 ? q.r : q.r - q.a * zp[u > 0 ? u - 1 : 0]) * rb;
+                zq[u] = (u == 0 ? -q.a : -q.a * zq[u > 0 ? u - 1 : 0]) * rb;
+            }
+        }
+        if (t == cnt - 1) {
+            ae = q.a;
+            be = q.b;
+            ce = q.c;
+            re = q.r;
+        }
+    }
+
+    // Backward substitution of the spikes.  Every array element is addressed with a compile-time
+    // index (a runtime index would move the arrays to scratch memory); (pl, ql, rl) keep the
+    // relation of the last interior point, whose position depends on cnt.
+    T pl, ql, rl;
+
+    __device__ __forceinline__ void bwd()
+    {
+        const int ni = cnt - 1;
+        pl = (T)0;
+        ql = (T)1;
+        rl = (T)0;
+#pragma unroll
+        for (int u = M - 1; u >= 0; --u) {
+            const bool valid = u < ni, lastp = u == ni - 1;
+            const T gu = g[u];
+            T gn = -gu, pn = zp[u], qn = zq[u];
+            if (u + 1 < M) {
+                const T g1 = g[u + 1], p1 = zp[u + 1], q1 = zq[u + 1];
+                gn = lastp ? -gu : -gu * g1;
+                pn = lastp ? zp[u] : zp[u] - gu * p1;
+                qn = lastp ? zq[u] : zq[u] - gu * q1;
+            }
+            g[u] = valid ? gn : gu;
+            zp[u] = valid ? pn : zp[u];
+            zq[u] = valid ? qn : zq[u];
+            pl = lastp ? pn : pl;
+            ql = lastp ? qn : ql;
+            rl = lastp ? gn : rl;
+        }
+    }
+
+    // (p, q, r) of the first and of the last interior point; a chunk of one point has none:
+    // then x_{e-1} = xi_{l-1} (0, 1, 0) and, seen from the left neighbour, x_s = xi_l (0, 0, 1).
+    __device__ __forceinline__ void first_rel(T &P, T &Q, T &R) const
+    {
+        if (cnt >= 2) {
+            P = zp[0];
+            Q = zq[0];
+            R = g[0];
+        } else {
+            P = (T)0;
+            Q = (T)0;
+            R = (T)1;
+        }
+    }
+    __device__ __forceinline__ void last_rel(T &P, T &Q, T &R) const
+    {
+        P = pl;
+        Q = ql;
+        R = rl;
+    }
+
+    // interface equation of chunk l, given the next chunk's first relation (PF, QF, RF)
+    __device__ __forceinline__ Eq<T> interface_eq(T PF, T QF, T RF) const
+    {
+        T PL, QL, RL;
+        last_rel(PL, QL, RL);
+        Eq<T> q;
+        q.a = ae * QL;
+        q.b = ae * RL + be + ce * QF;
+        q.c = ce * RF;
+        q.r = re - ae * PL - ce * PF;
+        return q;
+    }
+
+    __device__ __forceinline__ T value(int t, T xl, T xr) const
+    {
+        T v = xr;
+#pragma unroll
+        for (int u = 0; u < M; ++u)
+            if (u == t && u < cnt - 1)
+                v = zp[u] + zq[u] * xl + g[u] * xr;
+        return v;
+    }
+};
+
+// one cyclic-reduction step; lo.b and hi.b carry the reciprocals of the neighbours' diagonals
+template <typename T> __device__ __forceinline__ Eq<T> pcr_combine(const Eq<T> &me, const Eq<T> &lo, const Eq<T> &hi)
+{
+    const T al = me.a * lo.b, ga = me.c * hi.b;
+    Eq<T> n;
+    n.a = -al * lo.a;
+    n.c = -ga * hi.c;
+    n.b = me.b - al * lo.c - ga * hi.a;
+    n.r = me.r - al * lo.r - ga * hi.r;
+    return n;
+}
+
+template <typename T> __device__ __forceinline__ Eq<T> identity_eq()
+{
+    Eq<T> q;
+    q.a = (T)0;
+    q.b = (T)1;
+    q.c = (T)0;
+    q.r = (T)0;
+    return q;
+}
+
+
+// -------------------------------------------------------------------------------------------------
+// Generic line solves.  eq(k) returns the equation of interior unknown k (1..N) with the ends
+// already folded (a = 0 at k = 1, c = 0 at k = N); put(k, x) receives the solution of unknown k.
+// -------------------------------------------------------------------------------------------------
+
+// One line per wavefront; lane = chunk; the interface system is reduced with shuffles.
+// Every lane of the wavefront must call it.
+template <typename T, int M, typename EqF, typename PutF>
+__device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put)
+{
+    const int m = (N + NCH - 1) / NCH;
+    const int s = 1 + lane * m;
+    Chunk<T, M> ch;
+    ch.cnt = max(0, min(m, N - s + 1));
+    const bool last_real = ch.cnt > 0 && s + ch.cnt - 1 == N;
+#pragma unroll
+    for (int t = 0; t < M; ++t)
+        if (t < ch.cnt)
+            ch.put(t, eq(s + t));
+    ch.bwd();
+    T PF, QF, RF;
+    ch.first_rel(PF, QF, RF);
+    T nPF = __shfl_down(PF, 1), nQF = __shfl_down(QF, 1), nRF = __shfl_down(RF, 1);
+    if (last_real || lane == NCH - 1) {
+        nPF = (T)0;
+        nQF = (T)0;
+        nRF = (T)0;
+    }
+    Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
+#pragma unroll
+    for (int d = 1; d < NCH; d <<= 1) {
+        const T rb = (T)1 / e.b;
+        Eq<T> l, h;
+        l.a = __shfl_up(e.a, d);
+        l.b = __shfl_up(rb, d);
+        l.c = __shfl_up(e.c, d);
+        l.r = __shfl_up(e.r, d);
+        h.a = __shfl_down(e.a, d);
+        h.b = __shfl_down(rb, d);
+        h.c = __shfl_down(e.c, d);
+        h.r = __shfl_down(e.r, d);
+        if (lane < d)
+            l = identity_eq<T>();
+        if (lane + d >= NCH)
+            h = identity_eq<T>();
+        e = pcr_combine(e, l, h);
+    }
+    const T xi = e.r / e.b;
+    T xl = __shfl_up(xi, 1);
+    if (lane == 0)
+        xl = (T)0;
+#pragma unroll
+    for (int t = 0; t < M; ++t)
+        if (t < ch.cnt)
+            put(s + t, ch.value(t, xl, xi));
+}
+
+// XTJ lines per workgroup of XTJ * NCH threads (thread = (chunk chn, line col)); the interface
+// system is reduced through LDS `sh`.  Every thread of the workgroup must call it (it holds
+// barriers); threads of lines that do not exist pass active = false.
+template <typename T, int M, typename EqF, typename PutF>
+__device__ __forceinline__ void block_solve(T (*sh)[4][NCH][XTJ], int N, int col, int chn, bool active, EqF &&eq,
+                                            PutF &&put)
+{
+    const int m = (N + NCH - 1) / NCH;
+    const int s = 1 + chn * m;
+    Chunk<T, M> ch;
+    ch.cnt = active ? max(0, min(m, N - s + 1)) : 0;
+    const bool last_real = ch.cnt > 0 && s + ch.cnt - 1 == N;
+#pragma unroll
+    for (int t = 0; t < M; ++t)
+        if (t < ch.cnt)
+            ch.put(t, eq(s + t));
+    ch.bwd();
+    {
+        T PF, QF, RF;
+        ch.first_rel(PF, QF, RF);
+        sh[1][0][chn][col] = PF;
+        sh[1][1][chn][col] = QF;
+        sh[1][2][chn][col] = RF;
+    }
+    __syncthreads();
+    T nPF = (T)0, nQF = (T)0, nRF = (T)0;
+    if (!last_real && chn + 1 < NCH) {
+        nPF = sh[1][0][chn + 1][col];
+        nQF = sh[1][1][chn + 1][col];
+        nRF = sh[1][2][chn + 1][col];
+    }
+    Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
+    int buf = 0;
+    sh[0][0][chn][col] = e.a;
+    sh[0][1][chn][col] = (T)1 / e.b;
+    sh[0][2][chn][col] = e.c;
+    sh[0][3][chn][col] = e.r;
+    __syncthreads();
+#pragma unroll
+    for (int d = 1; d < NCH; d <<= 1) {
+        Eq<T> l = identity_eq<T>(), h = identity_eq<T>();
+        if (chn >= d) {
+            l.a = sh[buf][0][chn - d][col];
+            l.b = sh[buf][1][chn - d][col];
+            l.c = sh[buf][2][chn - d][col];
+            l.r = sh[buf][3][chn - d][col];
+        }
+        if (chn + d < NCH) {
+            h.a = sh[buf][0][chn + d][col];
+            h.b = sh[buf][1][chn + d][col];
+            h.c = sh[buf][2][chn + d][col];
+            h.r = sh[buf][3][chn + d][col];
+        }
+        e = pcr_combine(e, l, h);
+        buf ^= 1;
+        sh[buf][0][chn][col] = e.a;
+        sh[buf][1][chn][col] = (T)1 / e.b;
+        sh[buf][2][chn][col] = e.c;
+        sh[buf][3][chn][col] = e.r;
+        __syncthreads();
+    }
+    const T xi = e.r / e.b;
+    sh[buf ^ 1][0][chn][col] = xi;
+    __syncthreads();
+    const T xl = chn > 0 ? sh[buf ^ 1][0][chn - 1][col] : (T)0;
+    if (!active)
+        return;
+#pragma unroll
+    for (int t = 0; t < M; ++t)
+        if (t < ch.cnt)
+            put(s + t, ch.value(t, xl, xi));
+}
+
+}  // namespace lssamd

@@ -51,4 +51,40 @@ struct Tile {
     int32_t pad;
 };
 
+
+// -------------------------------------------------------------------------------------------------
+// Reverse mode (adjoint) of the time loop
+// -------------------------------------------------------------------------------------------------
+constexpr int NPAR = 6;    // model parameters per problem
+constexpr int NTERM = 12;  // derivative terms per problem
+
+// Adjoint state of one problem.  The forward values a reverse step needs (U_s, U_{s+1} and the
+// stage outputs of step s) are reached through the step's Prob table; Adj holds the lambdas.
+//   la, lb     lambda of the time levels: reverse step s reads lambda_{s+1} from ((s + 1) & 1 ? lb : la)
+//              and writes lambda_s to (s & 1 ? lb : la)
+//   lv2, lx2   lambda of the equations of the last v-stage (DR: its only v-stage) and of the second
+//   lv1, lx1   S-stage, of the first v-stage and of the first S-stage; written at equation points only
+//   elo2, elo1 lambda of the lower (v = 0) Dirichlet end of each v-line, per row
+//   inv*       inverse index maps: rows r with idx1[r] == i are inv1i[inv1o[i] .. inv1o[i + 1]) (same for idx2)
+//   dfac       derivative terms: term t says d q_t(i, j) / d param[tp[t]] = X_t[i] * Y_t[j] for the
+//              transformed coefficient q = a..f (tq[t] = 0..5); per set, nterm blocks of nx X then ny Y values
+template <typename T> struct Adj {
+    T *la, *lb;
+    T *lv2, *lx2, *lv1, *lx1;
+    T *elo2, *elo1;
+    const uint32_t *inv1o, *inv1i, *inv2o, *inv2i;
+    const T *dfac;
+    int64_t dfac_stride;
+    int32_t dfac_sets, nterm;
+    int8_t tp[NTERM];
+    int8_t tq[NTERM];
+};
+
+// one device-to-device copy of n elements (checkpoint and lambda transfers)
+struct CopyJob {
+    const void *src;
+    void *dst;
+    int64_t n;
+};
+
 }  // namespace lssamd

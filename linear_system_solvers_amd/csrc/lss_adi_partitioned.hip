@@ -21,140 +21,10 @@
 // The arithmetic of each point (coefficients, right-hand sides) is the reference's; only the
 // elimination order of the line solve differs from the reference's sequential Thomas sweep.
 
+#include "lss_adi_chunk.hpp"
 #include "lss_adi_stencil.hpp"
 
 namespace lssamd {
-
-constexpr int NCH = 64;   // chunks per line
-constexpr int XTJ = 16;   // S-lines per x-sweep workgroup
-constexpr int VROWS = 4;  // v-lines (wavefronts) per v-sweep workgroup
-
-template <typename T> struct Eq {
-    T a, b, c, r;
-};
-
-// Register-resident chunk elimination.  After fwd()/bwd(): g holds r_k, zp holds p_k, zq holds q_k
-// for the cnt - 1 interior points, and (ae, be, ce, re) is the equation of the chunk's last point.
-template <typename T, int M> struct Chunk {
-    T g[M], zp[M], zq[M];
-    T ae, be, ce, re;
-    int cnt;
-
-    __device__ __forceinline__ void put(int t, const Eq<T> &q)
-    {
-        if (t == cnt - 1) {
-            ae = q.a;
-            be = q.b;
-            ce = q.c;
-            re = q.r;
-        } else if (t == 0) {
-            const T rb = (T)1 / q.b;
-            g[0] = q.c * rb;
-            zp[0] = q.r * rb;
-            zq[0] = -q.a * rb;
-        } else {
-#pragma unroll
-            for (int u = 1; u < M; ++u)
-                if (u == t) {
-                    const T rb = (T)1 / (q.b - q.a * g[u - 1]);
-                    g[u] = q.c * rb;
-                    zp[u] = (q.r - q.a * zp[u - 1]) * rb;
-                    zq[u] = (-q.a * zq[u - 1]) * rb;
-                }
-        }
-    }
-
-    __device__ __forceinline__ void bwd()
-    {
-        const int ni = cnt - 1;
-#pragma unroll
-        for (int u = M - 1; u >= 0; --u) {
-            if (u < ni) {
-                if (u == ni - 1) {
-                    g[u] = -g[u];  // r = -c/beta of the last interior point
-                } else {
-                    const T gu = g[u];
-                    zp[u] = zp[u] - gu * zp[u + 1];
-                    zq[u] = zq[u] - gu * zq[u + 1];
-                    g[u] = -gu * g[u + 1];
-                }
-            }
-        }
-    }
-
-    // (p, q, r) of the first and of the last interior point; a chunk of one point has none:
-    // then x_{e-1} = xi_{l-1} (0, 1, 0) and, seen from the left neighbour, x_s = xi_l (0, 0, 1).
-    __device__ __forceinline__ void first_rel(T &P, T &Q, T &R) const
-    {
-        if (cnt >= 2) {
-            P = zp[0];
-            Q = zq[0];
-            R = g[0];
-        } else {
-            P = (T)0;
-            Q = (T)0;
-            R = (T)1;
-        }
-    }
-    __device__ __forceinline__ void last_rel(T &P, T &Q, T &R) const
-    {
-        P = (T)0;
-        Q = (T)1;
-        R = (T)0;
-#pragma unroll
-        for (int u = 0; u < M; ++u)
-            if (u == cnt - 2) {
-                P = zp[u];
-                Q = zq[u];
-                R = g[u];
-            }
-    }
-
-    // interface equation of chunk l, given the next chunk's first relation (PF, QF, RF)
-    __device__ __forceinline__ Eq<T> interface_eq(T PF, T QF, T RF) const
-    {
-        T PL, QL, RL;
-        last_rel(PL, QL, RL);
-        Eq<T> q;
-        q.a = ae * QL;
-        q.b = ae * RL + be + ce * QF;
-        q.c = ce * RF;
-        q.r = re - ae * PL - ce * PF;
-        return q;
-    }
-
-    __device__ __forceinline__ T value(int t, T xl, T xr) const
-    {
-        T v = xr;
-#pragma unroll
-        for (int u = 0; u < M; ++u)
-            if (u == t && u < cnt - 1)
-                v = zp[u] + zq[u] * xl + g[u] * xr;
-        return v;
-    }
-};
-
-// one cyclic-reduction step; lo.b and hi.b carry the reciprocals of the neighbours' diagonals
-template <typename T> __device__ __forceinline__ Eq<T> pcr_combine(const Eq<T> &me, const Eq<T> &lo, const Eq<T> &hi)
-{
-    const T al = me.a * lo.b, ga = me.c * hi.b;
-    Eq<T> n;
-    n.a = -al * lo.a;
-    n.c = -ga * hi.c;
-    n.b = me.b - al * lo.c - ga * hi.a;
-    n.r = me.r - al * lo.r - ga * hi.r;
-    return n;
-}
-
-template <typename T> __device__ __forceinline__ Eq<T> identity_eq()
-{
-    Eq<T> q;
-    q.a = (T)0;
-    q.b = (T)1;
-    q.c = (T)0;
-    q.r = (T)0;
-    return q;
-}
 
 // -------------------------------------------------------------------------------------------------
 // v-line sweep.  stage 1: rhs_v(u, Y1) -> Y2 (DR: next); stage 2: rhs_v(u, Y3) (CS/MCS) or
