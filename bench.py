@@ -11,7 +11,15 @@ metric: ADI grid-point-steps/s (sum over problems of nx*ny*steps) of the batched
 
 Also reported: the plain per-problem stepper on a sample of the same book, the batched tridiagonal
 kernel on B = 65536 systems of n = 256 (fp64, both layouts), the single 100x50x100 Heston DR solve,
-a parity sample of the book against the CPU oracle, and the oracle itself timed on host cores.
+a parity sample of the book against the CPU oracle, the oracle itself timed on host cores, and:
+  adjoint: Heston MCS 1024 x 512, 1000 steps (K = 100, r = 0.02, kappa = 3, theta_v = 0.2, sigma = 0.8,
+    rho = 0.2, S in [0, 200], v in [0, 1.2], hints (100, 10, 50, Nonuniform), theta = zeta = 0.5): forward,
+    checkpointed forward (every 32 steps) and backward of the bilinear price at (S, v) = (100, 0.04);
+    gradient wrt kappa, theta_v, sigma, rho, r checked by central differences (h = 1e-6 relative);
+  SABR book: 256 Hundsdorfer-Verwer problems 512 x 256, 500 steps, seed 4, r = 0 (beta ~ U[0.3,1],
+    nu ~ U[0.2,1], rho ~ U[-0.7,0.3], F0 = K ~ U[0.01,0.05], alpha0 ~ U[0.01,0.1], T ~ U[0.25,5],
+    F in [0, 5K], alpha in [0, 5 alpha0]) in fp64 and fp32, with the batched adjoint of the prices
+    d/d(alpha0, nu, rho, beta).
 """
 from __future__ import annotations
 
@@ -190,6 +198,174 @@ def cpu_baseline(models, log, budget_s: float = 15.0):
     return ps / secs, done, ps
 
 
+def adjoint_model(nx=1024, ny=512, steps=1000, **over):
+    m = L.Model(kind=L.MODEL_HESTON, strike=100.0, rate=0.02, sig_sig=0.8, sig_kappa=3.0, sig_theta=0.2, rho=0.2,
+                maturity=1.0, x_lo=0.0, x_hi=200.0, y_lo=0.0, y_hi=1.2, t_lo=0.0, t_hi=1.0, nx=nx, ny=ny,
+                nt=steps + 1, grid=L.NONUNIFORM, accumulation_point=100.0, alpha_scale=10.0, beta_scale=50.0,
+                splitting=L.MODIFIED_CRAIG_SNEYD, weighting=0.5, implicit_scheme=L.CRANK_NICOLSON,
+                tridiagonal=L.THOMAS_LU, direction=L.BACKWARD, coeff_mode=L.COEFF_SEPARABLE,
+                line_solver=L.LINE_PARTITIONED)
+    for k, v in over.items():
+        setattr(m, k, v)
+    return m
+
+
+HESTON_ATTR = {"kappa": "sig_kappa", "theta_v": "sig_theta", "sigma": "sig_sig", "rho": "rho", "r": "rate"}
+
+
+def bench_adjoint(torch, log, reps=3, ckpt=32):
+    m = adjoint_model()
+    plan = L.Plan.from_models([m], L.FP64)
+    xs, ys = plan.grid(0)
+    w = L.bilinear_weights(xs, ys, 100.0, 0.04)
+    u0 = torch.from_numpy(plan.initial(0)).cuda()
+    out = torch.empty_like(u0)
+    lam = torch.from_numpy(w).cuda()
+    gin = torch.empty_like(u0)
+    stream = torch.cuda.current_stream().cuda_stream
+    fwd = lambda: plan.run_dev([u0.data_ptr()], [out.data_ptr()], stream, graph=True)  # noqa: E731
+    fwd_ck = lambda: plan.forward([u0.data_ptr()], [out.data_ptr()], ckpt, stream)  # noqa: E731
+    fwd()
+    t_fwd = time_stream(torch, fwd, reps) / reps
+    price = float(np.sum(w * out.cpu().numpy()))
+    fwd_ck()
+    plan.backward([lam.data_ptr()], [gin.data_ptr()], stream)
+    t_ck = time_stream(torch, fwd_ck, reps) / reps
+    t_bwd, dp = 0.0, None
+    for _ in range(reps):
+        fwd_ck()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dp = plan.backward([lam.data_ptr()], [gin.data_ptr()], stream)
+        t_bwd += time.perf_counter() - t0
+    t_bwd /= reps
+    grad = {name: float(dp[0, k]) for k, name in enumerate(L.HESTON_PARAMS)}
+    log(f"adjoint 1024x512x1000 MCS: forward {t_fwd * 1e3:.1f} ms, checkpointed forward {t_ck * 1e3:.1f} ms, "
+        f"backward {t_bwd * 1e3:.1f} ms, price {price:.8f}")
+    # central differences of the GPU forward at full size
+    fd, worst = {}, 0.0
+    for name in L.HESTON_PARAMS:
+        attr = HESTON_ATTR[name]
+        v = getattr(m, attr)
+        h = 1e-6 * abs(v)
+        vals = []
+        for sgn in (1.0, -1.0):
+            pm = L.Plan.from_models([adjoint_model(**{attr: v + sgn * h})], L.FP64)
+            o = torch.empty_like(u0)
+            pm.run_dev([u0.data_ptr()], [o.data_ptr()], stream, graph=False)
+            torch.cuda.synchronize()
+            vals.append(float(np.sum(w * o.cpu().numpy())))
+            pm.close()
+        fd[name] = (vals[0] - vals[1]) / (2 * h)
+        worst = max(worst, abs(grad[name] - fd[name]) / max(abs(fd[name]), 1e-12))
+    log("adjoint gradient " + ", ".join(f"{k} {grad[k]:.8e} (fd {fd[k]:.8e})" for k in grad) +
+        f"; max rel diff {worst:.2e}")
+    ps = m.nx * m.ny * m.steps
+    plan.close()
+    return {"workload": "heston_mcs_1024x512x1000_price_at_100_0.04", "checkpoint_every": ckpt,
+            "forward_ms": round(t_fwd * 1e3, 2), "forward_ckpt_ms": round(t_ck * 1e3, 2),
+            "backward_ms": round(t_bwd * 1e3, 2),
+            "ratio_backward_over_forward": round(t_bwd / t_fwd, 3),
+            "ratio_gradient_over_forward": round((t_ck + t_bwd) / t_fwd, 3),
+            "forward_grid_point_steps_per_s": ps / t_fwd, "price": price, "gradient": grad, "fd_central": fd,
+            "fd_h_rel": 1e-6, "max_rel_diff_vs_fd": worst}
+
+
+def sabr_models(count=256, seed=4, nx=512, ny=256, steps=500):
+    rng = np.random.default_rng(seed)
+    models, a0s = [], []
+    for _ in range(count):
+        beta = float(rng.uniform(0.3, 1.0))
+        nu = float(rng.uniform(0.2, 1.0))
+        rho = float(rng.uniform(-0.7, 0.3))
+        K = float(rng.uniform(0.01, 0.05))
+        a0 = float(rng.uniform(0.01, 0.1))
+        T = float(rng.uniform(0.25, 5.0))
+        models.append(L.Model(kind=L.MODEL_SABR, strike=K, rate=0.0, sig_sig=nu, rho=rho, beta=beta, maturity=T,
+                              x_lo=0.0, x_hi=5.0 * K, y_lo=0.0, y_hi=5.0 * a0, t_lo=0.0, t_hi=T, nx=nx, ny=ny,
+                              nt=steps + 1, grid=L.NONUNIFORM, accumulation_point=K, alpha_scale=10.0,
+                              beta_scale=50.0, splitting=L.HUNDSDORFER_VERWER, weighting=0.5,
+                              implicit_scheme=L.CRANK_NICOLSON, tridiagonal=L.THOMAS_LU, direction=L.BACKWARD,
+                              coeff_mode=L.COEFF_SEPARABLE, line_solver=L.LINE_PARTITIONED))
+        a0s.append(a0)
+    return models, a0s
+
+
+def bench_sabr(torch, log, count=256, fd_check=3, oracle_check=1):
+    models, a0s = sabr_models(count)
+    ps = point_steps(models)
+    stream = torch.cuda.current_stream().cuda_stream
+    res = {"workload": f"sabr_hv_book_{count}x512x256x500_seed4", "grid_point_steps_per_solve": ps}
+    outs = {}
+    for fp, name in ((L.FP64, "f64"), (L.FP32, "f32")):
+        plan = L.Plan.from_models(models, fp)
+        ins = [torch.from_numpy(plan.initial(p)).cuda() for p in range(count)]
+        o = [torch.empty_like(u) for u in ins]
+        pin, pout = [u.data_ptr() for u in ins], [u.data_ptr() for u in o]
+        plan.run_dev(pin, pout, stream, graph=True)
+        secs = time_stream(torch, lambda: plan.run_dev(pin, pout, stream, graph=True), 2) / 2
+        res[name] = {"ms": round(secs * 1e3, 2), "grid_point_steps_per_s": ps / secs}
+        outs[name] = [t.cpu().numpy().astype(np.float64) for t in o]
+        log(f"sabr book {name}: {secs * 1e3:.1f} ms, {ps / secs:.3e} grid-point-steps/s")
+        if fp == L.FP64:
+            ws = []
+            for p in range(count):
+                xs, ys = plan.grid(p)
+                ws.append(L.bilinear_weights(xs, ys, models[p].strike, a0s[p]))
+            lams = [torch.from_numpy(wq).cuda() for wq in ws]
+            plan.forward(pin, pout, 32, stream)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dp = plan.backward([t.data_ptr() for t in lams], None, stream)
+            t_bwd = time.perf_counter() - t0
+            res["adjoint_backward_ms"] = round(t_bwd * 1e3, 2)
+            res["adjoint_ratio_backward_over_forward"] = round(t_bwd / secs, 3)
+            # d price / d alpha0: slope of the bilinear interpolant in alpha (grid held fixed)
+            da0 = []
+            for p in range(count):
+                xs, ys = plan.grid(p)
+                U = outs["f64"][p]
+                i = int(np.clip(np.searchsorted(xs, models[p].strike, side="right") - 1, 0, len(xs) - 2))
+                j = int(np.clip(np.searchsorted(ys, a0s[p], side="right") - 1, 0, len(ys) - 2))
+                wx = (models[p].strike - xs[i]) / (xs[i + 1] - xs[i])
+                da0.append(((1 - wx) * (U[i, j + 1] - U[i, j]) + wx * (U[i + 1, j + 1] - U[i + 1, j])) /
+                           (ys[j + 1] - ys[j]))
+            worst = 0.0
+            for p in range(fd_check):
+                for k, attr in enumerate(("sig_sig", "rho", "beta")):
+                    v = getattr(models[p], attr)
+                    h = 1e-6 * abs(v)
+                    vals = []
+                    for sgn in (1.0, -1.0):
+                        mq = sabr_models(count, nx=models[p].nx)[0][p]
+                        setattr(mq, attr, v + sgn * h)
+                        pq = L.Plan.from_models([mq], L.FP64)
+                        vals.append(float(np.sum(ws[p] * pq.run_host()[0])))
+                        pq.close()
+                    fdv = (vals[0] - vals[1]) / (2 * h)
+                    worst = max(worst, abs(dp[p, k] - fdv) / max(abs(fdv), 1e-12))
+            res["adjoint_params"] = ["alpha0", "nu", "rho", "beta"]
+            res["adjoint_sample"] = [[float(da0[p]), *map(float, dp[p, :3])] for p in range(min(count, 4))]
+            res["adjoint_fd_max_rel_diff"] = worst
+            res["adjoint_fd_problems_checked"] = fd_check
+            log(f"sabr adjoint: backward {t_bwd * 1e3:.1f} ms, fd check on {fd_check} problems max rel diff {worst:.2e}")
+        plan.close()
+    d = max(float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(outs["f32"], outs["f64"]))
+    res["f32_vs_f64_max_normwise"] = d
+    if oracle_check:
+        from oracle import oracle as O
+        from tests.helpers import oracle_solve
+        worst = 0.0
+        for p in range(oracle_check):
+            c = oracle_solve(O, models[p])
+            worst = max(worst, float(np.abs(outs["f64"][p] - c).max() / np.abs(c).max()))
+            d32 = float(np.abs(outs["f32"][p] - c).max() / np.abs(c).max())
+        res["f64_vs_oracle_max_normwise"] = worst
+        res["f32_vs_f64_oracle_normwise"] = d32
+        log(f"sabr parity: f64 vs oracle {worst:.2e}, f32 vs oracle {d32:.2e}, f32 vs f64 GPU {d:.2e}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,13 +446,18 @@ def main():
         result["tridiag_B65536_n256_f64"] = bench_tridiag(torch, log)
         m0, u0, s0 = bench_config0(torch, log)
         result["single_100x50x100_dr_ms"] = round(s0 * 1e3, 3)
+        plan.close()
+        plan = None
+        result["adjoint"] = bench_adjoint(torch, log)
+        result["sabr_book"] = bench_sabr(torch, log)
     if not args.no_cpu:
         rate, nprob, ps = cpu_baseline(models, log)
         result["cpu_baseline"] = {"value": rate, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
                                   "sample": f"{nprob} smallest book problems ({ps:.3e} point-steps), oracle faithful "
                                             f"mode (closure-per-coefficient evaluation like the reference), 1 thread"}
     print(json.dumps(result), flush=True)
-    plan.close()
+    if plan is not None:
+        plan.close()
 
 
 if __name__ == "__main__":

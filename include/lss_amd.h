@@ -148,8 +148,9 @@ int lss_plan_problem_shape(const lss_adi_plan *plan, int64_t p, int64_t *nx, int
  * Batched plain tridiagonal systems (replaces one thomas_lu_solver::solve per line,
  * sparse_solvers/tridiagonal/thomas_lu_solver/lss_thomas_lu_solver.hpp:125-191, with Neumann(0) ends
  * and a[0] = c[n-1] = 0, i.e. the plain algorithm).  B systems of length n; layout CONTIGUOUS
- * (x[k*n+i]) or STRIDED (x[i*B+k]); device pointers; enqueued on `stream`.  The library keeps a
- * grow-only device workspace per device for the solver's scratch.
+ * (x[k*n+i]) or STRIDED (x[i*B+k]); device pointers; enqueued on `stream`.  Systems of up to 1024
+ * unknowns (contiguous fp64: 512) are solved by the register-resident partitioned elimination with
+ * no scratch; longer ones one lane per system, with a grow-only device workspace kept per device.
  */
 int lss_tridiag_batched(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b,
                         const void *c, const void *d, void *x, void *stream);

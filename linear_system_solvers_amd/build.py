@@ -14,7 +14,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "liblss_amd.so")
-SOURCES = ["lss_adi_kernels.hip", "lss_adi_partitioned.hip", "lss_adi_adjoint.hip", "lss_adi_capi.hip"]
+SOURCES = ["lss_adi_kernels.hip", "lss_adi_partitioned.hip", "lss_adi_adjoint.hip", "lss_tridiag.hip", "lss_adi_capi.hip"]
 ARCH = os.environ.get("LSS_OFFLOAD_ARCH", "gfx950")
 
 

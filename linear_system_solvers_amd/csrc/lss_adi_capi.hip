@@ -41,6 +41,10 @@ template <typename T, int M>
 __global__ void k_adj_x(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
 template <typename T> __global__ void k_adj_u(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
 template <typename T> __global__ void k_copy_jobs(const CopyJob *);
+template <typename T, int M>
+__global__ void k_tridiag_rows(int64_t, int, const T *, const T *, const T *, const T *, T *);
+template <typename T, int M>
+__global__ void k_tridiag_cols(int64_t, int, const T *, const T *, const T *, const T *, T *);
 constexpr int P_NCH = 64, P_XTJ = 16, P_VROWS = 4;
 
 }  // namespace lssamd
@@ -1184,6 +1188,38 @@ template <typename T> lss_adi_plan *model_plan(const lss_model *models, int64_t 
     return plan;
 }
 
+constexpr int TRI_WAVES_H = 4;  // systems per workgroup of k_tridiag_rows
+
+template <typename T, int M>
+void tridiag_part_m(int layout, int64_t B, int n, const void *a, const void *b, const void *c, const void *d, void *x,
+                    size_t lds_rows, hipStream_t st)
+{
+    const T *pa = (const T *)a, *pb = (const T *)b, *pc = (const T *)c, *pd = (const T *)d;
+    T *px = (T *)x;
+    if (layout == LSS_LAYOUT_CONTIGUOUS) {
+        const unsigned blocks = (unsigned)((B + TRI_WAVES_H - 1) / TRI_WAVES_H);
+        hipLaunchKernelGGL((k_tridiag_rows<T, M>), dim3(blocks), dim3(64 * TRI_WAVES_H), (unsigned)lds_rows, st, B, n,
+                           pa, pb, pc, pd, px);
+    } else {
+        const unsigned blocks = (unsigned)((B + P_XTJ - 1) / P_XTJ);
+        hipLaunchKernelGGL((k_tridiag_cols<T, M>), dim3(blocks), dim3(P_XTJ * P_NCH), 0, st, B, n, pa, pb, pc, pd,
+                           px);
+    }
+}
+
+template <typename T>
+void tridiag_part(int M, int layout, int64_t B, int n, const void *a, const void *b, const void *c, const void *d,
+                  void *x, size_t lds_rows, hipStream_t st)
+{
+    switch (M) {
+    case 1: tridiag_part_m<T, 1>(layout, B, n, a, b, c, d, x, lds_rows, st); break;
+    case 2: tridiag_part_m<T, 2>(layout, B, n, a, b, c, d, x, lds_rows, st); break;
+    case 4: tridiag_part_m<T, 4>(layout, B, n, a, b, c, d, x, lds_rows, st); break;
+    case 8: tridiag_part_m<T, 8>(layout, B, n, a, b, c, d, x, lds_rows, st); break;
+    default: tridiag_part_m<T, 16>(layout, B, n, a, b, c, d, x, lds_rows, st); break;
+    }
+}
+
 // grow-only per-device workspace of lss_tridiag_batched
 std::mutex g_ws_mutex;
 std::vector<std::pair<void *, size_t>> g_ws;
@@ -1376,17 +1412,31 @@ int lss_tridiag_batched(int32_t fp, int32_t layout, int64_t B, int64_t n, const 
         if (B == 0)
             return;
         const size_t es = fp == LSS_FP64 ? 8 : 4;
-        void *scr = workspace((size_t)B * (size_t)n * es);
-        const unsigned blocks = (unsigned)((B + 255) / 256);
         hipStream_t st = (hipStream_t)stream;
-        if (fp == LSS_FP64)
-            hipLaunchKernelGGL(k_tridiag_lane<double>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
-                               (const double *)a, (const double *)b, (const double *)c, (const double *)d,
-                               (double *)x, (double *)scr);
-        else
-            hipLaunchKernelGGL(k_tridiag_lane<float>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
-                               (const float *)a, (const float *)b, (const float *)c, (const float *)d,
-                               (float *)x, (float *)scr);
+        // partitioned solvers for lines of up to NCH * 16 unknowns (rows: while the staged systems fit
+        // 64 KiB of LDS), otherwise one lane per system with a scratch workspace
+        int M = 1;
+        while (M * P_NCH < n && M < 32)
+            M *= 2;
+        const size_t lds_rows = (size_t)TRI_WAVES_H * 4 * (size_t)n * es;
+        const bool part = M <= 16 && (layout == LSS_LAYOUT_STRIDED || lds_rows <= 65536);
+        if (part) {
+            if (fp == LSS_FP64)
+                tridiag_part<double>(M, layout, B, (int)n, a, b, c, d, x, lds_rows, st);
+            else
+                tridiag_part<float>(M, layout, B, (int)n, a, b, c, d, x, lds_rows, st);
+        } else {
+            void *scr = workspace((size_t)B * (size_t)n * es);
+            const unsigned blocks = (unsigned)((B + 255) / 256);
+            if (fp == LSS_FP64)
+                hipLaunchKernelGGL(k_tridiag_lane<double>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
+                                   (const double *)a, (const double *)b, (const double *)c, (const double *)d,
+                                   (double *)x, (double *)scr);
+            else
+                hipLaunchKernelGGL(k_tridiag_lane<float>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
+                                   (const float *)a, (const float *)b, (const float *)c, (const float *)d,
+                                   (float *)x, (float *)scr);
+        }
         HIPCHK(hipGetLastError());
     });
 }
