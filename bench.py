@@ -13,7 +13,7 @@ Also reported: the plain per-problem stepper on a sample of the same book, the b
 kernel on B = 65536 systems of n = 256 (fp64, both layouts), the single 100x50x100 Heston DR solve,
 a parity sample of the book against the CPU oracle, the oracle itself timed on host cores, and:
   adjoint: Heston MCS 1024 x 512, 1000 steps (K = 100, r = 0.02, kappa = 3, theta_v = 0.2, sigma = 0.8,
-    rho = 0.2, S in [0, 200], v in [0, 1.2], hints (100, 10, 50, Nonuniform), theta = zeta = 0.5): forward,
+    rho = 0.2, S in [0, 200], v in [0, 1.2], hints (100, 10, 15, Nonuniform), theta = zeta = 0.5): forward,
     checkpointed forward (every 32 steps) and backward of the bilinear price at (S, v) = (100, 0.04);
     gradient wrt kappa, theta_v, sigma, rho, r checked by central differences (h = 1e-6 relative);
   SABR book: 256 Hundsdorfer-Verwer problems 512 x 256, 500 steps, seed 4, r = 0 (beta ~ U[0.3,1],
@@ -199,9 +199,11 @@ def cpu_baseline(models, log, budget_s: float = 15.0):
 
 
 def adjoint_model(nx=1024, ny=512, steps=1000, **over):
+    """v clustering beta_scale = 15: with 1000 steps the reference's explicit v = 0 row needs
+    3 nu E <= 2, i.e. beta_scale <= ~15 (beta_scale = 20 already diverges in the reference itself)."""
     m = L.Model(kind=L.MODEL_HESTON, strike=100.0, rate=0.02, sig_sig=0.8, sig_kappa=3.0, sig_theta=0.2, rho=0.2,
                 maturity=1.0, x_lo=0.0, x_hi=200.0, y_lo=0.0, y_hi=1.2, t_lo=0.0, t_hi=1.0, nx=nx, ny=ny,
-                nt=steps + 1, grid=L.NONUNIFORM, accumulation_point=100.0, alpha_scale=10.0, beta_scale=50.0,
+                nt=steps + 1, grid=L.NONUNIFORM, accumulation_point=100.0, alpha_scale=10.0, beta_scale=15.0,
                 splitting=L.MODIFIED_CRAIG_SNEYD, weighting=0.5, implicit_scheme=L.CRANK_NICOLSON,
                 tridiagonal=L.THOMAS_LU, direction=L.BACKWARD, coeff_mode=L.COEFF_SEPARABLE,
                 line_solver=L.LINE_PARTITIONED)

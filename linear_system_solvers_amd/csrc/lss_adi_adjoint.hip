@@ -21,6 +21,9 @@
 // workgroup reduces its contributions in double and adds them to its own slot, so the sums are
 // deterministic.
 
+// The fast path may fuse multiply-adds (the library is built with -ffp-contract=off so that the exact
+// line solver keeps the reference's rounding; this translation unit opts back in).
+#pragma clang fp contract(fast)
 #include "lss_adi_chunk.hpp"
 #include "lss_adi_stencil.hpp"
 
@@ -44,11 +47,11 @@ __device__ __forceinline__ void add_terms_q(const Prob<T> &p, const Adj<T> &A, i
                                             T gc, T gd, T ge, T gf, double *acc)
 {
     const int nx = p.nx, ny = p.ny;
-    const T *f = A.dfac + (A.dfac_sets > 1 ? (int64_t)step : 0) * A.dfac_stride;
+    const gptr<const T> f = G(A.dfac + (A.dfac_sets > 1 ? (int64_t)step : 0) * A.dfac_stride);
     for (int t = 0; t < A.nterm; ++t) {
         const int q = A.tq[t];
         const T gq = q == 0 ? ga : (q == 1 ? gb : (q == 2 ? gc : (q == 3 ? gd : (q == 4 ? ge : gf))));
-        const T *ft = f + (int64_t)t * (nx + ny);
+        const gptr<const T> ft = f + (int64_t)t * (nx + ny);
         const double v = (double)gq * (double)ft[i] * (double)ft[nx + j];
         const int k = A.tp[t];
 #pragma unroll
@@ -187,9 +190,10 @@ template <typename T> __device__ __forceinline__ bool inside(const Prob<T> &p, i
 // the Delta terms; HV: rhs_intermed_4 and the explicit stage H, whose multiplier is also lx2) and,
 // for HV, from the V2 equations (in = Y2).
 template <typename T>
-__device__ __forceinline__ T lam_y2(const Prob<T> &p, const Adj<T> &A, const T *cs, int i, int j)
+__device__ __forceinline__ T lam_y2(const Prob<T> &p, const Adj<T> &A, gptr<const T> cs, int i, int j)
 {
     const int ny = p.ny;
+    const gptr<const T> lx2 = G((const T *)A.lx2), lv2 = G((const T *)A.lv2);
     const bool hv = p.scheme == SCHEME_HV, mcs = p.scheme == SCHEME_MCS;
     T v = (T)0;
 #pragma unroll
@@ -205,11 +209,11 @@ __device__ __forceinline__ T lam_y2(const Prob<T> &p, const Adj<T> &A, const T *
                 T w = w_h(p, s, -di, -dj, 2);
                 if (dj == 0)
                     w += w_r4(p, s, -di);
-                v += w * A.lx2[q];
+                v += w * lx2[q];
                 if (di == 0)
-                    v += w_rv(p, s, -dj) * A.lv2[q];
+                    v += w_rv(p, s, -dj) * lv2[q];
             } else {
-                v += w_r3(p, s, -di, -dj, mcs, false) * A.lx2[q];
+                v += w_r3(p, s, -di, -dj, mcs, false) * lx2[q];
             }
         }
     }
@@ -240,13 +244,14 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
         const int nx = p.nx, ny = p.ny, N = ny - 2, P = nx - 1;
         const T *u = (step & 1) ? p.b : p.a;
         const T *un = (step & 1) ? p.a : p.b;
-        const T *lamn = ((step + 1) & 1) ? A.lb : A.la;
-        const T *cs = coeff_set(p, step);
+        const gptr<const T> lamn = G((const T *)(((step + 1) & 1) ? A.lb : A.la));
+        const gptr<const T> cs = coeff_set(p, step);
         const T one = (T)1, theta = p.theta;
         const bool hv = p.scheme == SCHEME_HV;
-        const T *yrow = (last ? un : p.y2) + (int64_t)i * ny;
-        const T *irow = ((last && hv) ? p.y2 : u) + (int64_t)i * ny;
-        T *L = (last ? A.lv2 : A.lv1) + (int64_t)i * ny;
+        const gptr<const T> yrow = G((last ? un : (const T *)p.y2) + (int64_t)i * ny);
+        const gptr<const T> irow = G(((last && hv) ? (const T *)p.y2 : u) + (int64_t)i * ny);
+        const gptr<T> L = G((last ? A.lv2 : A.lv1) + (int64_t)i * ny);
+        const gptr<const uint32_t> inv2o = G(A.inv2o), inv2i = G(A.inv2i);
         // lambda of the stage output at (i, j): B^T (explicit_boundary.hpp:202-238) for the last stage
         auto lam_out = [&](int j) -> T {
             if (!last)
@@ -254,8 +259,8 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
             T v = lamn[(int64_t)i * ny + j];
             if (i == P - 1 || i == P - 2) {
                 T sB = (T)0;
-                for (uint32_t k = A.inv2o[j]; k < A.inv2o[j + 1]; ++k)
-                    sB += lamn[(int64_t)P * ny + A.inv2i[k]];
+                for (uint32_t k = inv2o[j]; k < inv2o[j + 1]; ++k)
+                    sB += lamn[(int64_t)P * ny + inv2i[k]];
                 v += (i == P - 1) ? ((T)4 / (T)3) * sB : -(sB / (T)3);
             }
             return v;
@@ -277,7 +282,7 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
             add_terms(p, A, step, i, k, g, acc);
             if (k == 1) {
                 const T a1 = -theta * stencil_at(p, cs, i, 1).Mt;
-                (last ? A.elo2 : A.elo1)[i] = lam_out(0) - a1 * x;
+                G(last ? A.elo2 : A.elo1)[i] = lam_out(0) - a1 * x;
             }
         };
         wave_solve<T, M>(N, lane, eq, put);
@@ -309,29 +314,30 @@ __global__ void __launch_bounds__(XTJ *NCH) k_adj_x(const Prob<T> *__restrict__ 
     const bool colok = col < tl.count;
     const int j = tl.first + (colok ? col : 0);
     const int nx = p.nx, ny = p.ny, N = nx - 2;
-    const T *u = (step & 1) ? p.b : p.a;
-    const T *cs = coeff_set(p, step);
+    const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
+    const gptr<const T> cs = coeff_set(p, step);
     const T one = (T)1, theta = p.theta, zeta = p.zeta, gamma = p.gamma;
     const int sch = p.scheme;
     const bool hv = sch == SCHEME_HV, mcs = sch == SCHEME_MCS, dr = sch == SCHEME_DR;
-    const T *y = (stage == 1) ? p.y1 : p.y4;  // stage output (p.y4 is Y3's buffer for CS/MCS)
-    if (stage == 2 && !hv)
-        y = p.y3;
-    T *L = (stage == 2) ? A.lx2 : A.lx1;
+    // stage output: Y1, or Y3 (CS/MCS) / Y4 (HV)
+    const gptr<const T> y = G((const T *)((stage == 1) ? p.y1 : (hv ? p.y4 : p.y3)));
+    const gptr<T> L = G((stage == 2) ? A.lx2 : A.lx1);
+    const gptr<const T> lv2 = G((const T *)A.lv2), lv1 = G((const T *)A.lv1), lx2 = G((const T *)A.lx2);
+    const gptr<const T> y1 = G((const T *)p.y1), y2 = G((const T *)p.y2);
     auto lam_out = [&](int k) -> T {
         const int64_t q = (int64_t)k * ny + j;
         if (stage == 2)
-            return A.lv2[q];
+            return lv2[q];
         if (dr)
-            return A.lv2[q];
+            return lv2[q];
         if (!hv)
-            return A.lx2[q] + A.lv1[q];
-        T v = A.lv1[q];
+            return lx2[q] + lv1[q];
+        T v = lv1[q];
 #pragma unroll
         for (int di = -1; di <= 1; ++di) {
             const int ei = k + di;
             if (ei >= 1 && ei <= N)
-                v += w_h(p, stencil_at(p, cs, ei, j), -di, 0, 0) * A.lx2[(int64_t)ei * ny + j];
+                v += w_h(p, stencil_at(p, cs, ei, j), -di, 0, 0) * lx2[(int64_t)ei * ny + j];
         }
         return v;
     };
@@ -362,7 +368,6 @@ __global__ void __launch_bounds__(XTJ *NCH) k_adj_x(const Prob<T> *__restrict__ 
             g.Pt += x * u[q + 1];
             g.C += x * gamma * (u[q - ny - 1] - u[q - ny + 1] - u[q + ny - 1] + u[q + ny + 1]);
         } else {
-            const T *y2 = p.y2;
             auto D = [&](int64_t o) { return y2[q + o] - u[q + o]; };
             const T cornerD = D(ny + 1) - D(ny - 1) - D(-ny + 1) + D(-ny - 1);
             if (!hv) {  // rhs_intermed_3
@@ -380,7 +385,6 @@ __global__ void __launch_bounds__(XTJ *NCH) k_adj_x(const Prob<T> *__restrict__ 
                     g.Pt += x * zt * D(1);
                 }
             } else {  // rhs_intermed_4 on Y2, then the explicit stage H with the same multiplier
-                const T *y1 = p.y1;
                 g.M += -theta * x * y2[q - ny];
                 g.Z += theta * x * y2[q];
                 g.P += -theta * x * y2[q + ny];
@@ -418,20 +422,23 @@ __global__ void __launch_bounds__(256) k_adj_u(const Prob<T> *__restrict__ probs
     if (step < p.steps) {
         const int r = tl.first;
         const int nx = p.nx, ny = p.ny, P = nx - 1;
-        const T *u = (step & 1) ? p.b : p.a;
-        T *lam = (step & 1) ? A.lb : A.la;
-        const T *cs = coeff_set(p, step);
-        const T *bs = bnd_set(p, step);
+        const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
+        const gptr<T> lam = G((step & 1) ? A.lb : A.la);
+        const gptr<const T> cs = coeff_set(p, step);
+        const gptr<const T> bs = bnd_set(p, step);
+        const gptr<const T> lv2 = G((const T *)A.lv2), lv1 = G((const T *)A.lv1), lx2 = G((const T *)A.lx2),
+                            lx1 = G((const T *)A.lx1), elo2 = G((const T *)A.elo2), elo1 = G((const T *)A.elo1);
+        const gptr<const uint32_t> inv1o = G(A.inv1o), inv1i = G(A.inv1i), inv2o = G(A.inv2o), inv2i = G(A.inv2i);
         const T theta = p.theta, delta = p.delta, ni = p.ni, rho = p.rho;
         const int sch = p.scheme;
         const bool dr = sch == SCHEME_DR, hv = sch == SCHEME_HV, mcs = sch == SCHEME_MCS;
         // lambda of colA0[i]: every v-line whose lower end read it (through idx1)
         auto lamA = [&](int i) -> T {
             T v = (T)0;
-            for (uint32_t k = A.inv1o[i]; k < A.inv1o[i + 1]; ++k) {
-                const uint32_t rr = A.inv1i[k];
+            for (uint32_t k = inv1o[i]; k < inv1o[i + 1]; ++k) {
+                const uint32_t rr = inv1i[k];
                 if (rr >= 1 && (int)rr <= P - 1)
-                    v += A.elo2[rr] + (dr ? (T)0 : A.elo1[rr]);
+                    v += elo2[rr] + (dr ? (T)0 : elo1[rr]);
             }
             return v;
         };
@@ -447,32 +454,32 @@ __global__ void __launch_bounds__(256) k_adj_u(const Prob<T> *__restrict__ probs
                     const Stencil<T> s = stencil_at(p, cs, ei, ej);
                     const int64_t q = (int64_t)ei * ny + ej;
                     if (di == 0 && !hv)
-                        v += w_rv(p, s, -dj) * A.lv2[q];
+                        v += w_rv(p, s, -dj) * lv2[q];
                     if (di == 0 && !dr)
-                        v += w_rv(p, s, -dj) * A.lv1[q];
+                        v += w_rv(p, s, -dj) * lv1[q];
                     if (!dr && !hv)
-                        v += w_r3(p, s, -di, -dj, mcs, true) * A.lx2[q];
+                        v += w_r3(p, s, -di, -dj, mcs, true) * lx2[q];
                     if (hv)
-                        v += w_h(p, s, -di, -dj, 1) * A.lx2[q];
-                    v += w_r1(p, s, -di, -dj) * A.lx1[q];
+                        v += w_h(p, s, -di, -dj, 1) * lx2[q];
+                    v += w_r1(p, s, -di, -dj) * lx1[q];
                 }
             }
             // Dirichlet ends of the S-lines j with idx2[j] == c (implicit_kernel.hpp:86-101)
             if (r == 0 || r == P) {
                 const int ei = (r == 0) ? 1 : P - 1;
-                for (uint32_t k = A.inv2o[c]; k < A.inv2o[c + 1]; ++k) {
-                    const int j = (int)A.inv2i[k];
+                for (uint32_t k = inv2o[c]; k < inv2o[c + 1]; ++k) {
+                    const int j = (int)inv2i[k];
                     if (j < 1 || j > ny - 2)
                         continue;
                     const Stencil<T> s = stencil_at(p, cs, ei, j);
                     const T w = (r == 0) ? theta * s.M : theta * s.P;  // -(folded coefficient)
                     const int64_t q = (int64_t)ei * ny + j;
-                    T l = A.lx1[q];
+                    T l = lx1[q];
                     if (!dr)
-                        l += A.lx2[q];
+                        l += lx2[q];
                     v += w * l;
                     if (hv)  // H reads Y1 at rows 0 and P; Y1's ends are these U values
-                        v += -w * A.lx2[q];
+                        v += -w * lx2[q];
                 }
             }
             // boundary stage A (explicit_boundary.hpp:39-84): the v = 0 column of U_{s+1} read U_s
@@ -498,7 +505,7 @@ __global__ void __launch_bounds__(256) k_adj_u(const Prob<T> *__restrict__ probs
                 }
                 if (c == 0 && r >= 1 && A.nterm > 0) {
                     const T la = lamA(r);
-                    const T *ur = u + (int64_t)r * ny;
+                    const gptr<const T> ur = u + (int64_t)r * ny;
                     const T gd = (r < P) ? la * delta * (u[(int64_t)(r + 1) * ny] - u[(int64_t)(r - 1) * ny])
                                          : -la * delta * ((T)2 * p.h1 * bs[nx + ny]);
                     const T ge = la * ni * (-(T)3 * ur[0] + (T)4 * ur[1] - ur[2]);

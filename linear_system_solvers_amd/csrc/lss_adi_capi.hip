@@ -33,7 +33,7 @@ template <typename T> __global__ void k_vsweep(const Prob<T> *, const Tile *, in
 template <typename T> __global__ void k_hv_y3(const Prob<T> *, const Tile *, int);
 template <typename T>
 __global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *);
-template <typename T, int M> __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int, int);
+template <typename T, int M, int STAGE> __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int);
 template <typename T, int M> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int);
 template <typename T, int M>
 __global__ void k_adj_v(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
@@ -186,8 +186,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     std::vector<int> prefix[TB_N];     // prefix[tb][q] = tiles of internal problems [0, q)
     int max_steps = 0;
     int line_solver = LINE_EXACT;
-    int pm = 1;                        // partitioned: points per chunk bound (template M)
-    int apm = 1;                       // the same bound for the adjoint's transposed solves
+    int pmx = 1, pmy = 1;              // points per chunk bound (template M) of S-lines and v-lines
     bool adj_ok = false;               // every line fits the partitioned solver (needed by the adjoint)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -232,6 +231,34 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         return d.coeff_mode == LSS_COEFF_FIELDS ? (size_t)(7 * d.nx * d.ny + 3 * d.nx) : (size_t)(12 * (d.nx + d.ny));
     }
 
+    // SEPARABLE factor packing: factor vectors that are zero (or one) in every coefficient set are not
+    // stored; code[slot] says -1 (zero), -2 (one) or the packed position (see Prob::fcode)
+    static void pack_separable(const lss_adi_desc &d, std::vector<T> &out, int32_t *code)
+    {
+        const size_t nx = (size_t)d.nx, ny = (size_t)d.ny, stride = 12 * (nx + ny);
+        const T *src = (const T *)d.coeff;
+        out.assign(stride * (size_t)d.coeff_sets, (T)0);
+        for (int blk = 0; blk < 2; ++blk) {
+            const size_t n = blk ? ny : nx, base = blk ? 12 * nx : 0;
+            int packed = 0;
+            for (int slot = 0; slot < 12; ++slot) {
+                bool zero = true, one = true;
+                for (int64_t k = 0; k < d.coeff_sets; ++k)
+                    for (size_t i = 0; i < n; ++i) {
+                        const T v = src[(size_t)k * stride + base + slot * n + i];
+                        zero = zero && v == (T)0;
+                        one = one && v == (T)1;
+                    }
+                int c = zero ? -1 : (one ? -2 : packed++);
+                if (c >= 0)
+                    for (int64_t k = 0; k < d.coeff_sets; ++k)
+                        std::memcpy(out.data() + (size_t)k * stride + base + (size_t)c * n,
+                                    src + (size_t)k * stride + base + slot * n, n * sizeof(T));
+                code[blk * 12 + slot] = c;
+            }
+        }
+    }
+
     void build(const lss_adi_desc *descs, int64_t n)
     {
         count = n;
@@ -243,15 +270,16 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         }
         line_solver = descs[0].line_solver == LSS_LINE_PARTITIONED ? LINE_PARTITIONED : LINE_EXACT;
         {
-            int64_t mmax = 1;
-            for (int64_t q = 0; q < n; ++q)
-                mmax = std::max({mmax, (descs[q].nx - 2 + P_NCH - 1) / P_NCH, (descs[q].ny - 2 + P_NCH - 1) / P_NCH});
-            apm = 1;
-            while (apm < mmax)
-                apm *= 2;
-            adj_ok = apm <= 16;
-            if (line_solver == LINE_PARTITIONED)
-                pm = apm;
+            int64_t mx = 1, my = 1;
+            for (int64_t q = 0; q < n; ++q) {
+                mx = std::max(mx, (descs[q].nx - 2 + P_NCH - 1) / P_NCH);
+                my = std::max(my, (descs[q].ny - 2 + P_NCH - 1) / P_NCH);
+            }
+            while (pmx < mx)
+                pmx *= 2;
+            while (pmy < my)
+                pmy *= 2;
+            adj_ok = pmx <= 16 && pmy <= 16;
         }
         order.resize(n);
         std::iota(order.begin(), order.end(), 0);
@@ -335,8 +363,14 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             uint32_t *ix = (uint32_t *)(base + L.off_idx_bytes);
             p.idx1 = ix;
             p.idx2 = ix + d.nx;
-            HIPCHK(hipMemcpy((void *)p.coeff, d.coeff, sizeof(T) * coeff_stride(d) * (size_t)d.coeff_sets,
-                             hipMemcpyHostToDevice));
+            if (p.coeff_mode == COEFF_SEPARABLE) {
+                std::vector<T> packed;
+                pack_separable(d, packed, p.fcode);
+                HIPCHK(hipMemcpy((void *)p.coeff, packed.data(), sizeof(T) * packed.size(), hipMemcpyHostToDevice));
+            } else {
+                HIPCHK(hipMemcpy((void *)p.coeff, d.coeff, sizeof(T) * coeff_stride(d) * (size_t)d.coeff_sets,
+                                 hipMemcpyHostToDevice));
+            }
             HIPCHK(hipMemcpy((void *)p.bnd, d.bnd, sizeof(T) * (size_t)(d.nx + 2 * d.ny) * (size_t)d.bnd_sets,
                              hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(ix, d.idx1, sizeof(uint32_t) * (size_t)d.nx, hipMemcpyHostToDevice));
@@ -453,13 +487,22 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 return;
             }
             const dim3 g(grid(TB_X)), b(P_XTJ * P_NCH);
-            switch (pm) {
-            case 1: hipLaunchKernelGGL((k_xsweep_p<T, 1>), g, b, 0, st, pd, td + tile_off[TB_X], s, stage); break;
-            case 2: hipLaunchKernelGGL((k_xsweep_p<T, 2>), g, b, 0, st, pd, td + tile_off[TB_X], s, stage); break;
-            case 4: hipLaunchKernelGGL((k_xsweep_p<T, 4>), g, b, 0, st, pd, td + tile_off[TB_X], s, stage); break;
-            case 8: hipLaunchKernelGGL((k_xsweep_p<T, 8>), g, b, 0, st, pd, td + tile_off[TB_X], s, stage); break;
-            default: hipLaunchKernelGGL((k_xsweep_p<T, 16>), g, b, 0, st, pd, td + tile_off[TB_X], s, stage); break;
+            const Tile *tx = td + tile_off[TB_X];
+#define LSS_XS(M)                                                                                              \
+    do {                                                                                                       \
+        if (stage == 1)                                                                                        \
+            hipLaunchKernelGGL((k_xsweep_p<T, M, 1>), g, b, 0, st, pd, tx, s);                                 \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_xsweep_p<T, M, 2>), g, b, 0, st, pd, tx, s);                                 \
+    } while (0)
+            switch (pmx) {
+            case 1: LSS_XS(1); break;
+            case 2: LSS_XS(2); break;
+            case 4: LSS_XS(4); break;
+            case 8: LSS_XS(8); break;
+            default: LSS_XS(16); break;
             }
+#undef LSS_XS
         };
         auto vs = [&](int stage) {
             if (line_solver == LINE_EXACT) {
@@ -467,7 +510,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 return;
             }
             const dim3 g(grid(TB_V)), b(64 * P_VROWS);
-            switch (pm) {
+            switch (pmy) {
             case 1: hipLaunchKernelGGL((k_vsweep_p<T, 1>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
             case 2: hipLaunchKernelGGL((k_vsweep_p<T, 2>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
             case 4: hipLaunchKernelGGL((k_vsweep_p<T, 4>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
@@ -714,40 +757,53 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         }
     }
 
-    template <int M> void adj_launch(int s, const Prob<T> *table, hipStream_t st)
+    template <int M> void adj_v(int s, int last, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
+    {
+        hipLaunchKernelGGL((k_adj_v<T, M>), dim3(grid), dim3(64 * P_VROWS), 0, st, table, (const Adj<T> *)adj_dev.p,
+                           (const Tile *)tiles_dev.p + tile_off[TB_AV], s, last, slot);
+    }
+    template <int M> void adj_x(int s, int stage, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
+    {
+        hipLaunchKernelGGL((k_adj_x<T, M>), dim3(grid), dim3(P_XTJ * P_NCH), 0, st, table, (const Adj<T> *)adj_dev.p,
+                           (const Tile *)tiles_dev.p + tile_off[TB_AX], s, stage, slot);
+    }
+    void adj_v_m(int s, int last, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
+    {
+        switch (pmy) {
+        case 1: adj_v<1>(s, last, table, slot, st, grid); break;
+        case 2: adj_v<2>(s, last, table, slot, st, grid); break;
+        case 4: adj_v<4>(s, last, table, slot, st, grid); break;
+        case 8: adj_v<8>(s, last, table, slot, st, grid); break;
+        default: adj_v<16>(s, last, table, slot, st, grid); break;
+        }
+    }
+    void adj_x_m(int s, int stage, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
+    {
+        switch (pmx) {
+        case 1: adj_x<1>(s, stage, table, slot, st, grid); break;
+        case 2: adj_x<2>(s, stage, table, slot, st, grid); break;
+        case 4: adj_x<4>(s, stage, table, slot, st, grid); break;
+        case 8: adj_x<8>(s, stage, table, slot, st, grid); break;
+        default: adj_x<16>(s, stage, table, slot, st, grid); break;
+        }
+    }
+
+    void launch_adj_step(int s, const Prob<T> *table, hipStream_t st)
     {
         const int64_t na = active(s);
         auto grid = [&](int tb) { return (unsigned)prefix[tb][na]; };
         bool any_split2 = false;
         for (int64_t q = 0; q < na; ++q)
             any_split2 |= probs[q].scheme != SCHEME_DR;
-        const Adj<T> *ad = (const Adj<T> *)adj_dev.p;
-        const Tile *td = (const Tile *)tiles_dev.p;
         double *sl = (double *)slots_dev.p;
-        const dim3 bv(64 * P_VROWS), bx(P_XTJ * P_NCH), bu(256);
-        hipLaunchKernelGGL((k_adj_v<T, M>), dim3(grid(TB_AV)), bv, 0, st, table, ad, td + tile_off[TB_AV], s, 1,
-                           sl + slot_off[0] * NPAR);
+        adj_v_m(s, 1, table, sl + slot_off[0] * NPAR, st, grid(TB_AV));
         if (any_split2) {
-            hipLaunchKernelGGL((k_adj_x<T, M>), dim3(grid(TB_AX)), bx, 0, st, table, ad, td + tile_off[TB_AX], s, 2,
-                               sl + slot_off[1] * NPAR);
-            hipLaunchKernelGGL((k_adj_v<T, M>), dim3(grid(TB_AV)), bv, 0, st, table, ad, td + tile_off[TB_AV], s, 0,
-                               sl + slot_off[2] * NPAR);
+            adj_x_m(s, 2, table, sl + slot_off[1] * NPAR, st, grid(TB_AX));
+            adj_v_m(s, 0, table, sl + slot_off[2] * NPAR, st, grid(TB_AV));
         }
-        hipLaunchKernelGGL((k_adj_x<T, M>), dim3(grid(TB_AX)), bx, 0, st, table, ad, td + tile_off[TB_AX], s, 1,
-                           sl + slot_off[3] * NPAR);
-        hipLaunchKernelGGL(k_adj_u<T>, dim3(grid(TB_U)), bu, 0, st, table, ad, td + tile_off[TB_U], s,
-                           sl + slot_off[4] * NPAR);
-    }
-
-    void launch_adj_step(int s, const Prob<T> *table, hipStream_t st)
-    {
-        switch (apm) {
-        case 1: adj_launch<1>(s, table, st); break;
-        case 2: adj_launch<2>(s, table, st); break;
-        case 4: adj_launch<4>(s, table, st); break;
-        case 8: adj_launch<8>(s, table, st); break;
-        default: adj_launch<16>(s, table, st); break;
-        }
+        adj_x_m(s, 1, table, sl + slot_off[3] * NPAR, st, grid(TB_AX));
+        hipLaunchKernelGGL(k_adj_u<T>, dim3(grid(TB_U)), dim3(256), 0, st, table, (const Adj<T> *)adj_dev.p,
+                           (const Tile *)tiles_dev.p + tile_off[TB_U], s, sl + slot_off[4] * NPAR);
     }
 
     void forward_ckpt(const void *const *u_in, void *const *u_out, int K, hipStream_t stream) override

@@ -28,6 +28,12 @@ enum : int32_t { LINE_EXACT = 0, LINE_PARTITIONED = 1 };
 
 constexpr int TILE = 64;  // lines (or points) per workgroup tile: one wavefront
 
+// Global-memory pointers.  The arrays are reached through pointers stored in the problem tables, so
+// the compiler cannot see that they point to global memory and would emit flat instructions (extra
+// latency, and they count against the LDS wait counter too); kernels convert each pointer once.
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+template <typename T> __device__ __forceinline__ gptr<T> G(T *p) { return (gptr<T>)p; }
+
 template <typename T> struct Prob {
     int32_t nx, ny, steps, scheme;
     int32_t tri, coeff_mode, coeff_sets, bnd_sets;
@@ -40,6 +46,9 @@ template <typename T> struct Prob {
     const uint32_t *idx1;      // index_of maps (lss_grid_config.hpp:110-118)
     const uint32_t *idx2;
     const T *coeff;            // coeff_sets sets of coeff_stride values
+    // SEPARABLE: factor slot 2q + r of X (codes 0..11) and of Y (codes 12..23): -1 the vector is zero,
+    // -2 it is one, k >= 0 it is stored as packed vector k of its block
+    int32_t fcode[24];
     const T *bnd;              // bnd_sets sets of bnd_stride values: [v_upper(nx) | h_lower(ny) | h_upper*X'(1) (ny)]
     int64_t coeff_stride, bnd_stride;
 };

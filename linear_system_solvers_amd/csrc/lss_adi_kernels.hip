@@ -34,9 +34,9 @@ __global__ void __launch_bounds__(TILE) k_bnd_a(const Prob<T> *__restrict__ prob
         return;
     const int i = tl.first + threadIdx.x;
     const int nx = p.nx, ny = p.ny, N = nx - 1;
-    const T *u = (step & 1) ? p.b : p.a;
-    const T *cs = coeff_set(p, step);
-    const T *bs = bnd_set(p, step);
+    const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
+    const gptr<const T> cs = coeff_set(p, step);
+    const gptr<const T> bs = bnd_set(p, step);
     const T four = (T)4.0, three = (T)3.0, two = (T)2.0, one = (T)1.0;
     const T delta = p.delta, ni = p.ni, rho = p.rho;
     T v;
@@ -45,7 +45,7 @@ __global__ void __launch_bounds__(TILE) k_bnd_a(const Prob<T> *__restrict__ prob
     } else {
         T D, E, F;
         def_row0(p, cs, i, D, E, F);
-        const T *ui = u + (int64_t)i * ny;
+        const gptr<const T> ui = u + (int64_t)i * ny;
         if (i == N) {
             const T delta_ = two * p.h1 * bs[nx + ny];  // 2 h_1 * h_upper(t, Y(0)) X'(1)
             v = ((one - three * ni * E + rho * F) * ui[0]) + (four * ni * E * ui[1]) - (ni * E * ui[2]) -
@@ -55,8 +55,8 @@ __global__ void __launch_bounds__(TILE) k_bnd_a(const Prob<T> *__restrict__ prob
                 (ni * E * ui[2]) + (four * ni * E * ui[1]);
         }
     }
-    p.colA0[i] = v;
-    p.colA1[i] = bs[i];  // v_upper(t, X(zeta_i))
+    G(p.colA0)[i] = v;
+    G(p.colA1)[i] = bs[i];  // v_upper(t, X(zeta_i))
 }
 
 // -------------------------------------------------------------------------------------------------
@@ -72,8 +72,8 @@ __global__ void __launch_bounds__(TILE) k_bnd_b(const Prob<T> *__restrict__ prob
         return;
     const int j = tl.first + threadIdx.x;
     const int nx = p.nx, ny = p.ny;
-    T *nxt = (step & 1) ? p.a : p.b;
-    const T *bs = bnd_set(p, step);
+    const gptr<T> nxt = G((step & 1) ? p.a : p.b);
+    const gptr<const T> bs = bnd_set(p, step);
     const T two = (T)2.0, three = (T)3.0, four = (T)4.0;
     const uint32_t jj = p.idx2[j];
     const T e = (((four * nxt[(int64_t)(nx - 2) * ny + jj]) - nxt[(int64_t)(nx - 3) * ny + jj] -
@@ -138,15 +138,16 @@ __global__ void __launch_bounds__(TILE) k_xsweep(const Prob<T> *__restrict__ pro
         return;
     const int j = tl.first + threadIdx.x;
     const int nx = p.nx, ny = p.ny, end = nx - 2, tri = p.tri;
-    const T *u = (step & 1) ? p.b : p.a;
-    const T *cs = coeff_set(p, step);
+    const T *u0 = (step & 1) ? p.b : p.a;
+    const gptr<const T> u = G(u0);
+    const gptr<const T> cs = coeff_set(p, step);
     const T one = (T)1.0, theta = p.theta;
     const int kind = (stage == 1) ? 0 : (p.scheme == SCHEME_CS ? 1 : (p.scheme == SCHEME_MCS ? 2 : 3));
-    const T *in = (kind == 3) ? p.y2 : u;
-    const T *inh = (kind == 3) ? p.y3 : p.y1;
-    const T *inhn = p.y2;
-    T *out = (kind == 0) ? p.y1 : (kind == 3 ? p.y4 : p.y3);
-    T *scr = p.scr;
+    const gptr<const T> in = G((kind == 3) ? (const T *)p.y2 : u0);
+    const gptr<const T> inh = G((const T *)((kind == 3) ? p.y3 : p.y1));
+    const gptr<const T> inhn = G((const T *)p.y2);
+    const gptr<T> out = G((kind == 0) ? p.y1 : (kind == 3 ? p.y4 : p.y3));
+    const gptr<T> scr = G(p.scr);
     const uint32_t jj = p.idx2[j];
     const T lo = u[jj], hi = u[(int64_t)(nx - 1) * ny + jj];
     T gprev = (T)0, sprev = (T)0;
@@ -199,30 +200,30 @@ __global__ void __launch_bounds__(TILE) k_vsweep(const Prob<T> *__restrict__ pro
     const int ny = p.ny, end = ny - 2, tri = p.tri;
     const T *u = (step & 1) ? p.b : p.a;
     T *nxt = (step & 1) ? p.a : p.b;
-    const T *cs = coeff_set(p, step);
+    const gptr<const T> cs = coeff_set(p, step);
     const T one = (T)1.0, theta = p.theta;
-    const T *in, *inh;
-    T *out;
+    const T *in0, *inh0;
+    T *out0;
     if (stage == 1) {
-        in = u;
-        inh = p.y1;
-        out = (p.scheme == SCHEME_DR) ? nxt : p.y2;
+        in0 = u;
+        inh0 = p.y1;
+        out0 = (p.scheme == SCHEME_DR) ? nxt : p.y2;
     } else if (p.scheme == SCHEME_HV) {
-        in = p.y2;
-        inh = p.y4;
-        out = nxt;
+        in0 = p.y2;
+        inh0 = p.y4;
+        out0 = nxt;
     } else {
-        in = u;
-        inh = p.y3;
-        out = nxt;
+        in0 = u;
+        inh0 = p.y3;
+        out0 = nxt;
     }
     const int64_t row = (int64_t)i * ny;
     const uint32_t ii = p.idx1[i];
-    const T lo = p.colA0[ii], hi = p.colA1[ii];
-    T *scr = p.scr + row;
-    in += row;
-    inh += row;
-    out += row;
+    const T lo = G(p.colA0)[ii], hi = G(p.colA1)[ii];
+    const gptr<T> scr = G(p.scr + row);
+    const gptr<const T> in = G(in0 + row);
+    const gptr<const T> inh = G(inh0 + row);
+    const gptr<T> out = G(out0 + row);
     T gprev = (T)0, sprev = (T)0;
     for (int t = 1; t <= end; ++t) {
         const Stencil<T> s = stencil_at(p, cs, i, t);
@@ -259,13 +260,15 @@ __global__ void __launch_bounds__(TILE) k_hv_y3(const Prob<T> *__restrict__ prob
         return;
     const int r = tl.first;
     const int ny = p.ny;
-    const T *u = (step & 1) ? p.b : p.a;
-    const T *cs = coeff_set(p, step);
+    const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
+    const gptr<const T> cs = coeff_set(p, step);
+    const gptr<const T> y1 = G((const T *)p.y1), y2 = G((const T *)p.y2);
+    const gptr<T> y3 = G(p.y3);
     for (int c = 1 + threadIdx.x; c < ny - 1; c += blockDim.x) {
         const int64_t q = (int64_t)r * ny + c;
         const Stencil<T> s = stencil_at(p, cs, r, c);
-        const T val = hv_y3_at(p, s, u + q, p.y1 + q, p.y2 + q, ny);
-        p.y3[q] = val;
+        const T val = hv_y3_at(p, s, u + q, y1 + q, y2 + q, ny);
+        y3[q] = val;
     }
 }
 

@@ -21,6 +21,9 @@
 // The arithmetic of each point (coefficients, right-hand sides) is the reference's; only the
 // elimination order of the line solve differs from the reference's sequential Thomas sweep.
 
+// The fast path may fuse multiply-adds (the library is built with -ffp-contract=off so that the exact
+// line solver keeps the reference's rounding; this translation unit opts back in).
+#pragma clang fp contract(fast)
 #include "lss_adi_chunk.hpp"
 #include "lss_adi_stencil.hpp"
 
@@ -44,29 +47,29 @@ __global__ void __launch_bounds__(64 * VROWS) k_vsweep_p(const Prob<T> *__restri
     const int m = (N + NCH - 1) / NCH;
     const T *u = (step & 1) ? p.b : p.a;
     T *nxt = (step & 1) ? p.a : p.b;
-    const T *cs = coeff_set(p, step);
+    const gptr<const T> cs = coeff_set(p, step);
     const T one = (T)1.0, theta = p.theta;
-    const T *in, *inh;
-    T *out;
+    const T *in0, *inh0;
+    T *out0;
     if (stage == 1) {
-        in = u;
-        inh = p.y1;
-        out = (p.scheme == SCHEME_DR) ? nxt : p.y2;
+        in0 = u;
+        inh0 = p.y1;
+        out0 = (p.scheme == SCHEME_DR) ? nxt : p.y2;
     } else if (p.scheme == SCHEME_HV) {
-        in = p.y2;
-        inh = p.y4;
-        out = nxt;
+        in0 = p.y2;
+        inh0 = p.y4;
+        out0 = nxt;
     } else {
-        in = u;
-        inh = p.y3;
-        out = nxt;
+        in0 = u;
+        inh0 = p.y3;
+        out0 = nxt;
     }
     const int64_t row = (int64_t)i * ny;
-    in += row;
-    inh += row;
-    out += row;
+    const gptr<const T> in = G(in0 + row);
+    const gptr<const T> inh = G(inh0 + row);
+    const gptr<T> out = G(out0 + row);
     const uint32_t ii = p.idx1[i];
-    const T lo = p.colA0[ii], hi = p.colA1[ii];
+    const T lo = G(p.colA0)[ii], hi = G(p.colA1)[ii];
     const int s = 1 + lane * m;
     Chunk<T, M> ch;
     ch.cnt = max(0, min(m, N - s + 1));
@@ -139,10 +142,11 @@ __global__ void __launch_bounds__(64 * VROWS) k_vsweep_p(const Prob<T> *__restri
 // kind 3: HV rhs_intermed_4(Y2, Y3) with Y3 evaluated in place from (u, Y1, Y2) -> Y4 (= y4).
 // Ends u(0, idx2[j]), u(nx-1, idx2[j]) (implicit_kernel.hpp:86-101).
 // -------------------------------------------------------------------------------------------------
-template <typename T, int M>
+template <typename T, int M, int STAGE>
 __global__ void __launch_bounds__(XTJ *NCH) k_xsweep_p(const Prob<T> *__restrict__ probs,
-                                                       const Tile *__restrict__ tiles, int step, int stage)
+                                                       const Tile *__restrict__ tiles, int step)
 {
+    constexpr int stage = STAGE;
     __shared__ T sh[2][4][NCH][XTJ];
     const Tile tl = tiles[blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
@@ -153,11 +157,12 @@ __global__ void __launch_bounds__(XTJ *NCH) k_xsweep_p(const Prob<T> *__restrict
     const int j = tl.first + (colok ? col : 0);
     const int nx = p.nx, ny = p.ny, N = nx - 2;
     const int m = (N + NCH - 1) / NCH;
-    const T *u = (step & 1) ? p.b : p.a;
-    const T *cs = coeff_set(p, step);
+    const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
+    const gptr<const T> cs = coeff_set(p, step);
+    const gptr<const T> y1 = G((const T *)p.y1), y2 = G((const T *)p.y2);
     const T one = (T)1.0, theta = p.theta;
     const int kind = (stage == 1) ? 0 : (p.scheme == SCHEME_CS ? 1 : (p.scheme == SCHEME_MCS ? 2 : 3));
-    T *out = (kind == 0) ? p.y1 : (kind == 3 ? p.y4 : p.y3);
+    const gptr<T> out = G((kind == 0) ? p.y1 : (kind == 3 ? p.y4 : p.y3));
     const uint32_t jj = p.idx2[j];
     const T lo = u[jj], hi = u[(int64_t)(nx - 1) * ny + jj];
     const int s = 1 + chn * m;
@@ -177,13 +182,13 @@ __global__ void __launch_bounds__(XTJ *NCH) k_xsweep_p(const Prob<T> *__restrict
             if (kind == 0) {
                 e.r = rhs_intermed_1(p, st, u + q, ny);
             } else if (kind == 1) {
-                e.r = rhs_intermed_3<T, false>(p, st, u + q, p.y1 + q, p.y2 + q, ny);
+                e.r = rhs_intermed_3<T, false>(p, st, u + q, y1 + q, y2 + q, ny);
             } else if (kind == 2) {
-                e.r = rhs_intermed_3<T, true>(p, st, u + q, p.y1 + q, p.y2 + q, ny);
+                e.r = rhs_intermed_3<T, true>(p, st, u + q, y1 + q, y2 + q, ny);
             } else {
-                const T *y2 = p.y2 + q;
-                const T y3 = hv_y3_at(p, st, u + q, p.y1 + q, y2, ny);
-                e.r = (-theta * st.M * y2[-ny]) + (theta * st.Z * y2[0]) - (theta * st.P * y2[ny]) + y3;
+                const gptr<const T> y2q = y2 + q;
+                const T y3 = hv_y3_at(p, st, u + q, y1 + q, y2q, ny);
+                e.r = (-theta * st.M * y2q[-ny]) + (theta * st.Z * y2q[0]) - (theta * st.P * y2q[ny]) + y3;
             }
             if (k == 1) {
                 e.r = e.r - e.a * lo;
@@ -260,7 +265,8 @@ __global__ void __launch_bounds__(XTJ *NCH) k_xsweep_p(const Prob<T> *__restrict
 
 #define LSS_INST_P(T, M)                                                                  \
     template __global__ void k_vsweep_p<T, M>(const Prob<T> *, const Tile *, int, int);   \
-    template __global__ void k_xsweep_p<T, M>(const Prob<T> *, const Tile *, int, int);
+    template __global__ void k_xsweep_p<T, M, 1>(const Prob<T> *, const Tile *, int);                     \
+    template __global__ void k_xsweep_p<T, M, 2>(const Prob<T> *, const Tile *, int);
 LSS_INST_P(double, 1)
 LSS_INST_P(double, 2)
 LSS_INST_P(double, 4)

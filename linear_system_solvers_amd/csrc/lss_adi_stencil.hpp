@@ -10,15 +10,25 @@ template <typename T> struct Stencil {
     T M, Mt, P, Pt, Z, W, C;
 };
 
-template <typename T>
-__device__ __forceinline__ T sep(const T *X, const T *Y, int nx, int ny, int k, int i, int j)
+// factor slot `slot` of a packed SEPARABLE block at index i (codes: Prob::fcode)
+template <typename T, typename P>
+__device__ __forceinline__ T factor(P B, int code, int n, int i)
 {
-    return X[(2 * k) * nx + i] * Y[(2 * k) * ny + j] + X[(2 * k + 1) * nx + i] * Y[(2 * k + 1) * ny + j];
+    return code >= 0 ? B[(int64_t)code * n + i] : (code == -2 ? (T)1 : (T)0);
+}
+
+// q(i, j) = X[2k][i] Y[2k][j] + X[2k+1][i] Y[2k+1][j]
+template <typename T, typename P>
+__device__ __forceinline__ T sep(const Prob<T> &p, P X, P Y, int k, int i, int j)
+{
+    const int32_t *c = p.fcode;
+    return factor<T>(X, c[2 * k], p.nx, i) * factor<T>(Y, c[12 + 2 * k], p.ny, j) +
+           factor<T>(X, c[2 * k + 1], p.nx, i) * factor<T>(Y, c[12 + 2 * k + 1], p.ny, j);
 }
 
 // Stencil coefficients at (zeta_i, eta_j): general_heston_equation_coefficients (coefficients.hpp:89-95).
-template <typename T>
-__device__ __forceinline__ Stencil<T> stencil_at(const Prob<T> &p, const T *__restrict__ cs, int i, int j)
+template <typename T, typename P>
+__device__ __forceinline__ Stencil<T> stencil_at(const Prob<T> &p, P cs, int i, int j)
 {
     Stencil<T> s;
     if (p.coeff_mode == COEFF_FIELDS) {
@@ -31,10 +41,10 @@ __device__ __forceinline__ Stencil<T> stencil_at(const Prob<T> &p, const T *__re
         s.W = cs[5 * nxy + q];
         s.C = cs[6 * nxy + q];
     } else {
-        const T *X = cs, *Y = cs + 12 * (int64_t)p.nx;
-        const T a = sep(X, Y, p.nx, p.ny, 0, i, j), b = sep(X, Y, p.nx, p.ny, 1, i, j);
-        const T c = sep(X, Y, p.nx, p.ny, 2, i, j), d = sep(X, Y, p.nx, p.ny, 3, i, j);
-        const T e = sep(X, Y, p.nx, p.ny, 4, i, j), f = sep(X, Y, p.nx, p.ny, 5, i, j);
+        const P X = cs, Y = cs + 12 * (int64_t)p.nx;
+        const T a = sep<T>(p, X, Y, 0, i, j), b = sep<T>(p, X, Y, 1, i, j);
+        const T c = sep<T>(p, X, Y, 2, i, j), d = sep<T>(p, X, Y, 3, i, j);
+        const T e = sep<T>(p, X, Y, 4, i, j), f = sep<T>(p, X, Y, 5, i, j);
         const T two = (T)2.0, half = (T)0.5;
         s.M = (p.alpha * a - p.delta * d);
         s.Mt = (p.beta * b - p.ni * e);
@@ -48,37 +58,37 @@ __device__ __forceinline__ Stencil<T> stencil_at(const Prob<T> &p, const T *__re
 }
 
 // D, E, F on the v = 0 row at zeta_i (used by boundary stage A).
-template <typename T>
-__device__ __forceinline__ void def_row0(const Prob<T> &p, const T *__restrict__ cs, int i, T &D, T &E, T &F)
+template <typename T, typename P>
+__device__ __forceinline__ void def_row0(const Prob<T> &p, P cs, int i, T &D, T &E, T &F)
 {
     if (p.coeff_mode == COEFF_FIELDS) {
-        const T *r = cs + 7 * (int64_t)p.nx * p.ny;
+        const P r = cs + 7 * (int64_t)p.nx * p.ny;
         D = r[i];
         E = r[p.nx + i];
         F = r[2 * p.nx + i];
     } else {
-        const T *X = cs, *Y = cs + 12 * (int64_t)p.nx;
-        D = sep(X, Y, p.nx, p.ny, 3, i, 0);
-        E = sep(X, Y, p.nx, p.ny, 4, i, 0);
-        F = sep(X, Y, p.nx, p.ny, 5, i, 0);
+        const P X = cs, Y = cs + 12 * (int64_t)p.nx;
+        D = sep<T>(p, X, Y, 3, i, 0);
+        E = sep<T>(p, X, Y, 4, i, 0);
+        F = sep<T>(p, X, Y, 5, i, 0);
     }
 }
 
-template <typename T> __device__ __forceinline__ const T *coeff_set(const Prob<T> &p, int step)
+template <typename T> __device__ __forceinline__ gptr<const T> coeff_set(const Prob<T> &p, int step)
 {
-    return p.coeff + (p.coeff_sets > 1 ? (int64_t)step : 0) * p.coeff_stride;
+    return G(p.coeff + (p.coeff_sets > 1 ? (int64_t)step : 0) * p.coeff_stride);
 }
-template <typename T> __device__ __forceinline__ const T *bnd_set(const Prob<T> &p, int step)
+template <typename T> __device__ __forceinline__ gptr<const T> bnd_set(const Prob<T> &p, int step)
 {
-    return p.bnd + (p.bnd_sets > 1 ? (int64_t)step : 0) * p.bnd_stride;
+    return G(p.bnd + (p.bnd_sets > 1 ? (int64_t)step : 0) * p.bnd_stride);
 }
 
 // -------------------------------------------------------------------------------------------------
 // Right-hand sides (operand order of the reference's rhs functions)
 // -------------------------------------------------------------------------------------------------
 // rhs_intermed_1 (lss_heat_douglas_rachford_method.hpp:37-68); `in` points at (t, j), row stride ny
-template <typename T>
-__device__ __forceinline__ T rhs_intermed_1(const Prob<T> &p, const Stencil<T> &s, const T *in, int ny)
+template <typename T, typename P>
+__device__ __forceinline__ T rhs_intermed_1(const Prob<T> &p, const Stencil<T> &s, P in, int ny)
 {
     const T one = (T)1.0, gamma = p.gamma, theta = p.theta;
     return (gamma * s.C * in[-ny - 1]) + ((one - theta) * s.M * in[-ny]) - (gamma * s.C * in[-ny + 1]) +
@@ -88,9 +98,8 @@ __device__ __forceinline__ T rhs_intermed_1(const Prob<T> &p, const Stencil<T> &
 
 // CS rhs_intermed_3 (lss_heat_craig_sneyd_method.hpp:128-156), MCS adds the (zeta - theta) term
 // of lss_heat_modified_craig_sneyd_method.hpp:158-163 (with its "- P")
-template <typename T, bool MCS>
-__device__ __forceinline__ T rhs_intermed_3(const Prob<T> &p, const Stencil<T> &s, const T *in, const T *inh,
-                                            const T *inhn, int ny)
+template <typename T, bool MCS, typename P>
+__device__ __forceinline__ T rhs_intermed_3(const Prob<T> &p, const Stencil<T> &s, P in, P inh, P inhn, int ny)
 {
     const T gamma = p.gamma, theta = p.theta, zeta = p.zeta;
     T v = (-theta * s.M * in[-ny]) + (theta * s.Z * in[0]) - (theta * s.P * in[ny]) + (inh[0]) +
@@ -106,16 +115,16 @@ __device__ __forceinline__ T rhs_intermed_3(const Prob<T> &p, const Stencil<T> &
 }
 
 // HV rhs_intermed_4 (lss_heat_hundsdorfer_verwer_method.hpp:174-194)
-template <typename T>
-__device__ __forceinline__ T rhs_intermed_4(const Prob<T> &p, const Stencil<T> &s, const T *in, const T *inh, int ny)
+template <typename T, typename P>
+__device__ __forceinline__ T rhs_intermed_4(const Prob<T> &p, const Stencil<T> &s, P in, P inh, int ny)
 {
     const T theta = p.theta;
     return (-theta * s.M * in[-ny]) + (theta * s.Z * in[0]) - (theta * s.P * in[ny]) + inh[0];
 }
 
 // v-direction rhs shared by every scheme (e.g. lss_heat_douglas_rachford_method.hpp:106-126)
-template <typename T>
-__device__ __forceinline__ T rhs_v(const Prob<T> &p, const Stencil<T> &s, const T *in, const T *inh)
+template <typename T, typename P>
+__device__ __forceinline__ T rhs_v(const Prob<T> &p, const Stencil<T> &s, P in, P inh)
 {
     const T theta = p.theta;
     return (-theta * s.Mt * in[-1]) + (theta * s.W * in[0]) - (theta * s.Pt * in[1]) + inh[0];
@@ -123,9 +132,8 @@ __device__ __forceinline__ T rhs_v(const Prob<T> &p, const Stencil<T> &s, const 
 
 // HV explicit stage value Y3 at a point (lss_heat_hundsdorfer_verwer_method.hpp:146-168);
 // in = u, inh = Y1, inhn = Y2, each pointing at the point.
-template <typename T>
-__device__ __forceinline__ T hv_y3_at(const Prob<T> &p, const Stencil<T> &s, const T *in, const T *inh,
-                                      const T *inhn, int ny)
+template <typename T, typename P>
+__device__ __forceinline__ T hv_y3_at(const Prob<T> &p, const Stencil<T> &s, P in, P inh, P inhn, int ny)
 {
     const T one = (T)1.0, gamma = p.gamma, theta = p.theta, zeta = p.zeta;
     return (-theta * s.M * inh[-ny]) + ((one + theta * s.Z) * inh[0]) - (theta * s.P * inh[ny]) +

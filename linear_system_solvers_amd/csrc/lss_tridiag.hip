@@ -10,6 +10,9 @@
 //   k_tridiag_cols  strided layout x[i*B + k]: XTJ systems x NCH chunks per workgroup, so that each
 //                   access reads XTJ consecutive systems of one row.
 
+// The fast path may fuse multiply-adds (the library is built with -ffp-contract=off so that the exact
+// line solver keeps the reference's rounding; this translation unit opts back in).
+#pragma clang fp contract(fast)
 #include "lss_adi_chunk.hpp"
 
 namespace lssamd {

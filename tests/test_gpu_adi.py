@@ -150,10 +150,12 @@ def test_partitioned_matches_oracle(oracle, case, mode):
     assert _normwise(g, c) <= 1e-12
 
 
-def _stable(nx, ny, steps=10, **kw):
+def _stable(nx, ny, steps=10, fac=1.5, **kw):
     """Heston DR/CS/MCS/HV problem whose time step keeps the reference's explicit v = 0 row stable
-    (3 nu E <= 2 with the uniform-hints v map: k <= 1.5 h2 for kappa = 3, theta_v = 0.2)."""
-    k = min(0.01, 1.5 / (ny - 1))
+    (3 nu E <= 2 with the uniform-hints v map: k <= 1.5 h2 for kappa = 3, theta_v = 0.2).  MCS with
+    zeta != theta is unstable in the reference at moderate sizes (its "- P" correction term amplifies),
+    so the sweeps below run MCS at zeta = theta and test the (zeta - theta) term separately."""
+    k = min(0.01, fac / (ny - 1))
     return config0(nx=nx, ny=ny, nt=steps + 1, t_hi=k * steps, **kw)
 
 
@@ -165,7 +167,20 @@ def _assert_stable(c, m):
                                    (1026, 70), (70, 1026)])
 @pytest.mark.parametrize("splitting", [0, 1, 2, 3])
 def test_partitioned_sizes_and_schemes(oracle, nx, ny, splitting):
-    m = _stable(nx, ny, splitting=splitting, weighting=0.3, coeff_mode=L.COEFF_SEPARABLE,
+    m = _stable(nx, ny, splitting=splitting, weighting=0.5 if splitting >= 2 else 0.3, coeff_mode=L.COEFF_SEPARABLE,
+                line_solver=L.LINE_PARTITIONED)
+    g = _solve(m)
+    c = oracle_solve(oracle, m)
+    _assert_stable(c, m)
+    assert _normwise(g, c) <= 1e-12
+
+
+@pytest.mark.parametrize("nx,ny", [(4, 4), (6, 5), (9, 7), (66, 66), (67, 130)])
+@pytest.mark.parametrize("splitting", [2, 3])
+def test_partitioned_zeta_terms(oracle, nx, ny, splitting):
+    """MCS and HV with zeta != theta (MCS's (zeta - theta) correction with its "- P"; HV's zero-border
+    Y2 rows) at time steps where the reference scheme stays bounded."""
+    m = _stable(nx, ny, fac=0.2, splitting=splitting, weighting=0.3, coeff_mode=L.COEFF_SEPARABLE,
                 line_solver=L.LINE_PARTITIONED)
     g = _solve(m)
     c = oracle_solve(oracle, m)
@@ -176,7 +191,8 @@ def test_partitioned_sizes_and_schemes(oracle, nx, ny, splitting):
 @pytest.mark.parametrize("fp,tol", [(L.FP64, 1e-13), (L.FP32, 1e-5)])
 def test_partitioned_agrees_with_exact_sweeps_on_gpu(oracle, fp, tol):
     for splitting in range(4):
-        m = _stable(150, 60, steps=50, splitting=splitting, weighting=0.4, coeff_mode=L.COEFF_FIELDS)
+        m = _stable(150, 60, steps=50, splitting=splitting, weighting=0.5 if splitting >= 2 else 0.4,
+                    coeff_mode=L.COEFF_FIELDS)
         ex = _solve(m, fp)
         _assert_stable(ex.astype(np.float64), m)
         m.line_solver = L.LINE_PARTITIONED
