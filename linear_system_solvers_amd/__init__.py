@@ -35,7 +35,8 @@ MODEL_HESTON, MODEL_SABR = 0, 1
 EXPORTS = ("lss_last_error", "lss_version", "lss_adi_plan_create", "lss_adi_plan_destroy", "lss_adi_run_dev",
            "lss_adi_run_host", "lss_model_plan_create", "lss_model_initial", "lss_model_grid", "lss_plan_problem_shape",
            "lss_tridiag_batched", "lss_adi_run_profiled", "lss_adi_set_param_terms", "lss_plan_param_count",
-           "lss_adi_forward", "lss_adi_backward")
+           "lss_adi_forward", "lss_adi_backward", "lss_adi_autotune",
+           "lss_adi_set_mapping")
 MAX_PARAMS = 6
 MAX_TERMS = 12
 HESTON_PARAMS = ("kappa", "theta_v", "sigma", "rho", "r")
@@ -157,6 +158,9 @@ def lib():
         L.lss_adi_set_param_terms.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                               C.POINTER(C.c_int32), C.c_int64, C.c_void_p]
         L.lss_plan_param_count.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]
+        L.lss_adi_set_mapping.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.lss_adi_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32)]
         L.lss_adi_forward.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32,
                                       C.c_void_p]
         L.lss_adi_backward.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
@@ -237,6 +241,15 @@ class Plan:
         pout = (C.c_void_p * self.count)(*out_ptrs)
         _check(lib().lss_adi_run_dev(self._h, pin, pout, C.c_void_p(stream), int(graph)))
 
+
+    def autotune(self, stream: int = 0, again: bool = False):
+        """Tile/lane mapping chosen by the plan's autotuner: (S-lines per x-sweep group, v-lines per v-sweep group)."""
+        x, v = C.c_int32(), C.c_int32()
+        _check(lib().lss_adi_autotune(self._h, C.c_void_p(stream), int(again), C.byref(x), C.byref(v)))
+        return x.value, v.value
+
+    def set_mapping(self, s_lines_per_group: int, v_lines_per_group: int) -> None:
+        _check(lib().lss_adi_set_mapping(self._h, s_lines_per_group, v_lines_per_group))
 
     # ---- reverse mode -------------------------------------------------------------------------
     def param_count(self, p: int) -> int:

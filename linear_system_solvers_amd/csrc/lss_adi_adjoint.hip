@@ -399,7 +399,7 @@ __global__ void __launch_bounds__(XTJ *NCH) k_adj_x(const Prob<T> *__restrict__ 
         }
         add_terms(p, A, step, k, j, g, acc);
     };
-    block_solve<T, M>(sh, N, col, chn, colok, eq, put);
+    block_solve<T, M, XTJ>(sh, N, col, chn, colok, eq, put);
     block_acc_store<XTJ * NCH / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
 

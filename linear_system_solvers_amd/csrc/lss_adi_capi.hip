@@ -33,8 +33,8 @@ template <typename T> __global__ void k_vsweep(const Prob<T> *, const Tile *, in
 template <typename T> __global__ void k_hv_y3(const Prob<T> *, const Tile *, int);
 template <typename T>
 __global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *);
-template <typename T, int M, int STAGE> __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int);
-template <typename T, int M> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int);
+template <typename T, int M, int STAGE, int W> __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int);
+template <typename T, int M, int VR> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int);
 template <typename T, int M>
 __global__ void k_adj_v(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
 template <typename T, int M>
@@ -141,6 +141,8 @@ struct lss_adi_plan {
     virtual void forward_ckpt(const void *const *u_in, void *const *u_out, int ckpt, hipStream_t stream) = 0;
     virtual void backward(const void *const *lam_out, void *const *lam_in, double *dparams, hipStream_t stream) = 0;
     virtual void invalidate_adjoint() = 0;
+    virtual void tune(hipStream_t stream, bool again, int *xw, int *vr) = 0;
+    virtual void set_mapping(int xw, int vr) = 0;
     virtual void run(const void *const *u_in, void *const *u_out, hipStream_t stream, bool use_graph, bool host) = 0;
     virtual void profile(const void *const *u_in, void *const *u_out, hipStream_t stream, double *ms, double *bytes,
                          int64_t *launches) = 0;
@@ -180,13 +182,16 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     dev_buffer probs_dev, tiles_dev;
     // tile tables: rows [0,nx), cols [0,ny), S-lines [1,ny-2], v-lines [1,nx-2], interior rows
     // adjoint: S-line tiles of P_XTJ, v-line tiles of P_VROWS, one tile per row (all rows)
-    enum { TB_I = 0, TB_J, TB_X, TB_V, TB_R, TB_AX, TB_AV, TB_U, TB_N };
+    // forward partitioned sweeps: S-line tiles of 16 (TB_X) or 8 (TB_X8), v-line tiles of 4 (TB_V) or 16 (TB_V16)
+    enum { TB_I = 0, TB_J, TB_X, TB_V, TB_R, TB_AX, TB_AV, TB_U, TB_X8, TB_V16, TB_N };
     std::vector<Tile> tiles[TB_N];
     size_t tile_off[TB_N] = {};
     std::vector<int> prefix[TB_N];     // prefix[tb][q] = tiles of internal problems [0, q)
     int max_steps = 0;
     int line_solver = LINE_EXACT;
     int pmx = 1, pmy = 1;              // points per chunk bound (template M) of S-lines and v-lines
+    int xw = 16, vr = 4;               // S-lines per x-sweep workgroup, v-lines per v-sweep workgroup
+    bool tuned = false;                // the autotuner has chosen xw and vr
     bool adj_ok = false;               // every line fits the partitioned solver (needed by the adjoint)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
@@ -395,6 +400,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 case TB_AX: lo = 1, hi = p.ny - 1, step = P_XTJ; break;
                 case TB_AV: lo = 1, hi = p.nx - 1, step = P_VROWS; break;
                 case TB_U: lo = 0, hi = p.nx, step = 1; break;
+                case TB_X8: lo = 1, hi = p.ny - 1, step = 8; break;
+                case TB_V16: lo = 1, hi = p.nx - 1, step = 16; break;
                 default: lo = 1, hi = p.nx - 1, step = 1; break;
                 }
                 for (int s = lo; s < hi; s += step) {
@@ -457,6 +464,126 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         return b * sizeof(T);
     }
 
+    // partitioned sweeps with the chosen tile/lane mapping (xw, vr) and chunk bound (pmx, pmy)
+    template <int M, int W> void launch_x_m(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
+    {
+        const int tb = W == 16 ? TB_X : TB_X8;
+        const dim3 g((unsigned)prefix[tb][na]), b(W * P_NCH);
+        const Tile *tx = (const Tile *)tiles_dev.p + tile_off[tb];
+        if (stage == 1)
+            hipLaunchKernelGGL((k_xsweep_p<T, M, 1, W>), g, b, 0, st, pd, tx, s);
+        else
+            hipLaunchKernelGGL((k_xsweep_p<T, M, 2, W>), g, b, 0, st, pd, tx, s);
+    }
+    template <int W> void launch_x_w(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
+    {
+        switch (pmx) {
+        case 1: launch_x_m<1, W>(stage, s, st, pd, na); break;
+        case 2: launch_x_m<2, W>(stage, s, st, pd, na); break;
+        case 4: launch_x_m<4, W>(stage, s, st, pd, na); break;
+        case 8: launch_x_m<8, W>(stage, s, st, pd, na); break;
+        default: launch_x_m<16, W>(stage, s, st, pd, na); break;
+        }
+    }
+    void launch_x(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
+    {
+        if (xw == 8)
+            launch_x_w<8>(stage, s, st, pd, na);
+        else
+            launch_x_w<16>(stage, s, st, pd, na);
+    }
+    template <int M, int VR> void launch_v_m(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
+    {
+        const int tb = VR == 4 ? TB_V : TB_V16;
+        const dim3 g((unsigned)prefix[tb][na]), b(64 * VR);
+        hipLaunchKernelGGL((k_vsweep_p<T, M, VR>), g, b, 0, st, pd, (const Tile *)tiles_dev.p + tile_off[tb], s,
+                           stage);
+    }
+    template <int VR> void launch_v_r(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
+    {
+        switch (pmy) {
+        case 1: launch_v_m<1, VR>(stage, s, st, pd, na); break;
+        case 2: launch_v_m<2, VR>(stage, s, st, pd, na); break;
+        case 4: launch_v_m<4, VR>(stage, s, st, pd, na); break;
+        case 8: launch_v_m<8, VR>(stage, s, st, pd, na); break;
+        default: launch_v_m<16, VR>(stage, s, st, pd, na); break;
+        }
+    }
+    void launch_v(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
+    {
+        if (vr == 16)
+            launch_v_r<16>(stage, s, st, pd, na);
+        else
+            launch_v_r<4>(stage, s, st, pd, na);
+    }
+
+    void drop_graph()
+    {
+        if (graph_exec) {  // the captured loop uses the previous mapping
+            HIPCHK(hipGraphExecDestroy(graph_exec));
+            graph_exec = nullptr;
+            HIPCHK(hipGraphDestroy(graph));
+            graph = nullptr;
+        }
+    }
+
+    void tune(hipStream_t stream, bool again, int *x, int *v) override
+    {
+        if (!tuned || again) {
+            drop_graph();
+            autotune(stream);
+        }
+        *x = xw;
+        *v = vr;
+    }
+
+    void set_mapping(int x, int v) override
+    {
+        require((x == 16 || x == 8) && (v == 4 || v == 16), "mapping must be (16 or 8, 4 or 16)");
+        drop_graph();
+        xw = x;
+        vr = v;
+        tuned = true;
+    }
+
+    // Autotuner: times every (xw, vr) mapping on the first steps of the loop (at most 8, from the
+    // plan's current state) and keeps the fastest.  The mappings give identical results.
+    void autotune(hipStream_t st)
+    {
+        tuned = true;
+        if (line_solver != LINE_PARTITIONED)
+            return;
+        const int nst = std::min(max_steps, 8);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        float best = 1e30f;
+        int bx = 16, bv = 4;
+        for (int cx : {16, 8})
+            for (int cv : {4, 16}) {
+                xw = cx;
+                vr = cv;
+                for (int rep = 0; rep < 2; ++rep) {  // the first pass warms caches and code
+                    HIPCHK(hipEventRecord(e0, st));
+                    for (int s = 0; s < nst; ++s)
+                        launch_step(s, st);
+                    HIPCHK(hipEventRecord(e1, st));
+                }
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    bx = cx;
+                    bv = cv;
+                }
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        xw = bx;
+        vr = bv;
+    }
+
     // one loop step of every active problem; `table` replaces the plan's problem table (the adjoint's
     // recomputation writes each step's stage outputs into their own buffers)
     void launch_step(int s, hipStream_t st, Profiler *prof = nullptr, const Prob<T> *table = nullptr)
@@ -486,37 +613,14 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 hipLaunchKernelGGL(k_xsweep<T>, dim3(grid(TB_X)), dim3(TILE), 0, st, pd, td + tile_off[TB_X], s, stage);
                 return;
             }
-            const dim3 g(grid(TB_X)), b(P_XTJ * P_NCH);
-            const Tile *tx = td + tile_off[TB_X];
-#define LSS_XS(M)                                                                                              \
-    do {                                                                                                       \
-        if (stage == 1)                                                                                        \
-            hipLaunchKernelGGL((k_xsweep_p<T, M, 1>), g, b, 0, st, pd, tx, s);                                 \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_xsweep_p<T, M, 2>), g, b, 0, st, pd, tx, s);                                 \
-    } while (0)
-            switch (pmx) {
-            case 1: LSS_XS(1); break;
-            case 2: LSS_XS(2); break;
-            case 4: LSS_XS(4); break;
-            case 8: LSS_XS(8); break;
-            default: LSS_XS(16); break;
-            }
-#undef LSS_XS
+            launch_x(stage, s, st, pd, na);
         };
         auto vs = [&](int stage) {
             if (line_solver == LINE_EXACT) {
                 hipLaunchKernelGGL(k_vsweep<T>, dim3(grid(TB_V)), dim3(TILE), 0, st, pd, td + tile_off[TB_V], s, stage);
                 return;
             }
-            const dim3 g(grid(TB_V)), b(64 * P_VROWS);
-            switch (pmy) {
-            case 1: hipLaunchKernelGGL((k_vsweep_p<T, 1>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
-            case 2: hipLaunchKernelGGL((k_vsweep_p<T, 2>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
-            case 4: hipLaunchKernelGGL((k_vsweep_p<T, 4>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
-            case 8: hipLaunchKernelGGL((k_vsweep_p<T, 8>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
-            default: hipLaunchKernelGGL((k_vsweep_p<T, 16>), g, b, 0, st, pd, td + tile_off[TB_V], s, stage); break;
-            }
+            launch_v(stage, s, st, pd, na);
         };
         pre(LSS_K_XSWEEP1);
         xs(1);
@@ -813,6 +917,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             require(u_in[c] && u_out[c], "null surface pointer");
         if (ck != K)
             adj_setup(K);
+        if (!tuned)
+            autotune(stream);
         fwd_done = false;
         for (int64_t q = 0; q < count; ++q) {
             const Prob<T> &p = probs[q];
@@ -931,6 +1037,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             require(u_in[c] && u_out[c], "null surface pointer");
         const hipMemcpyKind kin = host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
         const hipMemcpyKind kout = host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+        if (!tuned)
+            autotune(stream);
         for (int64_t q = 0; q < count; ++q) {
             const Prob<T> &p = probs[q];
             HIPCHK(hipMemcpyAsync(p.a, u_in[order[q]], sizeof(T) * (size_t)p.nx * p.ny, kin, stream));
@@ -1428,6 +1536,26 @@ int lss_adi_set_param_terms(lss_adi_plan *plan, int64_t p, int32_t nparam, int32
             std::memcpy(tm.fac.data(), factors, tm.fac.size());
         plan->terms[p] = std::move(tm);
         plan->invalidate_adjoint();
+    });
+}
+
+int lss_adi_autotune(lss_adi_plan *plan, void *stream, int32_t again, int32_t *s_lines_per_group,
+                     int32_t *v_lines_per_group)
+{
+    return guarded([&] {
+        require(plan && s_lines_per_group && v_lines_per_group, "null pointer");
+        int x = 0, v = 0;
+        plan->tune((hipStream_t)stream, again != 0, &x, &v);
+        *s_lines_per_group = x;
+        *v_lines_per_group = v;
+    });
+}
+
+int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v_lines_per_group)
+{
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        plan->set_mapping(s_lines_per_group, v_lines_per_group);
     });
 }
 

@@ -209,11 +209,11 @@ __device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put
             put(s + t, ch.value(t, xl, xi));
 }
 
-// XTJ lines per workgroup of XTJ * NCH threads (thread = (chunk chn, line col)); the interface
+// W lines per workgroup of W * NCH threads (thread = (chunk chn, line col)); the interface
 // system is reduced through LDS `sh`.  Every thread of the workgroup must call it (it holds
 // barriers); threads of lines that do not exist pass active = false.
-template <typename T, int M, typename EqF, typename PutF>
-__device__ __forceinline__ void block_solve(T (*sh)[4][NCH][XTJ], int N, int col, int chn, bool active, EqF &&eq,
+template <typename T, int M, int W, typename EqF, typename PutF>
+__device__ __forceinline__ void block_solve(T (*sh)[4][NCH][W], int N, int col, int chn, bool active, EqF &&eq,
                                             PutF &&put)
 {
     const int m = (N + NCH - 1) / NCH;

@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(XTJ *NCH) k_tridiag_cols(int64_t B, int n, con
         return e;
     };
     auto put = [&](int kk, T v) { x[(int64_t)(kk - 1) * B + k] = v; };
-    block_solve<T, M>(sh, n, col, chn, live, eq, put);
+    block_solve<T, M, XTJ>(sh, n, col, chn, live, eq, put);
 }
 
 #define LSS_INST_T(T, M)                                                                                        \

@@ -211,3 +211,25 @@ def test_partitioned_book_matches_oracle(oracle):
         c = oracle_solve(oracle, m)
         _assert_stable(c, m)
         assert _normwise(g, c) <= 1e-12
+
+
+def test_every_tile_mapping_gives_identical_results():
+    """The autotuner's candidate tile/lane mappings change only the work distribution."""
+    rng = np.random.default_rng(21)
+    models = [_stable(int(rng.integers(5, 300)), int(rng.integers(5, 150)), steps=int(rng.integers(1, 12)),
+                      splitting=q % 4, coeff_mode=L.COEFF_SEPARABLE, line_solver=L.LINE_PARTITIONED)
+              for q in range(12)]
+    plan = L.Plan.from_models(models)
+    chosen = plan.autotune()
+    assert chosen[0] in (16, 8) and chosen[1] in (4, 16)
+    ref = None
+    for x in (16, 8):
+        for v in (4, 16):
+            plan.set_mapping(x, v)
+            outs = plan.run_host()
+            if ref is None:
+                ref = outs
+            for a, b in zip(ref, outs):
+                np.testing.assert_array_equal(a, b)
+    with pytest.raises(L.LssError):
+        plan.set_mapping(32, 4)
