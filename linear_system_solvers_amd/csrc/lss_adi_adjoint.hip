@@ -6,9 +6,10 @@
 // points it touches, so the adjoint step runs the stages backwards:
 //   k_adj_v (last)  B^T and the transposed solve of the last v-stage:   lambda(U_{s+1}) -> lv2, elo2
 //   k_adj_x (2)     transposed solve of X2 (CS, MCS, HV):               lv2 -> lx2
-//   k_adj_v (1)     lambda(Y2) gathered from lx2 (and lv2 for HV), transposed solve of V1 -> lv1, elo1
+//   k_adj_g (1)     lambda(Y2) gathered from lx2 (and lv2 for HV) (CS, MCS, HV): -> ly2
+//   k_adj_v (1)     transposed solve of V1:                              ly2 -> lv1, elo1
 //   k_adj_x (1)     lambda(Y1) gathered, transposed solve of X1:        -> lx1
-//   k_adj_u         lambda(U_s) gathered from every equation that reads U_s (the stencils of the
+//   k_adj_g (0)     lambda(U_s) gathered from every equation that reads U_s (the stencils of the
 //                   right-hand sides, the Dirichlet ends of the S-lines through idx2, boundary stage A)
 // A transposed solve of row equations (a_k, b_k, c_k) solves A^T l = r, whose row k is
 // (c_{k-1}, b_k, a_{k+1}); it reuses the register-resident partitioned elimination of the forward
@@ -188,9 +189,9 @@ template <typename T> __device__ __forceinline__ bool inside(const Prob<T> &p, i
 
 // lambda(Y2) at (i, j) (interior row i, any column): gathered from the X2 equations (CS/MCS: Y2 in
 // the Delta terms; HV: rhs_intermed_4 and the explicit stage H, whose multiplier is also lx2) and,
-// for HV, from the V2 equations (in = Y2).
-template <typename T>
-__device__ __forceinline__ T lam_y2(const Prob<T> &p, const Adj<T> &A, gptr<const T> cs, int i, int j)
+// for HV, from the V2 equations (in = Y2).  S(di, dj) gives the stencil at (i + di, j + dj).
+template <typename T, typename SF>
+__device__ __forceinline__ T lam_y2(const Prob<T> &p, const Adj<T> &A, SF &&S, int i, int j)
 {
     const int ny = p.ny;
     const gptr<const T> lx2 = G((const T *)A.lx2), lv2 = G((const T *)A.lv2);
@@ -203,7 +204,7 @@ __device__ __forceinline__ T lam_y2(const Prob<T> &p, const Adj<T> &A, gptr<cons
             const int ei = i + di, ej = j + dj;
             if (!inside(p, ei, ej))
                 continue;
-            const Stencil<T> s = stencil_at(p, cs, ei, ej);
+            const Stencil<T> s = S(di, dj);
             const int64_t q = (int64_t)ei * ny + ej;
             if (hv) {
                 T w = w_h(p, s, -di, -dj, 2);
@@ -253,9 +254,10 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
         const gptr<T> L = G((last ? A.lv2 : A.lv1) + (int64_t)i * ny);
         const gptr<const uint32_t> inv2o = G(A.inv2o), inv2i = G(A.inv2i);
         // lambda of the stage output at (i, j): B^T (explicit_boundary.hpp:202-238) for the last stage
+        const gptr<const T> ly2 = G((const T *)A.ly2) + (int64_t)i * ny;
         auto lam_out = [&](int j) -> T {
             if (!last)
-                return lam_y2(p, A, cs, i, j);
+                return ly2[j];
             T v = lamn[(int64_t)i * ny + j];
             if (i == P - 1 || i == P - 2) {
                 T sB = (T)0;
@@ -294,13 +296,13 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
 // Adjoint of an S-sweep (XTJ columns x NCH chunks per workgroup).  stage 2: X2 of CS/MCS/HV
 // (with HV's explicit stage H, whose multiplier equals X2's); stage 1: X1.
 // -------------------------------------------------------------------------------------------------
-template <typename T, int M>
-__global__ void __launch_bounds__(XTJ *NCH) k_adj_x(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
-                                                    const Tile *__restrict__ tiles, int step, int stage,
-                                                    double *__restrict__ slots)
+template <typename T, int M, int W>
+__global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
+                                                  const Tile *__restrict__ tiles, int step, int stage,
+                                                  double *__restrict__ slots)
 {
-    __shared__ T sh[2][4][NCH][XTJ];
-    __shared__ double red[XTJ * NCH / 64][NPAR];
+    __shared__ T sh[2][4][NCH][W];
+    __shared__ double red[W * NCH / 64][NPAR];
     const Tile tl = tiles[blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
     const Adj<T> &A = adjs[tl.prob];
@@ -310,7 +312,7 @@ __global__ void __launch_bounds__(XTJ *NCH) k_adj_x(const Prob<T> *__restrict__ 
         acc[z] = 0.0;
     if (step >= p.steps || (stage == 2 && p.scheme == SCHEME_DR))
         return;  // uniform over the workgroup
-    const int col = threadIdx.x % XTJ, chn = threadIdx.x / XTJ;
+    const int col = threadIdx.x % W, chn = threadIdx.x / W;
     const bool colok = col < tl.count;
     const int j = tl.first + (colok ? col : 0);
     const int nx = p.nx, ny = p.ny, N = nx - 2;
@@ -399,19 +401,24 @@ __global__ void __launch_bounds__(XTJ *NCH) k_adj_x(const Prob<T> *__restrict__ 
         }
         add_terms(p, A, step, k, j, g, acc);
     };
-    block_solve<T, M, XTJ>(sh, N, col, chn, colok, eq, put);
-    block_acc_store<XTJ * NCH / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
+    block_solve<T, M, W>(sh, N, col, chn, colok, eq, put);
+    block_acc_store<W * NCH / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
 
 // -------------------------------------------------------------------------------------------------
-// lambda(U_s): one workgroup per row r, threads over the columns.  Also the parameter terms of
-// boundary stage A (its D, E, F on the v = 0 row).
+// Gathers over two-dimensional tiles of GTR x GTC points (one thread per point).  The stencils of
+// the (GTR + 2) x (GTC + 2) equation points around the tile are evaluated once into LDS.
+//   WHAT 0: lambda(U_s) from every equation that reads U_s (the stencils of the right-hand sides,
+//           the Dirichlet ends of the S-lines through idx2, boundary stage A), and the parameter
+//           terms of boundary stage A (its D, E, F on the v = 0 row)
+//   WHAT 1: lambda(Y2) (CS, MCS, HV), read by the adjoint of V1
 // -------------------------------------------------------------------------------------------------
-template <typename T>
-__global__ void __launch_bounds__(256) k_adj_u(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
-                                               const Tile *__restrict__ tiles, int step, double *__restrict__ slots)
+template <typename T, int WHAT>
+__global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
+                                                    const Tile *__restrict__ tiles, int step, double *__restrict__ slots)
 {
-    __shared__ double red[4][NPAR];
+    __shared__ T sst[7][GTR + 2][GTC + 2];
+    __shared__ double red[GTR * GTC / 64][NPAR];
     const Tile tl = tiles[blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
     const Adj<T> &A = adjs[tl.prob];
@@ -419,30 +426,67 @@ __global__ void __launch_bounds__(256) k_adj_u(const Prob<T> *__restrict__ probs
 #pragma unroll
     for (int z = 0; z < NPAR; ++z)
         acc[z] = 0.0;
-    if (step < p.steps) {
-        const int r = tl.first;
-        const int nx = p.nx, ny = p.ny, P = nx - 1;
-        const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
-        const gptr<T> lam = G((step & 1) ? A.lb : A.la);
-        const gptr<const T> cs = coeff_set(p, step);
-        const gptr<const T> bs = bnd_set(p, step);
-        const gptr<const T> lv2 = G((const T *)A.lv2), lv1 = G((const T *)A.lv1), lx2 = G((const T *)A.lx2),
-                            lx1 = G((const T *)A.lx1), elo2 = G((const T *)A.elo2), elo1 = G((const T *)A.elo1);
-        const gptr<const uint32_t> inv1o = G(A.inv1o), inv1i = G(A.inv1i), inv2o = G(A.inv2o), inv2i = G(A.inv2i);
-        const T theta = p.theta, delta = p.delta, ni = p.ni, rho = p.rho;
-        const int sch = p.scheme;
-        const bool dr = sch == SCHEME_DR, hv = sch == SCHEME_HV, mcs = sch == SCHEME_MCS;
-        // lambda of colA0[i]: every v-line whose lower end read it (through idx1)
-        auto lamA = [&](int i) -> T {
-            T v = (T)0;
-            for (uint32_t k = inv1o[i]; k < inv1o[i + 1]; ++k) {
-                const uint32_t rr = inv1i[k];
-                if (rr >= 1 && (int)rr <= P - 1)
-                    v += elo2[rr] + (dr ? (T)0 : elo1[rr]);
+    const bool run = step < p.steps && (WHAT == 0 || p.scheme != SCHEME_DR);  // uniform over the workgroup
+    const int r0 = tl.first, c0 = tl.col;
+    const int nx = p.nx, ny = p.ny, P = nx - 1;
+    const gptr<const T> cs = coeff_set(p, step);
+    if (run) {
+        for (int idx = threadIdx.x; idx < (GTR + 2) * (GTC + 2); idx += GTR * GTC) {
+            const int lr = idx / (GTC + 2), lc = idx % (GTC + 2);
+            const int ei = r0 - 1 + lr, ej = c0 - 1 + lc;
+            if (inside(p, ei, ej)) {
+                const Stencil<T> st = stencil_at(p, cs, ei, ej);
+                sst[0][lr][lc] = st.M;
+                sst[1][lr][lc] = st.Mt;
+                sst[2][lr][lc] = st.P;
+                sst[3][lr][lc] = st.Pt;
+                sst[4][lr][lc] = st.Z;
+                sst[5][lr][lc] = st.W;
+                sst[6][lr][lc] = st.C;
             }
-            return v;
-        };
-        for (int c = threadIdx.x; c < ny; c += blockDim.x) {
+        }
+    }
+    __syncthreads();
+    const int tr = threadIdx.x / GTC, tc = threadIdx.x % GTC;
+    const int r = r0 + tr, c = c0 + tc;
+    // stencil at (r + di, c + dj); the point must be an equation point (inside)
+    auto S = [&](int di, int dj) -> Stencil<T> {
+        Stencil<T> st;
+        const int lr = tr + 1 + di, lc = tc + 1 + dj;
+        st.M = sst[0][lr][lc];
+        st.Mt = sst[1][lr][lc];
+        st.P = sst[2][lr][lc];
+        st.Pt = sst[3][lr][lc];
+        st.Z = sst[4][lr][lc];
+        st.W = sst[5][lr][lc];
+        st.C = sst[6][lr][lc];
+        return st;
+    };
+    if (run && tr < tl.count && c < ny) {
+        if (WHAT == 1) {
+            if (r >= 1 && r <= P - 1)
+                G(A.ly2)[(int64_t)r * ny + c] = lam_y2(p, A, S, r, c);
+        } else {
+            const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
+            const gptr<T> lam = G((step & 1) ? A.lb : A.la);
+            const gptr<const T> bs = bnd_set(p, step);
+            const gptr<const T> lv2 = G((const T *)A.lv2), lv1 = G((const T *)A.lv1), lx2 = G((const T *)A.lx2),
+                                lx1 = G((const T *)A.lx1), elo2 = G((const T *)A.elo2), elo1 = G((const T *)A.elo1);
+            const gptr<const uint32_t> inv1o = G(A.inv1o), inv1i = G(A.inv1i), inv2o = G(A.inv2o),
+                                       inv2i = G(A.inv2i);
+            const T theta = p.theta, delta = p.delta, ni = p.ni, rho = p.rho;
+            const int sch = p.scheme;
+            const bool dr = sch == SCHEME_DR, hv = sch == SCHEME_HV, mcs = sch == SCHEME_MCS;
+            // lambda of colA0[i]: every v-line whose lower end read it (through idx1)
+            auto lamA = [&](int i) -> T {
+                T v = (T)0;
+                for (uint32_t k = inv1o[i]; k < inv1o[i + 1]; ++k) {
+                    const uint32_t rr = inv1i[k];
+                    if (rr >= 1 && (int)rr <= P - 1)
+                        v += elo2[rr] + (dr ? (T)0 : elo1[rr]);
+                }
+                return v;
+            };
             T v = (T)0;
 #pragma unroll
             for (int di = -1; di <= 1; ++di) {
@@ -451,17 +495,17 @@ __global__ void __launch_bounds__(256) k_adj_u(const Prob<T> *__restrict__ probs
                     const int ei = r + di, ej = c + dj;
                     if (!inside(p, ei, ej))
                         continue;
-                    const Stencil<T> s = stencil_at(p, cs, ei, ej);
+                    const Stencil<T> st = S(di, dj);
                     const int64_t q = (int64_t)ei * ny + ej;
                     if (di == 0 && !hv)
-                        v += w_rv(p, s, -dj) * lv2[q];
+                        v += w_rv(p, st, -dj) * lv2[q];
                     if (di == 0 && !dr)
-                        v += w_rv(p, s, -dj) * lv1[q];
+                        v += w_rv(p, st, -dj) * lv1[q];
                     if (!dr && !hv)
-                        v += w_r3(p, s, -di, -dj, mcs, true) * lx2[q];
+                        v += w_r3(p, st, -di, -dj, mcs, true) * lx2[q];
                     if (hv)
-                        v += w_h(p, s, -di, -dj, 1) * lx2[q];
-                    v += w_r1(p, s, -di, -dj) * lx1[q];
+                        v += w_h(p, st, -di, -dj, 1) * lx2[q];
+                    v += w_r1(p, st, -di, -dj) * lx1[q];
                 }
             }
             // Dirichlet ends of the S-lines j with idx2[j] == c (implicit_kernel.hpp:86-101)
@@ -471,8 +515,8 @@ __global__ void __launch_bounds__(256) k_adj_u(const Prob<T> *__restrict__ probs
                     const int j = (int)inv2i[k];
                     if (j < 1 || j > ny - 2)
                         continue;
-                    const Stencil<T> s = stencil_at(p, cs, ei, j);
-                    const T w = (r == 0) ? theta * s.M : theta * s.P;  // -(folded coefficient)
+                    const Stencil<T> st = (j >= c - 1 && j <= c + 1) ? S(ei - r, j - c) : stencil_at(p, cs, ei, j);
+                    const T w = (r == 0) ? theta * st.M : theta * st.P;  // -(folded coefficient)
                     const int64_t q = (int64_t)ei * ny + j;
                     T l = lx1[q];
                     if (!dr)
@@ -516,7 +560,8 @@ __global__ void __launch_bounds__(256) k_adj_u(const Prob<T> *__restrict__ probs
             lam[(int64_t)r * ny + c] = v;
         }
     }
-    block_acc_store<4>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
+    if (WHAT == 0)
+        block_acc_store<GTR * GTC / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
 
 // -------------------------------------------------------------------------------------------------
@@ -531,9 +576,10 @@ template <typename T> __global__ void __launch_bounds__(256) k_copy_jobs(const C
         dst[k] = src[k];
 }
 
-#define LSS_INST_A(T, M)                                                                                       \
-    template __global__ void k_adj_v<T, M>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *); \
-    template __global__ void k_adj_x<T, M>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
+#define LSS_INST_A(T, M)                                                                                          \
+    template __global__ void k_adj_v<T, M>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);    \
+    template __global__ void k_adj_x<T, M, 16>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *); \
+    template __global__ void k_adj_x<T, M, 8>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
 LSS_INST_A(double, 1)
 LSS_INST_A(double, 2)
 LSS_INST_A(double, 4)
@@ -545,8 +591,10 @@ LSS_INST_A(float, 4)
 LSS_INST_A(float, 8)
 LSS_INST_A(float, 16)
 #undef LSS_INST_A
-template __global__ void k_adj_u<double>(const Prob<double> *, const Adj<double> *, const Tile *, int, double *);
-template __global__ void k_adj_u<float>(const Prob<float> *, const Adj<float> *, const Tile *, int, double *);
+template __global__ void k_adj_g<double, 0>(const Prob<double> *, const Adj<double> *, const Tile *, int, double *);
+template __global__ void k_adj_g<double, 1>(const Prob<double> *, const Adj<double> *, const Tile *, int, double *);
+template __global__ void k_adj_g<float, 0>(const Prob<float> *, const Adj<float> *, const Tile *, int, double *);
+template __global__ void k_adj_g<float, 1>(const Prob<float> *, const Adj<float> *, const Tile *, int, double *);
 template __global__ void k_copy_jobs<double>(const CopyJob *);
 template __global__ void k_copy_jobs<float>(const CopyJob *);
 

@@ -37,9 +37,9 @@ template <typename T, int M, int STAGE, int W> __global__ void k_xsweep_p(const 
 template <typename T, int M, int VR> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int);
 template <typename T, int M>
 __global__ void k_adj_v(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
-template <typename T, int M>
+template <typename T, int M, int W>
 __global__ void k_adj_x(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
-template <typename T> __global__ void k_adj_u(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
+template <typename T, int WHAT> __global__ void k_adj_g(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
 template <typename T> __global__ void k_copy_jobs(const CopyJob *);
 template <typename T, int M>
 __global__ void k_tridiag_rows(int64_t, int, const T *, const T *, const T *, const T *, T *);
@@ -404,13 +404,19 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 case TB_V16: lo = 1, hi = p.nx - 1, step = 16; break;
                 default: lo = 1, hi = p.nx - 1, step = 1; break;
                 }
-                for (int s = lo; s < hi; s += step) {
-                    Tile t;
-                    t.prob = (int32_t)q;
-                    t.first = s;
-                    t.count = std::min(step, hi - s);
-                    t.pad = 0;
-                    tiles[tb].push_back(t);
+                if (tb == TB_U) {  // two-dimensional gather tiles of the adjoint
+                    for (int r = 0; r < p.nx; r += GTR)
+                        for (int c = 0; c < p.ny; c += GTC)
+                            tiles[tb].push_back({(int32_t)q, r, std::min(GTR, p.nx - r), c});
+                } else {
+                    for (int s = lo; s < hi; s += step) {
+                        Tile t;
+                        t.prob = (int32_t)q;
+                        t.first = s;
+                        t.count = std::min(step, hi - s);
+                        t.col = 0;
+                        tiles[tb].push_back(t);
+                    }
                 }
                 prefix[tb][q + 1] = (int)tiles[tb].size();
             }
@@ -660,7 +666,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     int64_t max_nxy = 0;
     size_t slot_off[5] = {};                      // per adjoint kernel kind, in units of NPAR doubles
     size_t nslots = 0;
-    const int slot_tb[5] = {TB_AV, TB_AX, TB_AV, TB_AX, TB_U};
+    int slot_tb[5] = {TB_AV, TB_AX, TB_AV, TB_AX, TB_U};
+    int axw = 16;                                 // S-lines per adjoint x-sweep workgroup
 
     void invalidate_adjoint() override
     {
@@ -686,7 +693,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             const size_t nxy = (size_t)p.nx * p.ny, nx_ = (size_t)p.nx, ny_ = (size_t)p.ny;
             const bool dr = p.scheme == SCHEME_DR, hv = p.scheme == SCHEME_HV;
             const size_t nys = (dr ? 1 : 3) + ((hv && line_solver == LINE_EXACT) ? 1 : 0);
-            const size_t nl = dr ? 4 : 6;
+            const size_t nl = dr ? 4 : 7;
             const size_t nck = ((size_t)p.steps + K - 1) / K;
             const Terms *tm = terms.empty() ? nullptr : &terms[order[q]];
             const size_t nfac = tm ? tm->fac.size() / sizeof(T) : 0;
@@ -729,6 +736,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             A.lx1 = take(nxy);
             A.lx2 = dr ? nullptr : take(nxy);
             A.lv1 = dr ? nullptr : take(nxy);
+            A.ly2 = dr ? nullptr : take(nxy);
             A.elo2 = take(nx_);
             A.elo1 = take(nx_);
             ckpt_base[q] = take(nck * nxy);
@@ -838,7 +846,9 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         lam_jobs_dev.~dev_buffer();
         new (&lam_jobs_dev) dev_buffer();
         lam_jobs_dev.alloc(2 * n * sizeof(CopyJob));
-        // parameter slots
+        // parameter slots; S-lines of 16 chunks per point need the wider register budget of 8-line groups
+        axw = pmx >= 16 ? 8 : 16;
+        slot_tb[1] = slot_tb[3] = axw == 8 ? TB_X8 : TB_AX;
         nslots = 0;
         for (int k = 0; k < 5; ++k) {
             slot_off[k] = nslots;
@@ -868,8 +878,12 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     }
     template <int M> void adj_x(int s, int stage, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
     {
-        hipLaunchKernelGGL((k_adj_x<T, M>), dim3(grid), dim3(P_XTJ * P_NCH), 0, st, table, (const Adj<T> *)adj_dev.p,
-                           (const Tile *)tiles_dev.p + tile_off[TB_AX], s, stage, slot);
+        if (axw == 8)
+            hipLaunchKernelGGL((k_adj_x<T, M, 8>), dim3(grid), dim3(8 * P_NCH), 0, st, table,
+                               (const Adj<T> *)adj_dev.p, (const Tile *)tiles_dev.p + tile_off[TB_X8], s, stage, slot);
+        else
+            hipLaunchKernelGGL((k_adj_x<T, M, 16>), dim3(grid), dim3(16 * P_NCH), 0, st, table,
+                               (const Adj<T> *)adj_dev.p, (const Tile *)tiles_dev.p + tile_off[TB_AX], s, stage, slot);
     }
     void adj_v_m(int s, int last, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
     {
@@ -901,13 +915,16 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             any_split2 |= probs[q].scheme != SCHEME_DR;
         double *sl = (double *)slots_dev.p;
         adj_v_m(s, 1, table, sl + slot_off[0] * NPAR, st, grid(TB_AV));
+        const Adj<T> *ad = (const Adj<T> *)adj_dev.p;
+        const Tile *tu = (const Tile *)tiles_dev.p + tile_off[TB_U];
         if (any_split2) {
-            adj_x_m(s, 2, table, sl + slot_off[1] * NPAR, st, grid(TB_AX));
+            adj_x_m(s, 2, table, sl + slot_off[1] * NPAR, st, grid(slot_tb[1]));
+            hipLaunchKernelGGL((k_adj_g<T, 1>), dim3(grid(TB_U)), dim3(GTR * GTC), 0, st, table, ad, tu, s, sl);
             adj_v_m(s, 0, table, sl + slot_off[2] * NPAR, st, grid(TB_AV));
         }
-        adj_x_m(s, 1, table, sl + slot_off[3] * NPAR, st, grid(TB_AX));
-        hipLaunchKernelGGL(k_adj_u<T>, dim3(grid(TB_U)), dim3(256), 0, st, table, (const Adj<T> *)adj_dev.p,
-                           (const Tile *)tiles_dev.p + tile_off[TB_U], s, sl + slot_off[4] * NPAR);
+        adj_x_m(s, 1, table, sl + slot_off[3] * NPAR, st, grid(slot_tb[3]));
+        hipLaunchKernelGGL((k_adj_g<T, 0>), dim3(grid(TB_U)), dim3(GTR * GTC), 0, st, table, ad, tu, s,
+                           sl + slot_off[4] * NPAR);
     }
 
     void forward_ckpt(const void *const *u_in, void *const *u_out, int K, hipStream_t stream) override

@@ -57,7 +57,7 @@ struct Tile {
     int32_t prob;   // problem index
     int32_t first;  // first line / row / column of the tile
     int32_t count;  // number of lines in the tile (<= TILE)
-    int32_t pad;
+    int32_t col;    // first column of a two-dimensional tile (0 otherwise)
 };
 
 
@@ -65,6 +65,7 @@ struct Tile {
 // Reverse mode (adjoint) of the time loop
 // -------------------------------------------------------------------------------------------------
 constexpr int NPAR = 6;    // model parameters per problem
+constexpr int GTR = 8, GTC = 32;  // gather tiles of the adjoint: rows x columns (one thread per point)
 constexpr int NTERM = 12;  // derivative terms per problem
 
 // Adjoint state of one problem.  The forward values a reverse step needs (U_s, U_{s+1} and the
@@ -80,6 +81,7 @@ constexpr int NTERM = 12;  // derivative terms per problem
 template <typename T> struct Adj {
     T *la, *lb;
     T *lv2, *lx2, *lv1, *lx1;
+    T *ly2;                     // lambda(Y2) gathered from the X2 (and HV) equations (CS, MCS, HV)
     T *elo2, *elo1;
     const uint32_t *inv1o, *inv1i, *inv2o, *inv2i;
     const T *dfac;
