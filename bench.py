@@ -85,7 +85,7 @@ def time_stream(torch, fn, reps: int) -> float:
     return e0.elapsed_time(e1) / 1e3
 
 
-def bench_book(torch, models, steps: int, warmup: int, log):
+def bench_book(torch, models, steps: int, warmup: int, log, graph: bool = True):
     t0 = time.time()
     plan = L.Plan.from_models(models, L.FP64)
     log(f"book plan: {len(models)} problems built in {time.time() - t0:.1f}s")
@@ -95,10 +95,11 @@ def bench_book(torch, models, steps: int, warmup: int, log):
     stream = torch.cuda.current_stream().cuda_stream
     t0 = time.time()
     for _ in range(max(warmup, 1)):
-        plan.run_dev(pin, pout, stream, graph=True)
+        plan.run_dev(pin, pout, stream, graph=graph)
     torch.cuda.synchronize()
-    log(f"book warmup ({max(warmup, 1)} solves incl. graph capture) {time.time() - t0:.1f}s")
-    secs = time_stream(torch, lambda: plan.run_dev(pin, pout, stream, graph=True), steps)
+    log(f"book warmup ({max(warmup, 1)} solves incl. autotuning and graph capture) {time.time() - t0:.1f}s; "
+        f"mapping (S-lines, v-lines per workgroup) = {plan.autotune(stream)}")
+    secs = time_stream(torch, lambda: plan.run_dev(pin, pout, stream, graph=graph), steps)
     log(f"book timed: {steps} solves in {secs:.3f}s")
     prof = L.run_profiled(plan, pin, pout, stream)
     return plan, ins, outs, secs, prof
@@ -177,6 +178,35 @@ def parity_sample(models, outs, log, count=8):
     log(f"parity sample: {len(picked)} book problems vs oracle, max per-value rel err {worst:.2e}, "
         f"max normwise {worst_norm:.2e}")
     return worst, worst_norm, len(picked)
+
+
+def check_all(models, outs, log, threads: int = 16):
+    """Every book problem against the oracle (cached coefficients; same arithmetic as the faithful
+    mode), on a pool of host threads (ctypes releases the GIL during each solve)."""
+    import concurrent.futures as cf
+    from oracle import oracle as O
+    from tests.helpers import oracle_solve
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    t0 = time.time()
+
+    def one(q):
+        c = oracle_solve(O, models[q])
+        g = outs[q]
+        return q, float(np.abs(g - c).max() / np.abs(c).max()), rel_err(g, c)
+
+    worst_n, worst_v, over, done = 0.0, 0.0, [], 0
+    order = sorted(range(len(models)), key=lambda q: -models[q].nx * models[q].ny * models[q].steps)
+    with cf.ThreadPoolExecutor(threads) as ex:
+        for q, en, ev in ex.map(one, order):
+            worst_n, worst_v, done = max(worst_n, en), max(worst_v, ev), done + 1
+            if en > 1e-10:
+                over.append(q)
+            if done % 256 == 0:
+                log(f"check-all: {done}/{len(models)} problems, max normwise {worst_n:.2e} ({time.time() - t0:.0f}s)")
+    log(f"check-all: {done} problems in {time.time() - t0:.0f}s on {threads} threads, max normwise {worst_n:.2e}, "
+        f"max per-value {worst_v:.2e}, over 1e-10 normwise: {len(over)}")
+    return {"problems": done, "threads": threads, "max_rel_err_normwise": worst_n, "max_rel_err_per_value": worst_v,
+            "over_tolerance": over[:32], "tolerance_normwise": 1e-10, "seconds": round(time.time() - t0, 1)}
 
 
 def cpu_baseline(models, log, budget_s: float = 15.0):
@@ -376,6 +406,9 @@ def main():
     ap.add_argument("--count", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch the book's steps without hipGraph replay")
+    ap.add_argument("--check-all", action="store_true",
+                    help="check every book problem against the oracle on host threads (about 12 min on 16 cores)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device to device three times (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -400,7 +433,7 @@ def main():
     models = book_models(args.count)
     total_ps = point_steps(models)
     log(f"book: {len(models)} problems, {total_ps:.3e} grid-point-steps per solve")
-    plan, ins, outs, secs, prof = bench_book(torch, models, args.steps, args.warmup, log)
+    plan, ins, outs, secs, prof = bench_book(torch, models, args.steps, args.warmup, log, graph=not args.no_graph)
     value = total_ps * args.steps / secs
     ms_per_step = secs / args.steps * 1e3
     sweeps = {k: v for k, v in prof.items() if v[2] > 0}
@@ -425,7 +458,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "heston_book_2048_seed3", "problems": len(models),
-                   "grid_point_steps_per_solve": total_ps, "coeff": "separable", "line_solver": "partitioned", "graph": True},
+                   "grid_point_steps_per_solve": total_ps, "coeff": "separable", "line_solver": "partitioned",
+                   "graph": not args.no_graph, "mapping_s_v_lines_per_group": list(plan.autotune(0))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": ("k_" + dom.rstrip("12") + ("_p" if dom.startswith(("x", "v")) else "")),
@@ -433,7 +467,8 @@ def main():
                      "algo_bytes_per_launch": round(d_bytes / d_n),
                      "whole_step_algo_GBs": round(total_algo / (total_ms / 1e3) / 1e9, 1)},
     }
-    plan_sep = None
+    if args.check_all:
+        result["check_all"] = check_all(models, [o.cpu().numpy() for o in outs], log)
     if not args.no_extras:
         from tests.helpers import rel_err as _re  # noqa: F401
         outs_np = [o.cpu().numpy() for o in outs]

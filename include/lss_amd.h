@@ -106,15 +106,17 @@ int lss_adi_plan_destroy(lss_adi_plan *plan);
 int lss_adi_run_dev(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, void *stream, int32_t use_graph);
 /*
  * Autotuner (no counterpart in the reference): times the tile/lane mappings of the partitioned sweeps
- * (S-lines per x-sweep workgroup: 16 or 8; v-lines per v-sweep workgroup: 4 or 16) on the first steps
- * of the loop and keeps the fastest; the mappings give identical results.  Runs by itself before a
- * plan's first solve; `again` != 0 re-times (and drops a captured graph).  Overwrites the plan's state
- * buffers, synchronous with respect to `stream`.  Returns the chosen mapping.
+ * (S-lines per x-sweep workgroup: 16 or 8; v-lines per v-sweep workgroup: 4 or 16; tiles in launch
+ * order or XCD-contiguous order) on the first steps of the loop and keeps the fastest; the mappings
+ * give identical results.  Runs by itself before a plan's first solve; `again` != 0 re-times (and
+ * drops a captured graph).  Overwrites the plan's state buffers, synchronous with respect to
+ * `stream`.  Returns the chosen mapping.
  */
 int lss_adi_autotune(lss_adi_plan *plan, void *stream, int32_t again, int32_t *s_lines_per_group,
-                     int32_t *v_lines_per_group);
-/* Sets the mapping instead of timing it (16 or 8 S-lines, 4 or 16 v-lines per workgroup). */
-int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v_lines_per_group);
+                     int32_t *v_lines_per_group, int32_t *xcd_order);
+/* Sets the mapping instead of timing it (16 or 8 S-lines, 4 or 16 v-lines per workgroup, 0 or 1). */
+int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v_lines_per_group,
+                        int32_t xcd_order);
 int lss_adi_run_host(lss_adi_plan *plan, const void *const *u_in, void *const *u_out);
 
 /*

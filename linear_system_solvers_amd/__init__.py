@@ -158,9 +158,9 @@ def lib():
         L.lss_adi_set_param_terms.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                               C.POINTER(C.c_int32), C.c_int64, C.c_void_p]
         L.lss_plan_param_count.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]
-        L.lss_adi_set_mapping.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.lss_adi_set_mapping.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
         L.lss_adi_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
-                                       C.POINTER(C.c_int32)]
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.lss_adi_forward.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32,
                                       C.c_void_p]
         L.lss_adi_backward.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
@@ -243,13 +243,14 @@ class Plan:
 
 
     def autotune(self, stream: int = 0, again: bool = False):
-        """Tile/lane mapping chosen by the plan's autotuner: (S-lines per x-sweep group, v-lines per v-sweep group)."""
-        x, v = C.c_int32(), C.c_int32()
-        _check(lib().lss_adi_autotune(self._h, C.c_void_p(stream), int(again), C.byref(x), C.byref(v)))
-        return x.value, v.value
+        """Tile/lane mapping chosen by the plan's autotuner:
+        (S-lines per x-sweep group, v-lines per v-sweep group, XCD-contiguous tile order)."""
+        x, v, z = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().lss_adi_autotune(self._h, C.c_void_p(stream), int(again), C.byref(x), C.byref(v), C.byref(z)))
+        return x.value, v.value, z.value
 
-    def set_mapping(self, s_lines_per_group: int, v_lines_per_group: int) -> None:
-        _check(lib().lss_adi_set_mapping(self._h, s_lines_per_group, v_lines_per_group))
+    def set_mapping(self, s_lines_per_group: int, v_lines_per_group: int, xcd_order: int = 0) -> None:
+        _check(lib().lss_adi_set_mapping(self._h, s_lines_per_group, v_lines_per_group, xcd_order))
 
     # ---- reverse mode -------------------------------------------------------------------------
     def param_count(self, p: int) -> int:

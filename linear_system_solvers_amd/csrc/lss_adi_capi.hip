@@ -33,8 +33,8 @@ template <typename T> __global__ void k_vsweep(const Prob<T> *, const Tile *, in
 template <typename T> __global__ void k_hv_y3(const Prob<T> *, const Tile *, int);
 template <typename T>
 __global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *);
-template <typename T, int M, int STAGE, int W> __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int);
-template <typename T, int M, int VR> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int);
+template <typename T, int M, int STAGE, int W> __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int, int);
+template <typename T, int M, int VR> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int, int);
 template <typename T, int M>
 __global__ void k_adj_v(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
 template <typename T, int M, int W>
@@ -141,8 +141,8 @@ struct lss_adi_plan {
     virtual void forward_ckpt(const void *const *u_in, void *const *u_out, int ckpt, hipStream_t stream) = 0;
     virtual void backward(const void *const *lam_out, void *const *lam_in, double *dparams, hipStream_t stream) = 0;
     virtual void invalidate_adjoint() = 0;
-    virtual void tune(hipStream_t stream, bool again, int *xw, int *vr) = 0;
-    virtual void set_mapping(int xw, int vr) = 0;
+    virtual void tune(hipStream_t stream, bool again, int *xw, int *vr, int *xcd) = 0;
+    virtual void set_mapping(int xw, int vr, int xcd) = 0;
     virtual void run(const void *const *u_in, void *const *u_out, hipStream_t stream, bool use_graph, bool host) = 0;
     virtual void profile(const void *const *u_in, void *const *u_out, hipStream_t stream, double *ms, double *bytes,
                          int64_t *launches) = 0;
@@ -191,6 +191,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     int line_solver = LINE_EXACT;
     int pmx = 1, pmy = 1;              // points per chunk bound (template M) of S-lines and v-lines
     int xw = 16, vr = 4;               // S-lines per x-sweep workgroup, v-lines per v-sweep workgroup
+    int swz = 0;                       // XCD-aware tile order (xcd_tile) in the sweeps
     bool tuned = false;                // the autotuner has chosen xw and vr
     bool adj_ok = false;               // every line fits the partitioned solver (needed by the adjoint)
     hipStream_t cap_stream = nullptr;
@@ -477,9 +478,9 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         const dim3 g((unsigned)prefix[tb][na]), b(W * P_NCH);
         const Tile *tx = (const Tile *)tiles_dev.p + tile_off[tb];
         if (stage == 1)
-            hipLaunchKernelGGL((k_xsweep_p<T, M, 1, W>), g, b, 0, st, pd, tx, s);
+            hipLaunchKernelGGL((k_xsweep_p<T, M, 1, W>), g, b, 0, st, pd, tx, s, swz);
         else
-            hipLaunchKernelGGL((k_xsweep_p<T, M, 2, W>), g, b, 0, st, pd, tx, s);
+            hipLaunchKernelGGL((k_xsweep_p<T, M, 2, W>), g, b, 0, st, pd, tx, s, swz);
     }
     template <int W> void launch_x_w(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
@@ -503,7 +504,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         const int tb = VR == 4 ? TB_V : TB_V16;
         const dim3 g((unsigned)prefix[tb][na]), b(64 * VR);
         hipLaunchKernelGGL((k_vsweep_p<T, M, VR>), g, b, 0, st, pd, (const Tile *)tiles_dev.p + tile_off[tb], s,
-                           stage);
+                           stage, swz);
     }
     template <int VR> void launch_v_r(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
@@ -533,7 +534,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         }
     }
 
-    void tune(hipStream_t stream, bool again, int *x, int *v) override
+    void tune(hipStream_t stream, bool again, int *x, int *v, int *z) override
     {
         if (!tuned || again) {
             drop_graph();
@@ -541,14 +542,17 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         }
         *x = xw;
         *v = vr;
+        *z = swz;
     }
 
-    void set_mapping(int x, int v) override
+    void set_mapping(int x, int v, int z) override
     {
-        require((x == 16 || x == 8) && (v == 4 || v == 16), "mapping must be (16 or 8, 4 or 16)");
+        require((x == 16 || x == 8) && (v == 4 || v == 16) && (z == 0 || z == 1),
+                "mapping must be (16 or 8, 4 or 16, 0 or 1)");
         drop_graph();
         xw = x;
         vr = v;
+        swz = z;
         tuned = true;
     }
 
@@ -564,11 +568,13 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         float best = 1e30f;
-        int bx = 16, bv = 4;
-        for (int cx : {16, 8})
-            for (int cv : {4, 16}) {
+        int bx = 16, bv = 4, bs = 0;
+        for (int cz : {0, 1})
+            for (int cx : {16, 8})
+                for (int cv : {4, 16}) {
                 xw = cx;
                 vr = cv;
+                swz = cz;
                 for (int rep = 0; rep < 2; ++rep) {  // the first pass warms caches and code
                     HIPCHK(hipEventRecord(e0, st));
                     for (int s = 0; s < nst; ++s)
@@ -582,12 +588,14 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                     best = ms;
                     bx = cx;
                     bv = cv;
+                    bs = cz;
                 }
             }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         xw = bx;
         vr = bv;
+        swz = bs;
     }
 
     // one loop step of every active problem; `table` replaces the plan's problem table (the adjoint's
@@ -1557,22 +1565,24 @@ int lss_adi_set_param_terms(lss_adi_plan *plan, int64_t p, int32_t nparam, int32
 }
 
 int lss_adi_autotune(lss_adi_plan *plan, void *stream, int32_t again, int32_t *s_lines_per_group,
-                     int32_t *v_lines_per_group)
+                     int32_t *v_lines_per_group, int32_t *xcd_order)
 {
     return guarded([&] {
-        require(plan && s_lines_per_group && v_lines_per_group, "null pointer");
-        int x = 0, v = 0;
-        plan->tune((hipStream_t)stream, again != 0, &x, &v);
+        require(plan && s_lines_per_group && v_lines_per_group && xcd_order, "null pointer");
+        int x = 0, v = 0, z = 0;
+        plan->tune((hipStream_t)stream, again != 0, &x, &v, &z);
         *s_lines_per_group = x;
         *v_lines_per_group = v;
+        *xcd_order = z;
     });
 }
 
-int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v_lines_per_group)
+int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v_lines_per_group,
+                        int32_t xcd_order)
 {
     return guarded([&] {
         require(plan != nullptr, "null plan");
-        plan->set_mapping(s_lines_per_group, v_lines_per_group);
+        plan->set_mapping(s_lines_per_group, v_lines_per_group, xcd_order);
     });
 }
 

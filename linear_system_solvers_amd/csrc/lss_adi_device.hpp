@@ -53,6 +53,16 @@ template <typename T> struct Prob {
     int64_t coeff_stride, bnd_stride;
 };
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8), and every XCD has
+// its own L2.  xcd_tile maps block b so that each XCD takes a contiguous run of the g tiles:
+// neighbouring tiles, which share cache lines and stencil halos, then meet in the same L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_tile(int b, int g)
+{
+    const int q = g / NXCD, r = g % NXCD, x = b % NXCD, k = b / NXCD;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 struct Tile {
     int32_t prob;   // problem index
     int32_t first;  // first line / row / column of the tile

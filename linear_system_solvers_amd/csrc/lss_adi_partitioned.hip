@@ -29,9 +29,9 @@ namespace lssamd {
 // -------------------------------------------------------------------------------------------------
 template <typename T, int M, int VR>
 __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict__ probs,
-                                                      const Tile *__restrict__ tiles, int step, int stage)
+                                                      const Tile *__restrict__ tiles, int step, int stage, int swz)
 {
-    const Tile tl = tiles[blockIdx.x];
+    const Tile tl = tiles[swz ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (step >= p.steps || (stage == 2 && p.scheme == SCHEME_DR) || wave >= tl.count)
@@ -95,10 +95,10 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
 // -------------------------------------------------------------------------------------------------
 template <typename T, int M, int STAGE, int W>
 __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__ probs,
-                                                     const Tile *__restrict__ tiles, int step)
+                                                     const Tile *__restrict__ tiles, int step, int swz)
 {
     __shared__ T sh[2][4][NCH][W];
-    const Tile tl = tiles[blockIdx.x];
+    const Tile tl = tiles[swz ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
     if (step >= p.steps || (STAGE == 2 && p.scheme == SCHEME_DR))
         return;  // uniform over the workgroup
@@ -150,13 +150,13 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
     }
 }
 
-#define LSS_INST_P(T, M)                                                                   \
-    template __global__ void k_vsweep_p<T, M, 4>(const Prob<T> *, const Tile *, int, int);  \
-    template __global__ void k_vsweep_p<T, M, 16>(const Prob<T> *, const Tile *, int, int); \
-    template __global__ void k_xsweep_p<T, M, 1, 16>(const Prob<T> *, const Tile *, int);   \
-    template __global__ void k_xsweep_p<T, M, 2, 16>(const Prob<T> *, const Tile *, int);   \
-    template __global__ void k_xsweep_p<T, M, 1, 8>(const Prob<T> *, const Tile *, int);    \
-    template __global__ void k_xsweep_p<T, M, 2, 8>(const Prob<T> *, const Tile *, int);
+#define LSS_INST_P(T, M)                                                                        \
+    template __global__ void k_vsweep_p<T, M, 4>(const Prob<T> *, const Tile *, int, int, int);  \
+    template __global__ void k_vsweep_p<T, M, 16>(const Prob<T> *, const Tile *, int, int, int); \
+    template __global__ void k_xsweep_p<T, M, 1, 16>(const Prob<T> *, const Tile *, int, int);   \
+    template __global__ void k_xsweep_p<T, M, 2, 16>(const Prob<T> *, const Tile *, int, int);   \
+    template __global__ void k_xsweep_p<T, M, 1, 8>(const Prob<T> *, const Tile *, int, int);    \
+    template __global__ void k_xsweep_p<T, M, 2, 8>(const Prob<T> *, const Tile *, int, int);
 LSS_INST_P(double, 1)
 LSS_INST_P(double, 2)
 LSS_INST_P(double, 4)

@@ -221,15 +221,16 @@ def test_every_tile_mapping_gives_identical_results():
               for q in range(12)]
     plan = L.Plan.from_models(models)
     chosen = plan.autotune()
-    assert chosen[0] in (16, 8) and chosen[1] in (4, 16)
+    assert chosen[0] in (16, 8) and chosen[1] in (4, 16) and chosen[2] in (0, 1)
     ref = None
-    for x in (16, 8):
-        for v in (4, 16):
-            plan.set_mapping(x, v)
-            outs = plan.run_host()
-            if ref is None:
-                ref = outs
-            for a, b in zip(ref, outs):
-                np.testing.assert_array_equal(a, b)
+    for z in (0, 1):
+        for x in (16, 8):
+            for v in (4, 16):
+                plan.set_mapping(x, v, z)
+                outs = plan.run_host()
+                if ref is None:
+                    ref = outs
+                for a, b in zip(ref, outs):
+                    np.testing.assert_array_equal(a, b)
     with pytest.raises(L.LssError):
         plan.set_mapping(32, 4)
