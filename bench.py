@@ -37,6 +37,19 @@ sys.path.insert(0, ROOT)
 import linear_system_solvers_amd as L  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernels, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this
+# workload (scaled by the same counters read for a copy kernel of known size); see profiles/
+TRAFFIC_JSON = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+
+
+def measured_traffic(kind: str):
+    try:
+        with open(TRAFFIC_JSON) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = t.get("kernels", {}).get(kind)
+    return None if e is None else e.get("bytes_per_launch")
 
 
 def book_models(count: int = 2048, seed: int = 3, coeff_mode: int = L.COEFF_SEPARABLE,
@@ -410,6 +423,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch the book's steps without hipGraph replay")
+    ap.add_argument("--counters", default="",
+                    help="counter-collection run: S,V,X mapping (e.g. 8,4,1) set instead of autotuned; the book is "
+                         "solved --steps times with no events and no timing (hipEvents are not used under --pmc)")
     ap.add_argument("--check-all", action="store_true",
                     help="check every book problem against the oracle on host threads (about 12 min on 16 cores)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -435,6 +451,19 @@ def main():
         del x, y
     models = book_models(args.count)
     total_ps = point_steps(models)
+    if args.counters:
+        x, v, z = (int(t) for t in args.counters.split(","))
+        plan = L.Plan.from_models(models, L.FP64)
+        plan.set_mapping(x, v, z)
+        ins = [torch.from_numpy(plan.initial(p)).cuda() for p in range(plan.count)]
+        outs = [torch.empty_like(u) for u in ins]
+        for _ in range(args.steps):
+            plan.run_dev([u.data_ptr() for u in ins], [u.data_ptr() for u in outs], 0, graph=False)
+        torch.cuda.synchronize()
+        log(f"counter run: {args.steps} book solves with mapping {(x, v, z)}")
+        print(json.dumps({"counters_run": True, "solves": args.steps, "mapping": [x, v, z],
+                          "grid_point_steps_per_solve": total_ps}), flush=True)
+        return
     log(f"book: {len(models)} problems, {total_ps:.3e} grid-point-steps per solve")
     plan, ins, outs, secs, prof = bench_book(torch, models, args.steps, args.warmup, log, graph=not args.no_graph)
     value = total_ps * args.steps / secs
@@ -464,7 +493,7 @@ def main():
                    "grid_point_steps_per_solve": total_ps, "coeff": "separable", "line_solver": "partitioned",
                    "graph": not args.no_graph, "mapping_s_v_lines_per_group": list(plan.autotune(0))},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                      "kernel": ("k_" + dom.rstrip("12") + ("_p" if dom.startswith(("x", "v")) else "")),
                      "kernel_kind": dom, "launches": int(d_n), "avg_launch_us": round(d_ms / d_n * 1e3, 2),
                      "algo_bytes_per_launch": round(d_bytes / d_n),

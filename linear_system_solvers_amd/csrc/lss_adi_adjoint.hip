@@ -48,12 +48,14 @@ __device__ __forceinline__ void add_terms_q(const Prob<T> &p, const Adj<T> &A, i
                                             T gc, T gd, T ge, T gf, double *acc)
 {
     const int nx = p.nx, ny = p.ny;
-    const gptr<const T> f = G(A.dfac + (A.dfac_sets > 1 ? (int64_t)step : 0) * A.dfac_stride);
+    const gptr<const T> fX = G(A.dfac + (A.dfac_sets > 1 ? (int64_t)step : 0) * A.dfac_stride);
+    const gptr<const T> fY = fX + (int64_t)A.nfx * nx;
     for (int t = 0; t < A.nterm; ++t) {
         const int q = A.tq[t];
         const T gq = q == 0 ? ga : (q == 1 ? gb : (q == 2 ? gc : (q == 3 ? gd : (q == 4 ? ge : gf))));
-        const gptr<const T> ft = f + (int64_t)t * (nx + ny);
-        const double v = (double)gq * (double)ft[i] * (double)ft[nx + j];
+        const T xv = A.fx[t] >= 0 ? fX[(int64_t)A.fx[t] * nx + i] : (T)1;
+        const T yv = A.fy[t] >= 0 ? fY[(int64_t)A.fy[t] * ny + j] : (T)1;
+        const double v = (double)gq * (double)xv * (double)yv;
         const int k = A.tp[t];
 #pragma unroll
         for (int z = 0; z < NPAR; ++z)
@@ -275,19 +277,26 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
             e.r = lam_out(k);
             return e;
         };
-        auto put = [&](int k, T x) {
-            L[k] = x;
+        // the solution stays in registers until every load of the parameter terms is issued
+        T xs[M];
+        auto put = [&](int t, int k, T x) {
+            xs[t] = x;
             SG<T> g = sg_zero<T>();
             g.Mt = theta * x * (yrow[k - 1] - irow[k - 1]);
             g.W = -theta * x * (yrow[k] - irow[k]);
             g.Pt = theta * x * (yrow[k + 1] - irow[k + 1]);
             add_terms(p, A, step, i, k, g, acc);
-            if (k == 1) {
-                const T a1 = -theta * stencil_at(p, cs, i, 1).Mt;
-                G(last ? A.elo2 : A.elo1)[i] = lam_out(0) - a1 * x;
-            }
         };
         wave_solve<T, M>(N, lane, eq, put);
+        const int m = (N + NCH - 1) / NCH, s0 = 1 + lane * m, cnt = max(0, min(m, N - s0 + 1));
+#pragma unroll
+        for (int t = 0; t < M; ++t)
+            if (t < cnt)
+                L[s0 + t] = xs[t];
+        if (lane == 0) {
+            const T a1 = -theta * stencil_at(p, cs, i, 1).Mt;
+            G(last ? A.elo2 : A.elo1)[i] = lam_out(0) - a1 * xs[0];
+        }
     }
     block_acc_store<VROWS>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
@@ -351,9 +360,11 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
         e.r = lam_out(k);
         return e;
     };
-    auto put = [&](int k, T x) {
+    // the solution stays in registers until every load of the parameter terms is issued
+    T xs[M];
+    auto put = [&](int t, int k, T x) {
         const int64_t q = (int64_t)k * ny + j;
-        L[q] = x;
+        xs[t] = x;
         if (A.nterm == 0)
             return;
         SG<T> g = sg_zero<T>();
@@ -402,6 +413,13 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
         add_terms(p, A, step, k, j, g, acc);
     };
     block_solve<T, M, W>(sh, N, col, chn, colok, eq, put);
+    if (colok) {
+        const int m = (N + NCH - 1) / NCH, s0 = 1 + chn * m, cnt = max(0, min(m, N - s0 + 1));
+#pragma unroll
+        for (int t = 0; t < M; ++t)
+            if (t < cnt)
+                L[(int64_t)(s0 + t) * ny + j] = xs[t];
+    }
     block_acc_store<W * NCH / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
 

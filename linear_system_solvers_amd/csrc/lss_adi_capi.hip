@@ -685,6 +685,81 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
 
     const Prob<T> *seg_table(int k) const { return (const Prob<T> *)seg_dev.p + (size_t)k * count; }
 
+    // Derivative terms on the device: terms with an all-zero factor are dropped, all-ones factors are
+    // not stored (code -2) and identical factor vectors are stored once (see Adj::dfac).
+    static void pack_terms(const Terms &tm, size_t nx, size_t ny, T *dst, Adj<T> &A)
+    {
+        const T *src = (const T *)tm.fac.data();
+        const size_t L = nx + ny, per = (size_t)tm.nterm * L;
+        const int64_t sets = tm.sets;
+        auto vec = [&](int t, int64_t k, bool y) { return src + (size_t)k * per + (size_t)t * L + (y ? nx : 0); };
+        auto all = [&](int t, bool y, T v) {
+            const size_t n = y ? ny : nx;
+            for (int64_t k = 0; k < sets; ++k)
+                for (size_t i = 0; i < n; ++i)
+                    if (vec(t, k, y)[i] != v)
+                        return false;
+            return true;
+        };
+        auto same = [&](int t, int u, bool y) {
+            const size_t n = y ? ny : nx;
+            for (int64_t k = 0; k < sets; ++k)
+                if (std::memcmp(vec(t, k, y), vec(u, k, y), n * sizeof(T)) != 0)
+                    return false;
+            return true;
+        };
+        std::vector<int> ux, uy;  // representative term of each stored vector
+        int kept = 0;
+        int fx[NTERM], fy[NTERM], tp[NTERM], tq[NTERM];
+        for (int t = 0; t < tm.nterm; ++t) {
+            if (all(t, false, (T)0) || all(t, true, (T)0))
+                continue;
+            int codes[2];
+            for (int y = 0; y < 2; ++y) {
+                std::vector<int> &u = y ? uy : ux;
+                int c = -2;
+                if (!all(t, y, (T)1)) {
+                    c = -1;
+                    for (size_t z = 0; z < u.size() && c < 0; ++z)
+                        if (same(t, u[z], y))
+                            c = (int)z;
+                    if (c < 0) {
+                        c = (int)u.size();
+                        u.push_back(t);
+                    }
+                }
+                codes[y] = c;
+            }
+            fx[kept] = codes[0];
+            fy[kept] = codes[1];
+            tp[kept] = tm.tp[t];
+            tq[kept] = tm.tq[t];
+            ++kept;
+        }
+        const size_t stride = ux.size() * nx + uy.size() * ny;
+        std::vector<T> packed(std::max<size_t>(stride * (size_t)sets, 1));
+        for (int64_t k = 0; k < sets; ++k) {
+            T *o = packed.data() + (size_t)k * stride;
+            for (size_t z = 0; z < ux.size(); ++z)
+                std::memcpy(o + z * nx, vec(ux[z], k, false), nx * sizeof(T));
+            for (size_t z = 0; z < uy.size(); ++z)
+                std::memcpy(o + ux.size() * nx + z * ny, vec(uy[z], k, true), ny * sizeof(T));
+        }
+        HIPCHK(hipMemcpy(dst, packed.data(), packed.size() * sizeof(T), hipMemcpyHostToDevice));
+        A.dfac = dst;
+        A.dfac_sets = (int32_t)sets;
+        A.dfac_stride = (int64_t)stride;
+        A.nterm = kept;
+        A.nfx = (int32_t)ux.size();
+        A.nfy = (int32_t)uy.size();
+        for (int z = 0; z < NTERM; ++z) {
+            A.tp[z] = z < kept ? tp[z] : 0;
+            A.tq[z] = z < kept ? tq[z] : 0;
+            A.fx[z] = z < kept ? fx[z] : -2;
+            A.fy[z] = z < kept ? fy[z] : -2;
+        }
+    }
+
     void adj_setup(int K)
     {
         require(adj_ok, "the adjoint needs nx - 2 <= 1024 and ny - 2 <= 1024 for every problem");
@@ -753,17 +828,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             const Terms *tm = terms.empty() ? nullptr : &terms[order[q]];
             const size_t nfac = tm ? tm->fac.size() / sizeof(T) : 0;
             T *fac = take(nfac);
-            if (tm && tm->nterm > 0) {
-                HIPCHK(hipMemcpy(fac, tm->fac.data(), tm->fac.size(), hipMemcpyHostToDevice));
-                A.dfac = fac;
-                A.dfac_sets = (int32_t)tm->sets;
-                A.dfac_stride = (int64_t)tm->nterm * (int64_t)(nx_ + ny_);
-                A.nterm = tm->nterm;
-                for (int z = 0; z < NTERM; ++z) {
-                    A.tp[z] = tm->tp[z];
-                    A.tq[z] = tm->tq[z];
-                }
-            }
+            if (tm && tm->nterm > 0)
+                pack_terms(*tm, nx_, ny_, fac, A);
             // inverse index maps
             std::vector<uint32_t> h1, h2;
             h1.resize(probs[q].nx);

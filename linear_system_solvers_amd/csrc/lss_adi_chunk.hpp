@@ -154,7 +154,9 @@ template <typename T> __device__ __forceinline__ Eq<T> identity_eq()
 
 // -------------------------------------------------------------------------------------------------
 // Generic line solves.  eq(k) returns the equation of interior unknown k (1..N) with the ends
-// already folded (a = 0 at k = 1, c = 0 at k = N); put(k, x) receives the solution of unknown k.
+// already folded (a = 0 at k = 1, c = 0 at k = N); put(t, k, x) receives the solution of unknown k,
+// the t-th point of the calling thread's chunk (t is a compile-time index after unrolling, so a
+// caller may keep the values in registers and store them later).
 // -------------------------------------------------------------------------------------------------
 
 // One line per wavefront; lane = chunk; the interface system is reduced with shuffles.
@@ -206,7 +208,7 @@ __device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put
 #pragma unroll
     for (int t = 0; t < M; ++t)
         if (t < ch.cnt)
-            put(s + t, ch.value(t, xl, xi));
+            put(t, s + t, ch.value(t, xl, xi));
 }
 
 // W lines per workgroup of W * NCH threads (thread = (chunk chn, line col)); the interface
@@ -279,7 +281,7 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NCH][W], int N, int col, 
 #pragma unroll
     for (int t = 0; t < M; ++t)
         if (t < ch.cnt)
-            put(s + t, ch.value(t, xl, xi));
+            put(t, s + t, ch.value(t, xl, xi));
 }
 
 }  // namespace lssamd

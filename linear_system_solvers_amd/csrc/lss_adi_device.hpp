@@ -87,7 +87,9 @@ constexpr int NTERM = 12;  // derivative terms per problem
 //   elo2, elo1 lambda of the lower (v = 0) Dirichlet end of each v-line, per row
 //   inv*       inverse index maps: rows r with idx1[r] == i are inv1i[inv1o[i] .. inv1o[i + 1]) (same for idx2)
 //   dfac       derivative terms: term t says d q_t(i, j) / d param[tp[t]] = X_t[i] * Y_t[j] for the
-//              transformed coefficient q = a..f (tq[t] = 0..5); per set, nterm blocks of nx X then ny Y values
+//              transformed coefficient q = a..f (tq[t] = 0..5).  Distinct factor vectors are stored
+//              once: per set, nfx X vectors (nx each) then nfy Y vectors (ny each); X_t is vector fx[t]
+//              (-2: all ones, not stored), Y_t is vector fy[t]
 template <typename T> struct Adj {
     T *la, *lb;
     T *lv2, *lx2, *lv1, *lx1;
@@ -96,9 +98,8 @@ template <typename T> struct Adj {
     const uint32_t *inv1o, *inv1i, *inv2o, *inv2i;
     const T *dfac;
     int64_t dfac_stride;
-    int32_t dfac_sets, nterm;
-    int8_t tp[NTERM];
-    int8_t tq[NTERM];
+    int32_t dfac_sets, nterm, nfx, nfy;
+    int32_t tp[NTERM], tq[NTERM], fx[NTERM], fy[NTERM];
 };
 
 // one device-to-device copy of n elements (checkpoint and lambda transfers)

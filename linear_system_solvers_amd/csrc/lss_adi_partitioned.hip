@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
         }
         return q;
     };
-    auto put = [&](int jj, T x) { out[jj] = x; };
+    auto put = [&](int, int jj, T x) { out[jj] = x; };
     wave_solve<T, M>(N, lane, eq, put);
     if (lane == 0) {
         out[0] = lo;
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
         }
         return e;
     };
-    auto put = [&](int k, T x) { out[(int64_t)k * ny + j] = x; };
+    auto put = [&](int, int k, T x) { out[(int64_t)k * ny + j] = x; };
     block_solve<T, M, W>(sh, N, col, chn, colok, eq, put);
     if (colok && chn == 0) {
         out[j] = lo;

@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(64 * TRI_WAVES) k_tridiag_rows(int64_t B, int 
             e.r = sd[i];
             return e;
         };
-        auto put = [&](int kk, T v) { sd[kk - 1] = v; };  // every eq() has run before the first put()
+        auto put = [&](int, int kk, T v) { sd[kk - 1] = v; };  // every eq() has run before the first put()
         wave_solve<T, M>(n, lane, eq, put);
     }
     __syncthreads();
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(XTJ *NCH) k_tridiag_cols(int64_t B, int n, con
         e.r = d[q];
         return e;
     };
-    auto put = [&](int kk, T v) { x[(int64_t)(kk - 1) * B + k] = v; };
+    auto put = [&](int, int kk, T v) { x[(int64_t)(kk - 1) * B + k] = v; };
     block_solve<T, M, XTJ>(sh, n, col, chn, live, eq, put);
 }
 
