@@ -34,15 +34,18 @@ def flags() -> list[str]:
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
+    import concurrent.futures as cf
     cc = hipcc()
-    for src in SOURCES:
-        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
-        cmd = [cc, *flags(), "-c", os.path.join(CSRC, src), "-o", obj]
-        if verbose:
+    objs = [os.path.join(CSRC, src.replace(".hip", ".o")) for src in SOURCES]
+    cmds = [[cc, *flags(), "-c", os.path.join(CSRC, src), "-o", obj] for src, obj in zip(SOURCES, objs)]
+    if verbose:
+        for cmd in cmds:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
-        objs.append(obj)
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "4")), os.cpu_count() or 1))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        for r in ex.map(lambda c: subprocess.run(c), cmds):
+            if r.returncode != 0:
+                raise subprocess.CalledProcessError(r.returncode, r.args)
     tmp = LIB + ".tmp"
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

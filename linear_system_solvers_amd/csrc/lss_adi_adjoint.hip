@@ -49,12 +49,12 @@ __device__ __forceinline__ void add_terms_q(const Prob<T> &p, const Adj<T> &A, i
 {
     const int nx = p.nx, ny = p.ny;
     const gptr<const T> fX = G(A.dfac + (A.dfac_sets > 1 ? (int64_t)step : 0) * A.dfac_stride);
-    const gptr<const T> fY = fX + (int64_t)A.nfx * nx;
+    const gptr<const T> fY = fX + (int)A.nfx * nx;
     for (int t = 0; t < A.nterm; ++t) {
         const int q = A.tq[t];
         const T gq = q == 0 ? ga : (q == 1 ? gb : (q == 2 ? gc : (q == 3 ? gd : (q == 4 ? ge : gf))));
-        const T xv = A.fx[t] >= 0 ? fX[(int64_t)A.fx[t] * nx + i] : (T)1;
-        const T yv = A.fy[t] >= 0 ? fY[(int64_t)A.fy[t] * ny + j] : (T)1;
+        const T xv = A.fx[t] >= 0 ? fX[(int)A.fx[t] * nx + i] : (T)1;
+        const T yv = A.fy[t] >= 0 ? fY[(int)A.fy[t] * ny + j] : (T)1;
         const double v = (double)gq * (double)xv * (double)yv;
         const int k = A.tp[t];
 #pragma unroll
@@ -207,7 +207,7 @@ __device__ __forceinline__ T lam_y2(const Prob<T> &p, const Adj<T> &A, SF &&S, i
             if (!inside(p, ei, ej))
                 continue;
             const Stencil<T> s = S(di, dj);
-            const int64_t q = (int64_t)ei * ny + ej;
+            const int q = (int)ei * ny + ej;
             if (hv) {
                 T w = w_h(p, s, -di, -dj, 2);
                 if (dj == 0)
@@ -251,20 +251,20 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
         const gptr<const T> cs = coeff_set(p, step);
         const T one = (T)1, theta = p.theta;
         const bool hv = p.scheme == SCHEME_HV;
-        const gptr<const T> yrow = G((last ? un : (const T *)p.y2) + (int64_t)i * ny);
-        const gptr<const T> irow = G(((last && hv) ? (const T *)p.y2 : u) + (int64_t)i * ny);
-        const gptr<T> L = G((last ? A.lv2 : A.lv1) + (int64_t)i * ny);
+        const gptr<const T> yrow = G((last ? un : (const T *)p.y2) + (int)i * ny);
+        const gptr<const T> irow = G(((last && hv) ? (const T *)p.y2 : u) + (int)i * ny);
+        const gptr<T> L = G((last ? A.lv2 : A.lv1) + (int)i * ny);
         const gptr<const uint32_t> inv2o = G(A.inv2o), inv2i = G(A.inv2i);
         // lambda of the stage output at (i, j): B^T (explicit_boundary.hpp:202-238) for the last stage
-        const gptr<const T> ly2 = G((const T *)A.ly2) + (int64_t)i * ny;
+        const gptr<const T> ly2 = G((const T *)A.ly2) + (int)i * ny;
         auto lam_out = [&](int j) -> T {
             if (!last)
                 return ly2[j];
-            T v = lamn[(int64_t)i * ny + j];
+            T v = lamn[(int)i * ny + j];
             if (i == P - 1 || i == P - 2) {
                 T sB = (T)0;
                 for (uint32_t k = inv2o[j]; k < inv2o[j + 1]; ++k)
-                    sB += lamn[(int64_t)P * ny + inv2i[k]];
+                    sB += lamn[(int)P * ny + inv2i[k]];
                 v += (i == P - 1) ? ((T)4 / (T)3) * sB : -(sB / (T)3);
             }
             return v;
@@ -336,7 +336,7 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
     const gptr<const T> lv2 = G((const T *)A.lv2), lv1 = G((const T *)A.lv1), lx2 = G((const T *)A.lx2);
     const gptr<const T> y1 = G((const T *)p.y1), y2 = G((const T *)p.y2);
     auto lam_out = [&](int k) -> T {
-        const int64_t q = (int64_t)k * ny + j;
+        const int q = (int)k * ny + j;
         if (stage == 2)
             return lv2[q];
         if (dr)
@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
         for (int di = -1; di <= 1; ++di) {
             const int ei = k + di;
             if (ei >= 1 && ei <= N)
-                v += w_h(p, stencil_at(p, cs, ei, j), -di, 0, 0) * lx2[(int64_t)ei * ny + j];
+                v += w_h(p, stencil_at(p, cs, ei, j), -di, 0, 0) * lx2[(int)ei * ny + j];
         }
         return v;
     };
@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
     // the solution stays in registers until every load of the parameter terms is issued
     T xs[M];
     auto put = [&](int t, int k, T x) {
-        const int64_t q = (int64_t)k * ny + j;
+        const int q = (int)k * ny + j;
         xs[t] = x;
         if (A.nterm == 0)
             return;
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
             g.Pt += x * u[q + 1];
             g.C += x * gamma * (u[q - ny - 1] - u[q - ny + 1] - u[q + ny - 1] + u[q + ny + 1]);
         } else {
-            auto D = [&](int64_t o) { return y2[q + o] - u[q + o]; };
+            auto D = [&](int o) { return y2[q + o] - u[q + o]; };
             const T cornerD = D(ny + 1) - D(ny - 1) - D(-ny + 1) + D(-ny - 1);
             if (!hv) {  // rhs_intermed_3
                 g.M += -theta * x * u[q - ny];
@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
 #pragma unroll
         for (int t = 0; t < M; ++t)
             if (t < cnt)
-                L[(int64_t)(s0 + t) * ny + j] = xs[t];
+                L[(int)(s0 + t) * ny + j] = xs[t];
     }
     block_acc_store<W * NCH / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
@@ -483,7 +483,7 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
     if (run && tr < tl.count && c < ny) {
         if (WHAT == 1) {
             if (r >= 1 && r <= P - 1)
-                G(A.ly2)[(int64_t)r * ny + c] = lam_y2(p, A, S, r, c);
+                G(A.ly2)[(int)r * ny + c] = lam_y2(p, A, S, r, c);
         } else {
             const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
             const gptr<T> lam = G((step & 1) ? A.lb : A.la);
@@ -514,7 +514,7 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
                     if (!inside(p, ei, ej))
                         continue;
                     const Stencil<T> st = S(di, dj);
-                    const int64_t q = (int64_t)ei * ny + ej;
+                    const int q = (int)ei * ny + ej;
                     if (di == 0 && !hv)
                         v += w_rv(p, st, -dj) * lv2[q];
                     if (di == 0 && !dr)
@@ -535,7 +535,7 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
                         continue;
                     const Stencil<T> st = (j >= c - 1 && j <= c + 1) ? S(ei - r, j - c) : stencil_at(p, cs, ei, j);
                     const T w = (r == 0) ? theta * st.M : theta * st.P;  // -(folded coefficient)
-                    const int64_t q = (int64_t)ei * ny + j;
+                    const int q = (int)ei * ny + j;
                     T l = lx1[q];
                     if (!dr)
                         l += lx2[q];
@@ -567,15 +567,15 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
                 }
                 if (c == 0 && r >= 1 && A.nterm > 0) {
                     const T la = lamA(r);
-                    const gptr<const T> ur = u + (int64_t)r * ny;
-                    const T gd = (r < P) ? la * delta * (u[(int64_t)(r + 1) * ny] - u[(int64_t)(r - 1) * ny])
+                    const gptr<const T> ur = u + (int)r * ny;
+                    const T gd = (r < P) ? la * delta * (u[(int)(r + 1) * ny] - u[(int)(r - 1) * ny])
                                          : -la * delta * ((T)2 * p.h1 * bs[nx + ny]);
                     const T ge = la * ni * (-(T)3 * ur[0] + (T)4 * ur[1] - ur[2]);
                     const T gf = la * rho * ur[0];
                     add_terms_q(p, A, step, r, 0, (T)0, (T)0, (T)0, gd, ge, gf, acc);
                 }
             }
-            lam[(int64_t)r * ny + c] = v;
+            lam[(int)r * ny + c] = v;
         }
     }
     if (WHAT == 0)

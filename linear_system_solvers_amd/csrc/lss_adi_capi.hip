@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "lss_adi_device.hpp"
+#include "lss_adi_stencil.hpp"
 #include "lss_amd.h"
 #include "lss/lss_adi_sampler.hpp"
 
@@ -33,8 +34,9 @@ template <typename T> __global__ void k_vsweep(const Prob<T> *, const Tile *, in
 template <typename T> __global__ void k_hv_y3(const Prob<T> *, const Tile *, int);
 template <typename T>
 __global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *);
-template <typename T, int M, int STAGE, int W> __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int, int);
-template <typename T, int M, int VR> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int, int);
+template <typename T, int M, int STAGE, int W, int PAT>
+__global__ void k_xsweep_p(const Prob<T> *, const Tile *, int, int);
+template <typename T, int M, int VR, int PAT> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int, int);
 template <typename T, int M>
 __global__ void k_adj_v(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
 template <typename T, int M, int W>
@@ -192,6 +194,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     int pmx = 1, pmy = 1;              // points per chunk bound (template M) of S-lines and v-lines
     int xw = 16, vr = 4;               // S-lines per x-sweep workgroup, v-lines per v-sweep workgroup
     int swz = 0;                       // XCD-aware tile order (xcd_tile) in the sweeps
+    int fpat = PAT_GENERIC;            // factor pattern shared by every problem (SEPARABLE), see FPat
     bool tuned = false;                // the autotuner has chosen xw and vr
     bool adj_ok = false;               // every line fits the partitioned solver (needed by the adjoint)
     hipStream_t cap_stream = nullptr;
@@ -211,8 +214,9 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     static void validate(const lss_adi_desc &d)
     {
         require(d.nx >= 4 && d.ny >= 4, "nx and ny must be >= 4");
-        require(d.nx <= (1 << 20) && d.ny <= (1 << 20) && d.nx * d.ny <= (int64_t)1 << 31,
-                "grid too large for one problem");
+        // kernels index a problem's arrays with 32-bit offsets (7 coefficient fields included)
+        require(d.nx <= (1 << 20) && d.ny <= (1 << 20) && d.nx * d.ny <= (int64_t)1 << 28,
+                "grid too large for one problem (nx * ny <= 2^28)");
         require(d.steps >= 1 && d.steps < (1 << 30), "steps must be >= 1");
         require(d.splitting >= 0 && d.splitting <= 3, "unknown splitting method");
         require(d.tridiagonal == LSS_THOMAS_LU || d.tridiagonal == LSS_DOUBLE_SWEEP ||
@@ -387,6 +391,18 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             max_steps = std::max(max_steps, (int)d.steps);
             base += L.bytes;
         }
+        // a factor pattern shared by every problem lets the sweeps use the specialized stencil
+        fpat = PAT_GENERIC;
+        for (int cand : {(int)PAT_HESTON, (int)PAT_SABR0}) {
+            const int *c = cand == PAT_HESTON ? FPat<PAT_HESTON>::c : FPat<PAT_SABR0>::c;
+            bool all = true;
+            for (int64_t q = 0; q < n && all; ++q)
+                all = probs[q].coeff_mode == COEFF_SEPARABLE && std::equal(c, c + 24, probs[q].fcode);
+            if (all) {
+                fpat = cand;
+                break;
+            }
+        }
         // tile tables
         for (int tb = 0; tb < TB_N; ++tb) {
             prefix[tb].assign(n + 1, 0);
@@ -472,15 +488,25 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     }
 
     // partitioned sweeps with the chosen tile/lane mapping (xw, vr) and chunk bound (pmx, pmy)
-    template <int M, int W> void launch_x_m(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
+    template <int M, int W, int PAT> void launch_x_p(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
         const int tb = W == 16 ? TB_X : TB_X8;
         const dim3 g((unsigned)prefix[tb][na]), b(W * P_NCH);
         const Tile *tx = (const Tile *)tiles_dev.p + tile_off[tb];
         if (stage == 1)
-            hipLaunchKernelGGL((k_xsweep_p<T, M, 1, W>), g, b, 0, st, pd, tx, s, swz);
+            hipLaunchKernelGGL((k_xsweep_p<T, M, 1, W, PAT>), g, b, 0, st, pd, tx, s, swz);
         else
-            hipLaunchKernelGGL((k_xsweep_p<T, M, 2, W>), g, b, 0, st, pd, tx, s, swz);
+            hipLaunchKernelGGL((k_xsweep_p<T, M, 2, W, PAT>), g, b, 0, st, pd, tx, s, swz);
+    }
+    template <int M, int W> void launch_x_m(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
+    {
+        if constexpr (M <= 8) {
+            if (fpat == PAT_HESTON)
+                return launch_x_p<M, W, PAT_HESTON>(stage, s, st, pd, na);
+            if (fpat == PAT_SABR0)
+                return launch_x_p<M, W, PAT_SABR0>(stage, s, st, pd, na);
+        }
+        launch_x_p<M, W, PAT_GENERIC>(stage, s, st, pd, na);
     }
     template <int W> void launch_x_w(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
@@ -499,12 +525,22 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         else
             launch_x_w<16>(stage, s, st, pd, na);
     }
-    template <int M, int VR> void launch_v_m(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
+    template <int M, int VR, int PAT> void launch_v_p(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
         const int tb = VR == 4 ? TB_V : TB_V16;
         const dim3 g((unsigned)prefix[tb][na]), b(64 * VR);
-        hipLaunchKernelGGL((k_vsweep_p<T, M, VR>), g, b, 0, st, pd, (const Tile *)tiles_dev.p + tile_off[tb], s,
-                           stage, swz);
+        hipLaunchKernelGGL((k_vsweep_p<T, M, VR, PAT>), g, b, 0, st, pd, (const Tile *)tiles_dev.p + tile_off[tb],
+                           s, stage, swz);
+    }
+    template <int M, int VR> void launch_v_m(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
+    {
+        if constexpr (M <= 8) {
+            if (fpat == PAT_HESTON)
+                return launch_v_p<M, VR, PAT_HESTON>(stage, s, st, pd, na);
+            if (fpat == PAT_SABR0)
+                return launch_v_p<M, VR, PAT_SABR0>(stage, s, st, pd, na);
+        }
+        launch_v_p<M, VR, PAT_GENERIC>(stage, s, st, pd, na);
     }
     template <int VR> void launch_v_r(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {

@@ -34,6 +34,29 @@ constexpr int TILE = 64;  // lines (or points) per workgroup tile: one wavefront
 template <typename T> using gptr = __attribute__((address_space(1))) T *;
 template <typename T> __device__ __forceinline__ gptr<T> G(T *p) { return (gptr<T>)p; }
 
+// Loads and stores at an unsigned 32-bit element index from a base pointer: with the base in scalar
+// registers this is the global_load/store "saddr + 32-bit offset" form, free of 64-bit address
+// arithmetic (a problem's arrays hold at most 2^28 elements).
+template <typename T> __device__ __forceinline__ T ldg(gptr<const T> base, uint32_t idx)
+{
+    return *(gptr<const T>)((__attribute__((address_space(1))) const char *)base + idx * (uint32_t)sizeof(T));
+}
+template <typename T> __device__ __forceinline__ void stg(gptr<T> base, uint32_t idx, T v)
+{
+    *(gptr<T>)((__attribute__((address_space(1))) char *)base + idx * (uint32_t)sizeof(T)) = v;
+}
+// read-only array and read-only view at one grid point (offsets in elements, may be negative)
+template <typename T> struct GArr {
+    gptr<const T> b;
+    __device__ __forceinline__ T operator[](uint32_t i) const { return ldg(b, i); }
+    __device__ __forceinline__ GArr operator+(uint32_t o) const { return GArr{b + o}; }
+};
+template <typename T> struct GAt {
+    gptr<const T> b;
+    uint32_t q;
+    __device__ __forceinline__ T operator[](int d) const { return ldg(b, q + (uint32_t)d); }
+};
+
 template <typename T> struct Prob {
     int32_t nx, ny, steps, scheme;
     int32_t tri, coeff_mode, coeff_sets, bnd_sets;

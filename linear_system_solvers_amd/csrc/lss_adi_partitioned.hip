@@ -27,7 +27,7 @@ namespace lssamd {
 // v-line sweep.  stage 1: rhs_v(u, Y1) -> Y2 (DR: next); stage 2: rhs_v(u, Y3) (CS/MCS) or
 // rhs_v(Y2, Y4) (HV) -> next.  Ends colA0/colA1[idx1[i]] (implicit_kernel.hpp:69-84).
 // -------------------------------------------------------------------------------------------------
-template <typename T, int M, int VR>
+template <typename T, int M, int VR, int PAT>
 __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict__ probs,
                                                       const Tile *__restrict__ tiles, int step, int stage, int swz)
 {
@@ -40,7 +40,7 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
     const int ny = p.ny, N = ny - 2;
     const T *u = (step & 1) ? p.b : p.a;
     T *nxt = (step & 1) ? p.a : p.b;
-    const gptr<const T> cs = coeff_set(p, step);
+    const GArr<T> cs{coeff_set(p, step)};
     const T one = (T)1.0, theta = p.theta;
     const T *in0, *inh0;
     T *out0;
@@ -57,19 +57,18 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
         inh0 = p.y3;
         out0 = nxt;
     }
-    const int64_t row = (int64_t)i * ny;
-    const gptr<const T> in = G(in0 + row);
-    const gptr<const T> inh = G(inh0 + row);
-    const gptr<T> out = G(out0 + row);
+    const uint32_t row = (uint32_t)i * (uint32_t)ny;
+    const gptr<const T> in = G(in0), inh = G(inh0);
+    const gptr<T> out = G(out0);
     const uint32_t ii = p.idx1[i];
     const T lo = G(p.colA0)[ii], hi = G(p.colA1)[ii];
     auto eq = [&](int jj) -> Eq<T> {
-        const Stencil<T> st = stencil_at(p, cs, i, jj);
+        const Stencil<T> st = stencil_at<T, PAT>(p, cs, i, jj);
         Eq<T> q;
         q.a = (-theta * st.Mt);
         q.b = (one + theta * st.W);
         q.c = (-theta * st.Pt);
-        q.r = rhs_v(p, st, in + jj, inh + jj);
+        q.r = rhs_v(p, st, GAt<T>{in, row + jj}, GAt<T>{inh, row + jj});
         if (jj == 1) {  // fold the Dirichlet ends into the first / last equation
             q.r = q.r - q.a * lo;
             q.a = (T)0;
@@ -80,11 +79,11 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
         }
         return q;
     };
-    auto put = [&](int, int jj, T x) { out[jj] = x; };
+    auto put = [&](int, int jj, T x) { stg(out, row + jj, x); };
     wave_solve<T, M>(N, lane, eq, put);
     if (lane == 0) {
-        out[0] = lo;
-        out[ny - 1] = hi;
+        stg(out, row, lo);
+        stg(out, row + ny - 1, hi);
     }
 }
 
@@ -93,7 +92,7 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
 // HV rhs_intermed_4(Y2, Y3) with Y3 evaluated in place from (u, Y1, Y2) -> Y4 (= y4).
 // Ends u(0, idx2[j]), u(nx-1, idx2[j]) (implicit_kernel.hpp:86-101).
 // -------------------------------------------------------------------------------------------------
-template <typename T, int M, int STAGE, int W>
+template <typename T, int M, int STAGE, int W, int PAT>
 __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__ probs,
                                                      const Tile *__restrict__ tiles, int step, int swz)
 {
@@ -107,29 +106,30 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
     const int j = tl.first + (colok ? col : 0);
     const int nx = p.nx, ny = p.ny, N = nx - 2;
     const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
-    const gptr<const T> cs = coeff_set(p, step);
+    const GArr<T> cs{coeff_set(p, step)};
     const gptr<const T> y1 = G((const T *)p.y1), y2 = G((const T *)p.y2);
     const T one = (T)1.0, theta = p.theta;
     const int kind = (STAGE == 1) ? 0 : (p.scheme == SCHEME_CS ? 1 : (p.scheme == SCHEME_MCS ? 2 : 3));
     const gptr<T> out = G((STAGE == 1) ? p.y1 : (kind == 3 ? p.y4 : p.y3));
     const uint32_t jj = p.idx2[j];
-    const T lo = u[jj], hi = u[(int64_t)(nx - 1) * ny + jj];
+    const T lo = ldg(u, jj), hi = ldg(u, (uint32_t)(nx - 1) * (uint32_t)ny + jj);
     auto eq = [&](int k) -> Eq<T> {
-        const int64_t q = (int64_t)k * ny + j;
-        const Stencil<T> st = stencil_at(p, cs, k, j);
+        const uint32_t q = (uint32_t)k * (uint32_t)ny + (uint32_t)j;
+        const Stencil<T> st = stencil_at<T, PAT>(p, cs, k, j);
         Eq<T> e;
         e.a = (-theta * st.M);
         e.b = (one + theta * st.Z);
         e.c = (-theta * st.P);
+        const GAt<T> uq{u, q};
         if (STAGE == 1) {
-            e.r = rhs_intermed_1(p, st, u + q, ny);
+            e.r = rhs_intermed_1(p, st, uq, ny);
         } else if (kind == 1) {
-            e.r = rhs_intermed_3<T, false>(p, st, u + q, y1 + q, y2 + q, ny);
+            e.r = rhs_intermed_3<T, false>(p, st, uq, GAt<T>{y1, q}, GAt<T>{y2, q}, ny);
         } else if (kind == 2) {
-            e.r = rhs_intermed_3<T, true>(p, st, u + q, y1 + q, y2 + q, ny);
+            e.r = rhs_intermed_3<T, true>(p, st, uq, GAt<T>{y1, q}, GAt<T>{y2, q}, ny);
         } else {
-            const gptr<const T> y2q = y2 + q;
-            const T y3 = hv_y3_at(p, st, u + q, y1 + q, y2q, ny);
+            const GAt<T> y2q{y2, q};
+            const T y3 = hv_y3_at(p, st, uq, GAt<T>{y1, q}, y2q, ny);
             e.r = (-theta * st.M * y2q[-ny]) + (theta * st.Z * y2q[0]) - (theta * st.P * y2q[ny]) + y3;
         }
         if (k == 1) {
@@ -142,31 +142,33 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
         }
         return e;
     };
-    auto put = [&](int, int k, T x) { out[(int64_t)k * ny + j] = x; };
+    auto put = [&](int, int k, T x) { stg(out, (uint32_t)k * (uint32_t)ny + (uint32_t)j, x); };
     block_solve<T, M, W>(sh, N, col, chn, colok, eq, put);
     if (colok && chn == 0) {
-        out[j] = lo;
-        out[(int64_t)(nx - 1) * ny + j] = hi;
+        stg(out, (uint32_t)j, lo);
+        stg(out, (uint32_t)(nx - 1) * (uint32_t)ny + (uint32_t)j, hi);
     }
 }
 
-#define LSS_INST_P(T, M)                                                                        \
-    template __global__ void k_vsweep_p<T, M, 4>(const Prob<T> *, const Tile *, int, int, int);  \
-    template __global__ void k_vsweep_p<T, M, 16>(const Prob<T> *, const Tile *, int, int, int); \
-    template __global__ void k_xsweep_p<T, M, 1, 16>(const Prob<T> *, const Tile *, int, int);   \
-    template __global__ void k_xsweep_p<T, M, 2, 16>(const Prob<T> *, const Tile *, int, int);   \
-    template __global__ void k_xsweep_p<T, M, 1, 8>(const Prob<T> *, const Tile *, int, int);    \
-    template __global__ void k_xsweep_p<T, M, 2, 8>(const Prob<T> *, const Tile *, int, int);
-LSS_INST_P(double, 1)
-LSS_INST_P(double, 2)
-LSS_INST_P(double, 4)
-LSS_INST_P(double, 8)
-LSS_INST_P(double, 16)
-LSS_INST_P(float, 1)
-LSS_INST_P(float, 2)
-LSS_INST_P(float, 4)
-LSS_INST_P(float, 8)
-LSS_INST_P(float, 16)
+#define LSS_INST_P(T, M, PAT)                                                                          \
+    template __global__ void k_vsweep_p<T, M, 4, PAT>(const Prob<T> *, const Tile *, int, int, int);    \
+    template __global__ void k_vsweep_p<T, M, 16, PAT>(const Prob<T> *, const Tile *, int, int, int);   \
+    template __global__ void k_xsweep_p<T, M, 1, 16, PAT>(const Prob<T> *, const Tile *, int, int);     \
+    template __global__ void k_xsweep_p<T, M, 2, 16, PAT>(const Prob<T> *, const Tile *, int, int);     \
+    template __global__ void k_xsweep_p<T, M, 1, 8, PAT>(const Prob<T> *, const Tile *, int, int);      \
+    template __global__ void k_xsweep_p<T, M, 2, 8, PAT>(const Prob<T> *, const Tile *, int, int);
+#define LSS_INST_PM(T, M) LSS_INST_P(T, M, 0) LSS_INST_P(T, M, 1) LSS_INST_P(T, M, 2)
+LSS_INST_PM(double, 1)
+LSS_INST_PM(double, 2)
+LSS_INST_PM(double, 4)
+LSS_INST_PM(double, 8)
+LSS_INST_P(double, 16, 0)
+LSS_INST_PM(float, 1)
+LSS_INST_PM(float, 2)
+LSS_INST_PM(float, 4)
+LSS_INST_PM(float, 8)
+LSS_INST_P(float, 16, 0)
+#undef LSS_INST_PM
 #undef LSS_INST_P
 
 }  // namespace lssamd

@@ -10,29 +10,53 @@ template <typename T> struct Stencil {
     T M, Mt, P, Pt, Z, W, C;
 };
 
-// factor slot `slot` of a packed SEPARABLE block at index i (codes: Prob::fcode)
-template <typename T, typename P>
-__device__ __forceinline__ T factor(P B, int code, int n, int i)
+// Factor patterns of SEPARABLE coefficient sets (codes of Prob::fcode: slots 2q + r of X, then of
+// Y; -1 zero, -2 one, k >= 0 packed vector k).  PAT 0 reads a problem's codes at run time; PAT 1
+// and 2 are the patterns of the reference models' data (Heston with r != 0, SABR with r = 0), fixed
+// at compile time so that zero and unit factors cost nothing.  The plan selects a pattern only
+// when every problem has exactly its codes.
+enum : int32_t { PAT_GENERIC = 0, PAT_HESTON = 1, PAT_SABR0 = 2, PAT_COUNT = 3 };
+template <int PAT> struct FPat;
+template <> struct FPat<PAT_HESTON> {
+    static constexpr int c[24] = {0, -1, -2, -1, 1, -1, 2, 3, -2, -1, -2, -1,
+                                  0, -1, 1, -1, 2, -1, -2, 3, 4, -1, 5, -1};
+};
+template <> struct FPat<PAT_SABR0> {
+    static constexpr int c[24] = {0, -1, -2, -1, 1, -1, -1, 2, -2, -1, -2, -1,
+                                  0, -1, 1, -1, 2, -1, -2, 3, 4, -1, -1, -1};
+};
+
+// factor slot SLOT of a packed SEPARABLE block B (length-n vectors) at index i
+template <typename T, int PAT, int SLOT, typename P>
+__device__ __forceinline__ T factor(const Prob<T> &p, P B, int n, int i)
 {
-    return code >= 0 ? B[(int64_t)code * n + i] : (code == -2 ? (T)1 : (T)0);
+    if constexpr (PAT == PAT_GENERIC) {
+        const int code = p.fcode[SLOT];
+        return code >= 0 ? B[code * n + i] : (code == -2 ? (T)1 : (T)0);
+    } else {
+        constexpr int code = FPat<PAT>::c[SLOT];
+        if constexpr (code >= 0)
+            return B[code * n + i];
+        else
+            return code == -2 ? (T)1 : (T)0;
+    }
 }
 
-// q(i, j) = X[2k][i] Y[2k][j] + X[2k+1][i] Y[2k+1][j]
-template <typename T, typename P>
-__device__ __forceinline__ T sep(const Prob<T> &p, P X, P Y, int k, int i, int j)
+// q(i, j) = X[2K][i] Y[2K][j] + X[2K+1][i] Y[2K+1][j]
+template <typename T, int PAT, int K, typename P>
+__device__ __forceinline__ T sep(const Prob<T> &p, P X, P Y, int i, int j)
 {
-    const int32_t *c = p.fcode;
-    return factor<T>(X, c[2 * k], p.nx, i) * factor<T>(Y, c[12 + 2 * k], p.ny, j) +
-           factor<T>(X, c[2 * k + 1], p.nx, i) * factor<T>(Y, c[12 + 2 * k + 1], p.ny, j);
+    return factor<T, PAT, 2 * K>(p, X, p.nx, i) * factor<T, PAT, 12 + 2 * K>(p, Y, p.ny, j) +
+           factor<T, PAT, 2 * K + 1>(p, X, p.nx, i) * factor<T, PAT, 12 + 2 * K + 1>(p, Y, p.ny, j);
 }
 
 // Stencil coefficients at (zeta_i, eta_j): general_heston_equation_coefficients (coefficients.hpp:89-95).
-template <typename T, typename P>
+template <typename T, int PAT = PAT_GENERIC, typename P>
 __device__ __forceinline__ Stencil<T> stencil_at(const Prob<T> &p, P cs, int i, int j)
 {
     Stencil<T> s;
-    if (p.coeff_mode == COEFF_FIELDS) {
-        const int64_t nxy = (int64_t)p.nx * p.ny, q = (int64_t)i * p.ny + j;
+    if (PAT == PAT_GENERIC && p.coeff_mode == COEFF_FIELDS) {
+        const int nxy = p.nx * p.ny, q = i * p.ny + j;
         s.M = cs[q];
         s.Mt = cs[nxy + q];
         s.P = cs[2 * nxy + q];
@@ -41,10 +65,10 @@ __device__ __forceinline__ Stencil<T> stencil_at(const Prob<T> &p, P cs, int i, 
         s.W = cs[5 * nxy + q];
         s.C = cs[6 * nxy + q];
     } else {
-        const P X = cs, Y = cs + 12 * (int64_t)p.nx;
-        const T a = sep<T>(p, X, Y, 0, i, j), b = sep<T>(p, X, Y, 1, i, j);
-        const T c = sep<T>(p, X, Y, 2, i, j), d = sep<T>(p, X, Y, 3, i, j);
-        const T e = sep<T>(p, X, Y, 4, i, j), f = sep<T>(p, X, Y, 5, i, j);
+        const P X = cs, Y = cs + 12 * p.nx;
+        const T a = sep<T, PAT, 0>(p, X, Y, i, j), b = sep<T, PAT, 1>(p, X, Y, i, j);
+        const T c = sep<T, PAT, 2>(p, X, Y, i, j), d = sep<T, PAT, 3>(p, X, Y, i, j);
+        const T e = sep<T, PAT, 4>(p, X, Y, i, j), f = sep<T, PAT, 5>(p, X, Y, i, j);
         const T two = (T)2.0, half = (T)0.5;
         s.M = (p.alpha * a - p.delta * d);
         s.Mt = (p.beta * b - p.ni * e);
@@ -62,15 +86,15 @@ template <typename T, typename P>
 __device__ __forceinline__ void def_row0(const Prob<T> &p, P cs, int i, T &D, T &E, T &F)
 {
     if (p.coeff_mode == COEFF_FIELDS) {
-        const P r = cs + 7 * (int64_t)p.nx * p.ny;
+        const P r = cs + 7 * (int)p.nx * p.ny;
         D = r[i];
         E = r[p.nx + i];
         F = r[2 * p.nx + i];
     } else {
-        const P X = cs, Y = cs + 12 * (int64_t)p.nx;
-        D = sep<T>(p, X, Y, 3, i, 0);
-        E = sep<T>(p, X, Y, 4, i, 0);
-        F = sep<T>(p, X, Y, 5, i, 0);
+        const P X = cs, Y = cs + 12 * (int)p.nx;
+        D = sep<T, PAT_GENERIC, 3>(p, X, Y, i, 0);
+        E = sep<T, PAT_GENERIC, 4>(p, X, Y, i, 0);
+        F = sep<T, PAT_GENERIC, 5>(p, X, Y, i, 0);
     }
 }
 
