@@ -491,7 +491,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "heston_book_2048_seed3", "problems": len(models),
                    "grid_point_steps_per_solve": total_ps, "coeff": "separable", "line_solver": "partitioned",
-                   "graph": not args.no_graph, "mapping_s_v_lines_per_group": list(plan.autotune(0))},
+                   "graph": not args.no_graph, "kernel_config": plan.kernel_config()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                      "kernel": ("k_" + dom.rstrip("12") + ("_p" if dom.startswith(("x", "v")) else "")),

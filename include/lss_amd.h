@@ -106,7 +106,7 @@ int lss_adi_plan_destroy(lss_adi_plan *plan);
 int lss_adi_run_dev(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, void *stream, int32_t use_graph);
 /*
  * Autotuner (no counterpart in the reference): times the tile/lane mappings of the partitioned sweeps
- * (S-lines per x-sweep workgroup: 16 or 8; v-lines per v-sweep workgroup: 4 or 16; tiles in launch
+ * (S-lines per x-sweep workgroup: 16, 8 or 4; v-lines per v-sweep workgroup: 4 or 16; tiles in launch
  * order or XCD-contiguous order) on the first steps of the loop and keeps the fastest; the mappings
  * give identical results.  Runs by itself before a plan's first solve; `again` != 0 re-times (and
  * drops a captured graph).  Overwrites the plan's state buffers, synchronous with respect to
@@ -114,9 +114,13 @@ int lss_adi_run_dev(lss_adi_plan *plan, const void *const *u_in, void *const *u_
  */
 int lss_adi_autotune(lss_adi_plan *plan, void *stream, int32_t again, int32_t *s_lines_per_group,
                      int32_t *v_lines_per_group, int32_t *xcd_order);
-/* Sets the mapping instead of timing it (16 or 8 S-lines, 4 or 16 v-lines per workgroup, 0 or 1). */
+/* Sets the mapping instead of timing it (16, 8 or 4 S-lines, 4 or 16 v-lines per workgroup, 0 or 1). */
 int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v_lines_per_group,
                         int32_t xcd_order);
+/* Kernel configuration of a plan (8 values): line solver, S-lines and v-lines per workgroup, XCD order,
+ * compile-time factor pattern of the coefficients (0 none, 1 Heston, 2 SABR with r = 0), chunk bounds
+ * of S-lines and v-lines, and whether the autotuner has run. */
+int lss_plan_kernel_config(const lss_adi_plan *plan, int32_t *config);
 int lss_adi_run_host(lss_adi_plan *plan, const void *const *u_in, void *const *u_out);
 
 /*

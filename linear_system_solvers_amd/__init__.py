@@ -36,7 +36,7 @@ EXPORTS = ("lss_last_error", "lss_version", "lss_adi_plan_create", "lss_adi_plan
            "lss_adi_run_host", "lss_model_plan_create", "lss_model_initial", "lss_model_grid", "lss_plan_problem_shape",
            "lss_tridiag_batched", "lss_adi_run_profiled", "lss_adi_set_param_terms", "lss_plan_param_count",
            "lss_adi_forward", "lss_adi_backward", "lss_adi_autotune",
-           "lss_adi_set_mapping")
+           "lss_adi_set_mapping", "lss_plan_kernel_config")
 MAX_PARAMS = 6
 MAX_TERMS = 12
 HESTON_PARAMS = ("kappa", "theta_v", "sigma", "rho", "r")
@@ -159,6 +159,7 @@ def lib():
                                               C.POINTER(C.c_int32), C.c_int64, C.c_void_p]
         L.lss_plan_param_count.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]
         L.lss_adi_set_mapping.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
+        L.lss_plan_kernel_config.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
         L.lss_adi_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.lss_adi_forward.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32,
@@ -248,6 +249,13 @@ class Plan:
         x, v, z = C.c_int32(), C.c_int32(), C.c_int32()
         _check(lib().lss_adi_autotune(self._h, C.c_void_p(stream), int(again), C.byref(x), C.byref(v), C.byref(z)))
         return x.value, v.value, z.value
+
+    def kernel_config(self) -> dict:
+        c = (C.c_int32 * 8)()
+        _check(lib().lss_plan_kernel_config(self._h, c))
+        keys = ("line_solver", "s_lines_per_group", "v_lines_per_group", "xcd_order", "factor_pattern",
+                "s_chunk_bound", "v_chunk_bound", "tuned")
+        return dict(zip(keys, list(c)))
 
     def set_mapping(self, s_lines_per_group: int, v_lines_per_group: int, xcd_order: int = 0) -> None:
         _check(lib().lss_adi_set_mapping(self._h, s_lines_per_group, v_lines_per_group, xcd_order))

@@ -145,6 +145,7 @@ struct lss_adi_plan {
     virtual void invalidate_adjoint() = 0;
     virtual void tune(hipStream_t stream, bool again, int *xw, int *vr, int *xcd) = 0;
     virtual void set_mapping(int xw, int vr, int xcd) = 0;
+    virtual void kernel_config(int32_t *cfg) const = 0;
     virtual void run(const void *const *u_in, void *const *u_out, hipStream_t stream, bool use_graph, bool host) = 0;
     virtual void profile(const void *const *u_in, void *const *u_out, hipStream_t stream, double *ms, double *bytes,
                          int64_t *launches) = 0;
@@ -185,7 +186,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     // tile tables: rows [0,nx), cols [0,ny), S-lines [1,ny-2], v-lines [1,nx-2], interior rows
     // adjoint: S-line tiles of P_XTJ, v-line tiles of P_VROWS, one tile per row (all rows)
     // forward partitioned sweeps: S-line tiles of 16 (TB_X) or 8 (TB_X8), v-line tiles of 4 (TB_V) or 16 (TB_V16)
-    enum { TB_I = 0, TB_J, TB_X, TB_V, TB_R, TB_AX, TB_AV, TB_U, TB_X8, TB_V16, TB_N };
+    enum { TB_I = 0, TB_J, TB_X, TB_V, TB_R, TB_AX, TB_AV, TB_U, TB_X8, TB_V16, TB_X4, TB_N };
     std::vector<Tile> tiles[TB_N];
     size_t tile_off[TB_N] = {};
     std::vector<int> prefix[TB_N];     // prefix[tb][q] = tiles of internal problems [0, q)
@@ -418,6 +419,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 case TB_AV: lo = 1, hi = p.nx - 1, step = P_VROWS; break;
                 case TB_U: lo = 0, hi = p.nx, step = 1; break;
                 case TB_X8: lo = 1, hi = p.ny - 1, step = 8; break;
+                case TB_X4: lo = 1, hi = p.ny - 1, step = 4; break;
                 case TB_V16: lo = 1, hi = p.nx - 1, step = 16; break;
                 default: lo = 1, hi = p.nx - 1, step = 1; break;
                 }
@@ -490,7 +492,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     // partitioned sweeps with the chosen tile/lane mapping (xw, vr) and chunk bound (pmx, pmy)
     template <int M, int W, int PAT> void launch_x_p(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
-        const int tb = W == 16 ? TB_X : TB_X8;
+        const int tb = W == 16 ? TB_X : (W == 8 ? TB_X8 : TB_X4);
         const dim3 g((unsigned)prefix[tb][na]), b(W * P_NCH);
         const Tile *tx = (const Tile *)tiles_dev.p + tile_off[tb];
         if (stage == 1)
@@ -522,6 +524,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     {
         if (xw == 8)
             launch_x_w<8>(stage, s, st, pd, na);
+        else if (xw == 4)
+            launch_x_w<4>(stage, s, st, pd, na);
         else
             launch_x_w<16>(stage, s, st, pd, na);
     }
@@ -581,10 +585,22 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         *z = swz;
     }
 
+    void kernel_config(int32_t *cfg) const override
+    {
+        cfg[0] = line_solver;
+        cfg[1] = xw;
+        cfg[2] = vr;
+        cfg[3] = swz;
+        cfg[4] = fpat;
+        cfg[5] = pmx;
+        cfg[6] = pmy;
+        cfg[7] = tuned ? 1 : 0;
+    }
+
     void set_mapping(int x, int v, int z) override
     {
-        require((x == 16 || x == 8) && (v == 4 || v == 16) && (z == 0 || z == 1),
-                "mapping must be (16 or 8, 4 or 16, 0 or 1)");
+        require((x == 16 || x == 8 || x == 4) && (v == 4 || v == 16) && (z == 0 || z == 1),
+                "mapping must be (16, 8 or 4; 4 or 16; 0 or 1)");
         drop_graph();
         xw = x;
         vr = v;
@@ -606,7 +622,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         float best = 1e30f;
         int bx = 16, bv = 4, bs = 0;
         for (int cz : {0, 1})
-            for (int cx : {16, 8})
+            for (int cx : {16, 8, 4})
                 for (int cv : {4, 16}) {
                 xw = cx;
                 vr = cv;
@@ -1685,6 +1701,14 @@ int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v
     return guarded([&] {
         require(plan != nullptr, "null plan");
         plan->set_mapping(s_lines_per_group, v_lines_per_group, xcd_order);
+    });
+}
+
+int lss_plan_kernel_config(const lss_adi_plan *plan, int32_t *config)
+{
+    return guarded([&] {
+        require(plan && config, "null pointer");
+        plan->kernel_config(config);
     });
 }
 

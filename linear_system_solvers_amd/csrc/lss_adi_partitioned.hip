@@ -156,7 +156,9 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
     template __global__ void k_xsweep_p<T, M, 1, 16, PAT>(const Prob<T> *, const Tile *, int, int);     \
     template __global__ void k_xsweep_p<T, M, 2, 16, PAT>(const Prob<T> *, const Tile *, int, int);     \
     template __global__ void k_xsweep_p<T, M, 1, 8, PAT>(const Prob<T> *, const Tile *, int, int);      \
-    template __global__ void k_xsweep_p<T, M, 2, 8, PAT>(const Prob<T> *, const Tile *, int, int);
+    template __global__ void k_xsweep_p<T, M, 2, 8, PAT>(const Prob<T> *, const Tile *, int, int);      \
+    template __global__ void k_xsweep_p<T, M, 1, 4, PAT>(const Prob<T> *, const Tile *, int, int);      \
+    template __global__ void k_xsweep_p<T, M, 2, 4, PAT>(const Prob<T> *, const Tile *, int, int);
 #define LSS_INST_PM(T, M) LSS_INST_P(T, M, 0) LSS_INST_P(T, M, 1) LSS_INST_P(T, M, 2)
 LSS_INST_PM(double, 1)
 LSS_INST_PM(double, 2)

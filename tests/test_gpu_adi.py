@@ -224,7 +224,7 @@ def test_every_tile_mapping_gives_identical_results():
     assert chosen[0] in (16, 8) and chosen[1] in (4, 16) and chosen[2] in (0, 1)
     ref = None
     for z in (0, 1):
-        for x in (16, 8):
+        for x in (16, 8, 4):
             for v in (4, 16):
                 plan.set_mapping(x, v, z)
                 outs = plan.run_host()
@@ -234,3 +234,17 @@ def test_every_tile_mapping_gives_identical_results():
                     np.testing.assert_array_equal(a, b)
     with pytest.raises(L.LssError):
         plan.set_mapping(32, 4)
+
+
+def test_reference_models_use_the_specialized_factor_patterns():
+    """Heston (r != 0) and SABR (r = 0) separable data match the compile-time patterns exactly."""
+    h = L.Plan.from_models([_stable(40, 20, coeff_mode=L.COEFF_SEPARABLE, line_solver=L.LINE_PARTITIONED)])
+    assert h.kernel_config()["factor_pattern"] == 1
+    sabr = L.Model(kind=L.MODEL_SABR, strike=0.03, rate=0.0, sig_sig=0.5, rho=-0.3, beta=0.6, x_lo=0.0, x_hi=0.15,
+                   y_lo=0.0, y_hi=0.25, nx=40, ny=24, nt=5, t_hi=0.01, grid=L.NONUNIFORM, accumulation_point=0.03,
+                   alpha_scale=10.0, beta_scale=50.0, splitting=L.HUNDSDORFER_VERWER,
+                   coeff_mode=L.COEFF_SEPARABLE, line_solver=L.LINE_PARTITIONED)
+    s = L.Plan.from_models([sabr])
+    assert s.kernel_config()["factor_pattern"] == 2
+    mixed = L.Plan.from_models([_stable(40, 20, coeff_mode=L.COEFF_SEPARABLE, line_solver=L.LINE_PARTITIONED), sabr])
+    assert mixed.kernel_config()["factor_pattern"] == 0
