@@ -426,10 +426,13 @@ def main():
     ap.add_argument("--counters", default="",
                     help="counter-collection run: S,V,X mapping (e.g. 8,4,1) set instead of autotuned; the book is "
                          "solved --steps times with no events and no timing (hipEvents are not used under --pmc)")
+    ap.add_argument("--only", choices=("adjoint", "sabr", "tridiag"), default="",
+                    help="run only this secondary workload (no book, no JSON bench line)")
     ap.add_argument("--check-all", action="store_true",
                     help="check every book problem against the oracle on host threads (about 12 min on 16 cores)")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="first copy 256 MiB device to device three times (reference kernel for HBM counters)")
+                    help="first run reference kernels of known HBM traffic (a 256 MiB fill, a 256 MiB elementwise "
+                         "read and write) for scaling the HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; the book shards by problem with no exchange, "
@@ -442,13 +445,18 @@ def main():
     import torch
     torch.cuda.init()
     if args.calibrate_copy:
+        # reference kernels of known traffic: a fill writing 256 MiB, then an elementwise kernel
+        # reading 256 MiB and writing 256 MiB
         x = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda").uniform_()
         y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
+        torch.mul(x, 2.0, out=y)
         torch.cuda.synchronize()
-        log("calibration copy: 3 x 256 MiB device-to-device")
+        log("calibration: fill of 256 MiB, then 256 MiB read + 256 MiB written by one elementwise kernel")
         del x, y
+    if args.only:  # one secondary workload alone (for kernel traces of it)
+        fn = {"adjoint": bench_adjoint, "sabr": bench_sabr, "tridiag": bench_tridiag}[args.only]
+        print(json.dumps({"only": args.only, "result": fn(torch, log)}), flush=True)
+        return
     models = book_models(args.count)
     total_ps = point_steps(models)
     if args.counters:

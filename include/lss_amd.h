@@ -117,9 +117,12 @@ int lss_adi_autotune(lss_adi_plan *plan, void *stream, int32_t again, int32_t *s
 /* Sets the mapping instead of timing it (16, 8 or 4 S-lines, 4 or 16 v-lines per workgroup, 0 or 1). */
 int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v_lines_per_group,
                         int32_t xcd_order);
-/* Kernel configuration of a plan (8 values): line solver, S-lines and v-lines per workgroup, XCD order,
- * compile-time factor pattern of the coefficients (0 none, 1 Heston, 2 SABR with r = 0), chunk bounds
- * of S-lines and v-lines, and whether the autotuner has run. */
+/* Kernel configuration of a plan (LSS_KERNEL_CONFIG_LEN values): line solver, S-lines and v-lines per
+ * workgroup, XCD order, compile-time factor pattern of the coefficients (0 none, 1 Heston, 2 SABR with
+ * r = 0), chunk bounds of S-lines and v-lines, whether the autotuner has run, the compile-time pattern
+ * of the adjoint kernels (factor and derivative-term tables; -1 before the adjoint's first use) and
+ * the S-lines per adjoint S-sweep workgroup (0 before first use). */
+enum { LSS_KERNEL_CONFIG_LEN = 10 };
 int lss_plan_kernel_config(const lss_adi_plan *plan, int32_t *config);
 int lss_adi_run_host(lss_adi_plan *plan, const void *const *u_in, void *const *u_out);
 

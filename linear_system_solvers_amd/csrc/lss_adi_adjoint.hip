@@ -28,6 +28,8 @@
 #include "lss_adi_chunk.hpp"
 #include "lss_adi_stencil.hpp"
 
+#include <type_traits>
+
 namespace lssamd {
 
 // gradient with respect to the stencil coefficients at one point
@@ -42,43 +44,140 @@ template <typename T> __device__ __forceinline__ SG<T> sg_zero()
     return g;
 }
 
-// acc[k] += sum over terms of (d J / d q_t(i, j)) * X_t[i] * Y_t[j] for parameter k = tp[t]
-template <typename T>
-__device__ __forceinline__ void add_terms_q(const Prob<T> &p, const Adj<T> &A, int step, int i, int j, T ga, T gb,
-                                            T gc, T gd, T ge, T gf, double *acc)
+// compile-time loop: f(std::integral_constant<int, t>) for t = B .. E - 1
+template <int B, int E, typename F> __device__ __forceinline__ void static_for(F &&f)
 {
-    const int nx = p.nx, ny = p.ny;
-    const gptr<const T> fX = G(A.dfac + (A.dfac_sets > 1 ? (int64_t)step : 0) * A.dfac_stride);
-    const gptr<const T> fY = fX + (int)A.nfx * nx;
-    for (int t = 0; t < A.nterm; ++t) {
-        const int q = A.tq[t];
-        const T gq = q == 0 ? ga : (q == 1 ? gb : (q == 2 ? gc : (q == 3 ? gd : (q == 4 ? ge : gf))));
-        const T xv = A.fx[t] >= 0 ? fX[(int)A.fx[t] * nx + i] : (T)1;
-        const T yv = A.fy[t] >= 0 ? fY[(int)A.fy[t] * ny + j] : (T)1;
-        const double v = (double)gq * (double)xv * (double)yv;
-        const int k = A.tp[t];
-#pragma unroll
-        for (int z = 0; z < NPAR; ++z)
-            acc[z] += (k == z) ? v : 0.0;
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
     }
 }
 
-// M = alpha a - delta d, M~ = beta b - ni e, P = alpha a + delta d, P~ = beta b + ni e,
-// Z = 2 alpha a - rho f / 2, W = 2 beta b - rho f / 2, C = c  (coefficients.hpp:89-95)
-template <typename T>
-__device__ __forceinline__ void add_terms(const Prob<T> &p, const Adj<T> &A, int step, int i, int j, const SG<T> &g,
-                                          double *acc)
+// first term of TPat<PAT> with the same coefficient as term t and the same factor along the
+// direction that varies within a thread (FIX 1: X, FIX 2: Y); such terms share one running sum
+template <int PAT, int FIX> __device__ constexpr int term_rep(int t)
 {
-    if (A.nterm == 0)
-        return;
+    using TP = TPat<PAT>;
+    for (int u = 0; u < t; ++u)
+        if (TP::tq[u] == TP::tq[t] && (FIX == 1 ? TP::fx[u] == TP::fx[t] : TP::fy[u] == TP::fy[t]))
+            return u;
+    return t;
+}
+
+template <typename T> __device__ __forceinline__ T pick6(int q, const T (&g)[6])
+{
+    return q == 0 ? g[0] : (q == 1 ? g[1] : (q == 2 ? g[2] : (q == 3 ? g[3] : (q == 4 ? g[4] : g[5]))));
+}
+
+// Parameter-term sums of one thread.  Term t of a problem contributes g_{tq[t]}(i, j) X_t[i] Y_t[j]
+// to parameter tp[t], g_q being d J / d q at the point (q = a..f).  FIX 1 (S-line kernels: the
+// thread's column j is fixed) sums g X_t[i] over the thread's points and multiplies by Y_t[j] once
+// in finish(); FIX 2 (v-line kernels: row i fixed) the other way round; FIX 0 applies both factors
+// at each point.  PAT 1, 2: the term table is TPat<PAT>, fixed at compile time; PAT 0: the
+// problem's own (Adj::tp, tq, fx, fy).  Sums are kept in double.
+template <typename T, int PAT, int FIX> struct TermSums {
+    double s[NTERM];
+    gptr<const T> fX, fY;
+    int nx, ny, n;
+
+    __device__ __forceinline__ TermSums(const Adj<T> &A, int step, int nx_, int ny_)
+    {
+        nx = nx_;
+        ny = ny_;
+        fX = G(A.dfac + (A.dfac_sets > 1 ? (int64_t)step : 0) * A.dfac_stride);
+        fY = fX + (int)A.nfx * nx;
+        if constexpr (PAT == PAT_GENERIC)
+            n = A.nterm;
+        else
+            n = TPat<PAT>::n;
+#pragma unroll
+        for (int t = 0; t < NTERM; ++t)
+            s[t] = 0.0;
+    }
+
+    __device__ __forceinline__ double xf(int code, int i) const
+    {
+        return code >= 0 ? (double)fX[code * nx + i] : 1.0;
+    }
+    __device__ __forceinline__ double yf(int code, int j) const
+    {
+        return code >= 0 ? (double)fY[code * ny + j] : 1.0;
+    }
+
+    // one point (i, j) with d J / d (a, b, c, d, e, f) = g
+    __device__ __forceinline__ void add(const Adj<T> &A, int i, int j, const T (&g)[6])
+    {
+        static_for<0, NTERM>([&](auto tt) {
+            constexpr int t = decltype(tt)::value;
+            if constexpr (PAT == PAT_GENERIC) {
+                if (t < n) {
+                    double v = (double)pick6(A.tq[t], g);
+                    if (FIX != 1)
+                        v *= yf(A.fy[t], j);
+                    if (FIX != 2)
+                        v *= xf(A.fx[t], i);
+                    s[t] += v;
+                }
+            } else {
+                using TP = TPat<PAT>;
+                if constexpr (t < TP::n) {
+                    if constexpr (FIX == 0 || term_rep<PAT, FIX>(t) == t) {
+                        double v = (double)g[TP::tq[t]];
+                        if constexpr (FIX != 1 && TP::fy[t] >= 0)
+                            v *= (double)fY[TP::fy[t] * ny + j];
+                        if constexpr (FIX != 2 && TP::fx[t] >= 0)
+                            v *= (double)fX[TP::fx[t] * nx + i];
+                        s[t] += v;
+                    }
+                }
+            }
+        });
+    }
+
+    // acc[p] += the sums of parameter p, with the fixed factor of each term applied at (i, j)
+    __device__ __forceinline__ void finish(const Adj<T> &A, int i, int j, double *acc) const
+    {
+        static_for<0, NTERM>([&](auto tt) {
+            constexpr int t = decltype(tt)::value;
+            if constexpr (PAT == PAT_GENERIC) {
+                if (t < n) {
+                    double v = s[t];
+                    if (FIX == 1)
+                        v *= yf(A.fy[t], j);
+                    if (FIX == 2)
+                        v *= xf(A.fx[t], i);
+                    const int k = A.tp[t];
+#pragma unroll
+                    for (int z = 0; z < NPAR; ++z)
+                        acc[z] += (k == z) ? v : 0.0;
+                }
+            } else {
+                using TP = TPat<PAT>;
+                if constexpr (t < TP::n) {
+                    double v = s[FIX == 0 ? t : term_rep<PAT, FIX>(t)];
+                    if constexpr (FIX == 1 && TP::fy[t] >= 0)
+                        v *= (double)fY[TP::fy[t] * ny + j];
+                    if constexpr (FIX == 2 && TP::fx[t] >= 0)
+                        v *= (double)fX[TP::fx[t] * nx + i];
+                    acc[TP::tp[t]] += v;
+                }
+            }
+        });
+    }
+};
+
+// d J / d (a..f) from d J / d (M, M~, P, P~, Z, W, C): M = alpha a - delta d, M~ = beta b - ni e,
+// P = alpha a + delta d, P~ = beta b + ni e, Z = 2 alpha a - rho f / 2, W = 2 beta b - rho f / 2,
+// C = c  (coefficients.hpp:89-95)
+template <typename T> __device__ __forceinline__ void sg_to_q(const Prob<T> &p, const SG<T> &g, T (&q)[6])
+{
     const T two = (T)2, half = (T)0.5;
-    const T ga = p.alpha * (g.M + g.P) + two * p.alpha * g.Z;
-    const T gb = p.beta * (g.Mt + g.Pt) + two * p.beta * g.W;
-    const T gc = g.C;
-    const T gd = p.delta * (g.P - g.M);
-    const T ge = p.ni * (g.Pt - g.Mt);
-    const T gf = -half * p.rho * (g.Z + g.W);
-    add_terms_q(p, A, step, i, j, ga, gb, gc, gd, ge, gf, acc);
+    q[0] = p.alpha * (g.M + g.P) + two * p.alpha * g.Z;
+    q[1] = p.beta * (g.Mt + g.Pt) + two * p.beta * g.W;
+    q[2] = g.C;
+    q[3] = p.delta * (g.P - g.M);
+    q[4] = p.ni * (g.Pt - g.Mt);
+    q[5] = -half * p.rho * (g.Z + g.W);
 }
 
 // sum acc over the workgroup and add it to slot[0..NPAR); every thread of the block must call it
@@ -227,7 +326,7 @@ __device__ __forceinline__ T lam_y2(const Prob<T> &p, const Adj<T> &A, SF &&S, i
 // Adjoint of a v-sweep (one wavefront per row i).  last: the step's last v-stage, whose output went
 // through boundary stage B into U_{s+1}; otherwise V1 of CS/MCS/HV.
 // -------------------------------------------------------------------------------------------------
-template <typename T, int M>
+template <typename T, int M, int PAT>
 __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
                                                       const Tile *__restrict__ tiles, int step, int last,
                                                       double *__restrict__ slots)
@@ -248,9 +347,10 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
         const T *u = (step & 1) ? p.b : p.a;
         const T *un = (step & 1) ? p.a : p.b;
         const gptr<const T> lamn = G((const T *)(((step + 1) & 1) ? A.lb : A.la));
-        const gptr<const T> cs = coeff_set(p, step);
+        const GArr<T> cs{coeff_set(p, step)};
         const T one = (T)1, theta = p.theta;
         const bool hv = p.scheme == SCHEME_HV;
+        TermSums<T, PAT, 2> ts(A, step, nx, ny);
         const gptr<const T> yrow = G((last ? un : (const T *)p.y2) + (int)i * ny);
         const gptr<const T> irow = G(((last && hv) ? (const T *)p.y2 : u) + (int)i * ny);
         const gptr<T> L = G((last ? A.lv2 : A.lv1) + (int)i * ny);
@@ -270,10 +370,11 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
             return v;
         };
         auto eq = [&](int k) -> Eq<T> {
+            k = min(k, N);  // points past the line's end repeat the last equation
             Eq<T> e;
-            e.b = one + theta * stencil_at(p, cs, i, k).W;
-            e.a = (k >= 2) ? -theta * stencil_at(p, cs, i, k - 1).Pt : (T)0;
-            e.c = (k <= N - 1) ? -theta * stencil_at(p, cs, i, k + 1).Mt : (T)0;
+            e.b = one + theta * stencil_at<T, PAT>(p, cs, i, k).W;
+            e.a = (k >= 2) ? -theta * stencil_at<T, PAT>(p, cs, i, k - 1).Pt : (T)0;
+            e.c = (k <= N - 1) ? -theta * stencil_at<T, PAT>(p, cs, i, k + 1).Mt : (T)0;
             e.r = lam_out(k);
             return e;
         };
@@ -285,18 +386,24 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
             g.Mt = theta * x * (yrow[k - 1] - irow[k - 1]);
             g.W = -theta * x * (yrow[k] - irow[k]);
             g.Pt = theta * x * (yrow[k + 1] - irow[k + 1]);
-            add_terms(p, A, step, i, k, g, acc);
+            if (ts.n > 0) {
+                T gq[6];
+                sg_to_q(p, g, gq);
+                ts.add(A, i, k, gq);
+            }
         };
         wave_solve<T, M>(N, lane, eq, put);
-        const int m = (N + NCH - 1) / NCH, s0 = 1 + lane * m, cnt = max(0, min(m, N - s0 + 1));
+        int s0, cnt;
+        chunk_span(N, lane, s0, cnt);
 #pragma unroll
         for (int t = 0; t < M; ++t)
             if (t < cnt)
                 L[s0 + t] = xs[t];
         if (lane == 0) {
-            const T a1 = -theta * stencil_at(p, cs, i, 1).Mt;
+            const T a1 = -theta * stencil_at<T, PAT>(p, cs, i, 1).Mt;
             G(last ? A.elo2 : A.elo1)[i] = lam_out(0) - a1 * xs[0];
         }
+        ts.finish(A, i, 0, acc);
     }
     block_acc_store<VROWS>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
@@ -305,7 +412,7 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
 // Adjoint of an S-sweep (XTJ columns x NCH chunks per workgroup).  stage 2: X2 of CS/MCS/HV
 // (with HV's explicit stage H, whose multiplier equals X2's); stage 1: X1.
 // -------------------------------------------------------------------------------------------------
-template <typename T, int M, int W>
+template <typename T, int M, int W, int PAT>
 __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
                                                   const Tile *__restrict__ tiles, int step, int stage,
                                                   double *__restrict__ slots)
@@ -326,8 +433,9 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
     const int j = tl.first + (colok ? col : 0);
     const int nx = p.nx, ny = p.ny, N = nx - 2;
     const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
-    const gptr<const T> cs = coeff_set(p, step);
+    const GArr<T> cs{coeff_set(p, step)};
     const T one = (T)1, theta = p.theta, zeta = p.zeta, gamma = p.gamma;
+    TermSums<T, PAT, 1> ts(A, step, nx, ny);
     const int sch = p.scheme;
     const bool hv = sch == SCHEME_HV, mcs = sch == SCHEME_MCS, dr = sch == SCHEME_DR;
     // stage output: Y1, or Y3 (CS/MCS) / Y4 (HV)
@@ -335,29 +443,28 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
     const gptr<T> L = G((stage == 2) ? A.lx2 : A.lx1);
     const gptr<const T> lv2 = G((const T *)A.lv2), lv1 = G((const T *)A.lv1), lx2 = G((const T *)A.lx2);
     const gptr<const T> y1 = G((const T *)p.y1), y2 = G((const T *)p.y2);
-    auto lam_out = [&](int k) -> T {
-        const int q = (int)k * ny + j;
-        if (stage == 2)
-            return lv2[q];
-        if (dr)
-            return lv2[q];
-        if (!hv)
-            return lx2[q] + lv1[q];
-        T v = lv1[q];
-#pragma unroll
-        for (int di = -1; di <= 1; ++di) {
-            const int ei = k + di;
-            if (ei >= 1 && ei <= N)
-                v += w_h(p, stencil_at(p, cs, ei, j), -di, 0, 0) * lx2[(int)ei * ny + j];
-        }
-        return v;
-    };
+    // lambda of the stage output at (k, j).  Stage 1 of HV: Y1 is also read by the explicit stage H,
+    // whose weights on Y1 in column j (-theta P_{k-1}, 1 + theta Z_k, -theta M_{k+1}; see w_h) are
+    // the coefficients of this row of the transposed system.
     auto eq = [&](int k) -> Eq<T> {
+        k = min(k, N);  // points past the line's end repeat the last equation
+        const int q = (int)k * ny + j;
         Eq<T> e;
-        e.b = one + theta * stencil_at(p, cs, k, j).Z;
-        e.a = (k >= 2) ? -theta * stencil_at(p, cs, k - 1, j).P : (T)0;
-        e.c = (k <= N - 1) ? -theta * stencil_at(p, cs, k + 1, j).M : (T)0;
-        e.r = lam_out(k);
+        e.b = one + theta * stencil_at<T, PAT>(p, cs, k, j).Z;
+        e.a = (k >= 2) ? -theta * stencil_at<T, PAT>(p, cs, k - 1, j).P : (T)0;
+        e.c = (k <= N - 1) ? -theta * stencil_at<T, PAT>(p, cs, k + 1, j).M : (T)0;
+        if (stage == 2 || dr) {
+            e.r = lv2[q];
+        } else if (!hv) {
+            e.r = lx2[q] + lv1[q];
+        } else {
+            T v = lv1[q] + e.b * lx2[q];
+            if (k >= 2)
+                v += e.a * lx2[q - ny];
+            if (k <= N - 1)
+                v += e.c * lx2[q + ny];
+            e.r = v;
+        }
         return e;
     };
     // the solution stays in registers until every load of the parameter terms is issued
@@ -365,7 +472,7 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
     auto put = [&](int t, int k, T x) {
         const int q = (int)k * ny + j;
         xs[t] = x;
-        if (A.nterm == 0)
+        if (ts.n == 0)
             return;
         SG<T> g = sg_zero<T>();
         // implicit rows [-theta M, 1 + theta Z, -theta P] applied to the stage output
@@ -410,16 +517,20 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
                 g.C += x * zeta * gamma * cornerD;
             }
         }
-        add_terms(p, A, step, k, j, g, acc);
+        T gq[6];
+        sg_to_q(p, g, gq);
+        ts.add(A, k, j, gq);
     };
     block_solve<T, M, W>(sh, N, col, chn, colok, eq, put);
     if (colok) {
-        const int m = (N + NCH - 1) / NCH, s0 = 1 + chn * m, cnt = max(0, min(m, N - s0 + 1));
+        int s0, cnt;
+        chunk_span(N, chn, s0, cnt);
 #pragma unroll
         for (int t = 0; t < M; ++t)
             if (t < cnt)
                 L[(int)(s0 + t) * ny + j] = xs[t];
     }
+    ts.finish(A, 0, j, acc);
     block_acc_store<W * NCH / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
 
@@ -431,7 +542,7 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
 //           terms of boundary stage A (its D, E, F on the v = 0 row)
 //   WHAT 1: lambda(Y2) (CS, MCS, HV), read by the adjoint of V1
 // -------------------------------------------------------------------------------------------------
-template <typename T, int WHAT>
+template <typename T, int WHAT, int PAT>
 __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
                                                     const Tile *__restrict__ tiles, int step, double *__restrict__ slots)
 {
@@ -447,13 +558,13 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
     const bool run = step < p.steps && (WHAT == 0 || p.scheme != SCHEME_DR);  // uniform over the workgroup
     const int r0 = tl.first, c0 = tl.col;
     const int nx = p.nx, ny = p.ny, P = nx - 1;
-    const gptr<const T> cs = coeff_set(p, step);
+    const GArr<T> cs{coeff_set(p, step)};
     if (run) {
         for (int idx = threadIdx.x; idx < (GTR + 2) * (GTC + 2); idx += GTR * GTC) {
             const int lr = idx / (GTC + 2), lc = idx % (GTC + 2);
             const int ei = r0 - 1 + lr, ej = c0 - 1 + lc;
             if (inside(p, ei, ej)) {
-                const Stencil<T> st = stencil_at(p, cs, ei, ej);
+                const Stencil<T> st = stencil_at<T, PAT>(p, cs, ei, ej);
                 sst[0][lr][lc] = st.M;
                 sst[1][lr][lc] = st.Mt;
                 sst[2][lr][lc] = st.P;
@@ -533,7 +644,7 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
                     const int j = (int)inv2i[k];
                     if (j < 1 || j > ny - 2)
                         continue;
-                    const Stencil<T> st = (j >= c - 1 && j <= c + 1) ? S(ei - r, j - c) : stencil_at(p, cs, ei, j);
+                    const Stencil<T> st = (j >= c - 1 && j <= c + 1) ? S(ei - r, j - c) : stencil_at<T, PAT>(p, cs, ei, j);
                     const T w = (r == 0) ? theta * st.M : theta * st.P;  // -(folded coefficient)
                     const int q = (int)ei * ny + j;
                     T l = lx1[q];
@@ -553,7 +664,7 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
                         if (i < 1 || i > P)
                             continue;
                         T D, E, F;
-                        def_row0(p, cs, i, D, E, F);
+                        def_row0<T, PAT>(p, cs, i, D, E, F);
                         const T la = lamA(i);
                         if (di == 0)
                             v += ((T)1 - (T)3 * ni * E + rho * F) * la;
@@ -562,17 +673,20 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
                     }
                 } else if (r >= 1) {
                     T D, E, F;
-                    def_row0(p, cs, r, D, E, F);
+                    def_row0<T, PAT>(p, cs, r, D, E, F);
                     v += (c == 1 ? (T)4 * ni * E : -ni * E) * lamA(r);
                 }
                 if (c == 0 && r >= 1 && A.nterm > 0) {
+                    TermSums<T, PAT, 0> ts(A, step, nx, ny);
                     const T la = lamA(r);
                     const gptr<const T> ur = u + (int)r * ny;
                     const T gd = (r < P) ? la * delta * (u[(int)(r + 1) * ny] - u[(int)(r - 1) * ny])
                                          : -la * delta * ((T)2 * p.h1 * bs[nx + ny]);
                     const T ge = la * ni * (-(T)3 * ur[0] + (T)4 * ur[1] - ur[2]);
                     const T gf = la * rho * ur[0];
-                    add_terms_q(p, A, step, r, 0, (T)0, (T)0, (T)0, gd, ge, gf, acc);
+                    const T gq[6] = {(T)0, (T)0, (T)0, gd, ge, gf};
+                    ts.add(A, r, 0, gq);
+                    ts.finish(A, r, 0, acc);
                 }
             }
             lam[(int)r * ny + c] = v;
@@ -594,25 +708,22 @@ template <typename T> __global__ void __launch_bounds__(256) k_copy_jobs(const C
         dst[k] = src[k];
 }
 
-#define LSS_INST_A(T, M)                                                                                          \
-    template __global__ void k_adj_v<T, M>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);    \
-    template __global__ void k_adj_x<T, M, 16>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *); \
-    template __global__ void k_adj_x<T, M, 8>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
-LSS_INST_A(double, 1)
-LSS_INST_A(double, 2)
-LSS_INST_A(double, 4)
-LSS_INST_A(double, 8)
-LSS_INST_A(double, 16)
-LSS_INST_A(float, 1)
-LSS_INST_A(float, 2)
-LSS_INST_A(float, 4)
-LSS_INST_A(float, 8)
-LSS_INST_A(float, 16)
+#define LSS_INST_A(T, M, PAT)                                                                                  \
+    template __global__ void k_adj_v<T, M, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);    \
+    template __global__ void k_adj_x<T, M, 16, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *); \
+    template __global__ void k_adj_x<T, M, 8, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
+#define LSS_INST_AP(T, PAT)                                                                                    \
+    LSS_INST_A(T, 1, PAT) LSS_INST_A(T, 2, PAT) LSS_INST_A(T, 4, PAT) LSS_INST_A(T, 8, PAT) LSS_INST_A(T, 16, PAT) \
+    template __global__ void k_adj_g<T, 0, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);       \
+    template __global__ void k_adj_g<T, 1, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
+LSS_INST_AP(double, 0)
+LSS_INST_AP(double, 1)
+LSS_INST_AP(double, 2)
+LSS_INST_AP(float, 0)
+LSS_INST_AP(float, 1)
+LSS_INST_AP(float, 2)
+#undef LSS_INST_AP
 #undef LSS_INST_A
-template __global__ void k_adj_g<double, 0>(const Prob<double> *, const Adj<double> *, const Tile *, int, double *);
-template __global__ void k_adj_g<double, 1>(const Prob<double> *, const Adj<double> *, const Tile *, int, double *);
-template __global__ void k_adj_g<float, 0>(const Prob<float> *, const Adj<float> *, const Tile *, int, double *);
-template __global__ void k_adj_g<float, 1>(const Prob<float> *, const Adj<float> *, const Tile *, int, double *);
 template __global__ void k_copy_jobs<double>(const CopyJob *);
 template __global__ void k_copy_jobs<float>(const CopyJob *);
 

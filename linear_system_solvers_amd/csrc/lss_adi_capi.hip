@@ -37,11 +37,12 @@ __global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T 
 template <typename T, int M, int STAGE, int W, int PAT>
 __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int, int);
 template <typename T, int M, int VR, int PAT> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int, int);
-template <typename T, int M>
+template <typename T, int M, int PAT>
 __global__ void k_adj_v(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
-template <typename T, int M, int W>
+template <typename T, int M, int W, int PAT>
 __global__ void k_adj_x(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
-template <typename T, int WHAT> __global__ void k_adj_g(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
+template <typename T, int WHAT, int PAT>
+__global__ void k_adj_g(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
 template <typename T> __global__ void k_copy_jobs(const CopyJob *);
 template <typename T, int M>
 __global__ void k_tridiag_rows(int64_t, int, const T *, const T *, const T *, const T *, T *);
@@ -595,6 +596,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         cfg[5] = pmx;
         cfg[6] = pmy;
         cfg[7] = tuned ? 1 : 0;
+        cfg[8] = ck > 0 ? apat : -1;
+        cfg[9] = ck > 0 ? axw : 0;
     }
 
     void set_mapping(int x, int v, int z) override
@@ -728,6 +731,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     size_t nslots = 0;
     int slot_tb[5] = {TB_AV, TB_AX, TB_AV, TB_AX, TB_U};
     int axw = 16;                                 // S-lines per adjoint x-sweep workgroup
+    int apat = PAT_GENERIC;                       // compile-time factor and term tables of the adjoint
 
     void invalidate_adjoint() override
     {
@@ -913,6 +917,24 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             A.inv2o = u + o1.size() + i1.size();
             A.inv2i = u + o1.size() + i1.size() + o2.size();
         }
+        // the reference models' derivative-term tables (with their factor pattern) select the
+        // specialized adjoint kernels
+        apat = PAT_GENERIC;
+        if (fpat == PAT_HESTON || fpat == PAT_SABR0) {
+            bool all = true;
+            for (int64_t q = 0; q < n && all; ++q) {
+                const Adj<T> &A = adjs[q];
+                const int nt = fpat == PAT_HESTON ? TPat<PAT_HESTON>::n : TPat<PAT_SABR0>::n;
+                const int *tp = fpat == PAT_HESTON ? TPat<PAT_HESTON>::tp : TPat<PAT_SABR0>::tp;
+                const int *tq = fpat == PAT_HESTON ? TPat<PAT_HESTON>::tq : TPat<PAT_SABR0>::tq;
+                const int *fx = fpat == PAT_HESTON ? TPat<PAT_HESTON>::fx : TPat<PAT_SABR0>::fx;
+                const int *fy = fpat == PAT_HESTON ? TPat<PAT_HESTON>::fy : TPat<PAT_SABR0>::fy;
+                all = A.nterm == nt && std::equal(tp, tp + nt, A.tp) && std::equal(tq, tq + nt, A.tq) &&
+                      std::equal(fx, fx + nt, A.fx) && std::equal(fy, fy + nt, A.fy);
+            }
+            if (all)
+                apat = fpat;
+        }
         adj_dev.~dev_buffer();
         new (&adj_dev) dev_buffer();
         adj_dev.alloc(n * sizeof(Adj<T>));
@@ -997,19 +1019,51 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         }
     }
 
+    template <int M, int PAT>
+    void adj_v_p(int s, int last, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
+    {
+        hipLaunchKernelGGL((k_adj_v<T, M, PAT>), dim3(grid), dim3(64 * P_VROWS), 0, st, table,
+                           (const Adj<T> *)adj_dev.p, (const Tile *)tiles_dev.p + tile_off[TB_AV], s, last, slot);
+    }
     template <int M> void adj_v(int s, int last, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
     {
-        hipLaunchKernelGGL((k_adj_v<T, M>), dim3(grid), dim3(64 * P_VROWS), 0, st, table, (const Adj<T> *)adj_dev.p,
-                           (const Tile *)tiles_dev.p + tile_off[TB_AV], s, last, slot);
+        if (apat == PAT_HESTON)
+            adj_v_p<M, PAT_HESTON>(s, last, table, slot, st, grid);
+        else if (apat == PAT_SABR0)
+            adj_v_p<M, PAT_SABR0>(s, last, table, slot, st, grid);
+        else
+            adj_v_p<M, PAT_GENERIC>(s, last, table, slot, st, grid);
+    }
+    template <int M, int PAT>
+    void adj_x_p(int s, int stage, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
+    {
+        if (axw == 8)
+            hipLaunchKernelGGL((k_adj_x<T, M, 8, PAT>), dim3(grid), dim3(8 * P_NCH), 0, st, table,
+                               (const Adj<T> *)adj_dev.p, (const Tile *)tiles_dev.p + tile_off[TB_X8], s, stage, slot);
+        else
+            hipLaunchKernelGGL((k_adj_x<T, M, 16, PAT>), dim3(grid), dim3(16 * P_NCH), 0, st, table,
+                               (const Adj<T> *)adj_dev.p, (const Tile *)tiles_dev.p + tile_off[TB_AX], s, stage, slot);
     }
     template <int M> void adj_x(int s, int stage, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
     {
-        if (axw == 8)
-            hipLaunchKernelGGL((k_adj_x<T, M, 8>), dim3(grid), dim3(8 * P_NCH), 0, st, table,
-                               (const Adj<T> *)adj_dev.p, (const Tile *)tiles_dev.p + tile_off[TB_X8], s, stage, slot);
+        if (apat == PAT_HESTON)
+            adj_x_p<M, PAT_HESTON>(s, stage, table, slot, st, grid);
+        else if (apat == PAT_SABR0)
+            adj_x_p<M, PAT_SABR0>(s, stage, table, slot, st, grid);
         else
-            hipLaunchKernelGGL((k_adj_x<T, M, 16>), dim3(grid), dim3(16 * P_NCH), 0, st, table,
-                               (const Adj<T> *)adj_dev.p, (const Tile *)tiles_dev.p + tile_off[TB_AX], s, stage, slot);
+            adj_x_p<M, PAT_GENERIC>(s, stage, table, slot, st, grid);
+    }
+    template <int WHAT> void adj_g(int s, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
+    {
+        const Adj<T> *ad = (const Adj<T> *)adj_dev.p;
+        const Tile *tu = (const Tile *)tiles_dev.p + tile_off[TB_U];
+        if (apat == PAT_HESTON)
+            hipLaunchKernelGGL((k_adj_g<T, WHAT, PAT_HESTON>), dim3(grid), dim3(GTR * GTC), 0, st, table, ad, tu, s, slot);
+        else if (apat == PAT_SABR0)
+            hipLaunchKernelGGL((k_adj_g<T, WHAT, PAT_SABR0>), dim3(grid), dim3(GTR * GTC), 0, st, table, ad, tu, s, slot);
+        else
+            hipLaunchKernelGGL((k_adj_g<T, WHAT, PAT_GENERIC>), dim3(grid), dim3(GTR * GTC), 0, st, table, ad, tu, s,
+                               slot);
     }
     void adj_v_m(int s, int last, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
     {
@@ -1041,16 +1095,13 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             any_split2 |= probs[q].scheme != SCHEME_DR;
         double *sl = (double *)slots_dev.p;
         adj_v_m(s, 1, table, sl + slot_off[0] * NPAR, st, grid(TB_AV));
-        const Adj<T> *ad = (const Adj<T> *)adj_dev.p;
-        const Tile *tu = (const Tile *)tiles_dev.p + tile_off[TB_U];
         if (any_split2) {
             adj_x_m(s, 2, table, sl + slot_off[1] * NPAR, st, grid(slot_tb[1]));
-            hipLaunchKernelGGL((k_adj_g<T, 1>), dim3(grid(TB_U)), dim3(GTR * GTC), 0, st, table, ad, tu, s, sl);
+            adj_g<1>(s, table, sl, st, grid(TB_U));
             adj_v_m(s, 0, table, sl + slot_off[2] * NPAR, st, grid(TB_AV));
         }
         adj_x_m(s, 1, table, sl + slot_off[3] * NPAR, st, grid(slot_tb[3]));
-        hipLaunchKernelGGL((k_adj_g<T, 0>), dim3(grid(TB_U)), dim3(GTR * GTC), 0, st, table, ad, tu, s,
-                           sl + slot_off[4] * NPAR);
+        adj_g<0>(s, table, sl + slot_off[4] * NPAR, st, grid(TB_U));
     }
 
     void forward_ckpt(const void *const *u_in, void *const *u_out, int K, hipStream_t stream) override

@@ -14,6 +14,8 @@
 
 #include "lss_adi_device.hpp"
 
+#include <type_traits>
+
 namespace lssamd {
 
 constexpr int NCH = 64;   // chunks per line
@@ -171,135 +173,177 @@ template <typename T> __device__ __forceinline__ Eq<T> identity_eq()
 
 
 // -------------------------------------------------------------------------------------------------
-// Generic line solves.  eq(k) returns the equation of interior unknown k (1..N) with the ends
-// already folded (a = 0 at k = 1, c = 0 at k = N); put(t, k, x) receives the solution of unknown k,
-// the t-th point of the calling thread's chunk (t is a compile-time index after unrolling, so a
-// caller may keep the values in registers and store them later).
+// Generic line solves.  A line of N interior unknowns is cut into chunks of C = chunk_len(N) points
+// (the smallest power of two with NCH * C >= N, so the partition depends on N alone); the last
+// non-empty chunk may be partial and the chunks after it are empty.  Each thread with a non-empty
+// chunk evaluates eq for all C of its points without a per-point branch, so the loads of all its
+// points are issued together.
+//   eq(k) returns the equation of interior unknown k (1..N) with the ends already folded (a = 0 at
+//   k = 1, c = 0 at k = N).  It is also called with N < k <= NCH * C (points past the end of the
+//   last chunk, whose equations are discarded): it must then read only in-bounds data.
+//   put(t, k, x) receives the solution of unknown k (k <= N), the t-th point of the calling
+//   thread's chunk (t is a compile-time index after unrolling, so a caller may keep the values in
+//   registers and store them later).
 // -------------------------------------------------------------------------------------------------
+__device__ __forceinline__ int chunk_len(int N)
+{
+    int c = 1;
+    while (c * NCH < N)
+        c <<= 1;
+    return c;
+}
+
+// f(std::integral_constant<int, C>) for the chunk length c (a power of two <= M; uniform)
+template <int M, typename F> __device__ __forceinline__ void with_chunk_len(int c, F &&f)
+{
+    if constexpr (M > 1) {
+        if (c < M) {
+            with_chunk_len<M / 2>(c, f);
+            return;
+        }
+    }
+    f(std::integral_constant<int, M>{});
+}
 
 // One line per wavefront; lane = chunk; the interface system is reduced with shuffles.
-// Every lane of the wavefront must call it.
+// Every lane of the wavefront must call it; chunk_len(N) <= M.
 template <typename T, int M, typename EqF, typename PutF>
 __device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put)
 {
-    const int m = (N + NCH - 1) / NCH;
-    const int s = 1 + lane * m;
-    Chunk<T, M> ch;
-    ch.cnt = max(0, min(m, N - s + 1));
-    const bool last_real = ch.cnt > 0 && s + ch.cnt - 1 == N;
+    with_chunk_len<M>(chunk_len(N), [&](auto cc) {
+        constexpr int C = decltype(cc)::value;
+        const int s = 1 + lane * C;
+        Chunk<T, C> ch;
+        ch.cnt = max(0, min(C, N - s + 1));
+        const bool last_real = ch.cnt > 0 && s + ch.cnt - 1 == N;
+        if (ch.cnt > 0) {
 #pragma unroll
-    for (int t = 0; t < M; ++t)
-        if (t < ch.cnt)
-            ch.put(t, eq(s + t));
-    ch.bwd();
-    T PF, QF, RF;
-    ch.first_rel(PF, QF, RF);
-    T nPF = __shfl_down(PF, 1), nQF = __shfl_down(QF, 1), nRF = __shfl_down(RF, 1);
-    if (last_real || lane == NCH - 1) {
-        nPF = (T)0;
-        nQF = (T)0;
-        nRF = (T)0;
-    }
-    Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
+            for (int t = 0; t < C; ++t)
+                ch.put(t, eq(s + t));
+            ch.bwd();
+        }
+        T PF, QF, RF;
+        ch.first_rel(PF, QF, RF);
+        T nPF = __shfl_down(PF, 1), nQF = __shfl_down(QF, 1), nRF = __shfl_down(RF, 1);
+        if (last_real || lane == NCH - 1) {
+            nPF = (T)0;
+            nQF = (T)0;
+            nRF = (T)0;
+        }
+        Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
 #pragma unroll
-    for (int d = 1; d < NCH; d <<= 1) {
-        const T rb = fast_rcp(e.b);
-        Eq<T> l, h;
-        l.a = __shfl_up(e.a, d);
-        l.b = __shfl_up(rb, d);
-        l.c = __shfl_up(e.c, d);
-        l.r = __shfl_up(e.r, d);
-        h.a = __shfl_down(e.a, d);
-        h.b = __shfl_down(rb, d);
-        h.c = __shfl_down(e.c, d);
-        h.r = __shfl_down(e.r, d);
-        if (lane < d)
-            l = identity_eq<T>();
-        if (lane + d >= NCH)
-            h = identity_eq<T>();
-        e = pcr_combine(e, l, h);
-    }
-    const T xi = e.r * fast_rcp(e.b);
-    T xl = __shfl_up(xi, 1);
-    if (lane == 0)
-        xl = (T)0;
+        for (int d = 1; d < NCH; d <<= 1) {
+            const T rb = fast_rcp(e.b);
+            Eq<T> l, h;
+            l.a = __shfl_up(e.a, d);
+            l.b = __shfl_up(rb, d);
+            l.c = __shfl_up(e.c, d);
+            l.r = __shfl_up(e.r, d);
+            h.a = __shfl_down(e.a, d);
+            h.b = __shfl_down(rb, d);
+            h.c = __shfl_down(e.c, d);
+            h.r = __shfl_down(e.r, d);
+            if (lane < d)
+                l = identity_eq<T>();
+            if (lane + d >= NCH)
+                h = identity_eq<T>();
+            e = pcr_combine(e, l, h);
+        }
+        const T xi = e.r * fast_rcp(e.b);
+        T xl = __shfl_up(xi, 1);
+        if (lane == 0)
+            xl = (T)0;
 #pragma unroll
-    for (int t = 0; t < M; ++t)
-        if (t < ch.cnt)
-            put(t, s + t, ch.value(t, xl, xi));
+        for (int t = 0; t < C; ++t)
+            if (t < ch.cnt)
+                put(t, s + t, ch.value(t, xl, xi));
+    });
 }
 
 // W lines per workgroup of W * NCH threads (thread = (chunk chn, line col)); the interface
 // system is reduced through LDS `sh`.  Every thread of the workgroup must call it (it holds
-// barriers); threads of lines that do not exist pass active = false.
+// barriers) with the same N (chunk_len(N) <= M); threads of lines that do not exist pass
+// active = false (their eq is not called).
 template <typename T, int M, int W, typename EqF, typename PutF>
 __device__ __forceinline__ void block_solve(T (*sh)[4][NCH][W], int N, int col, int chn, bool active, EqF &&eq,
                                             PutF &&put)
 {
-    const int m = (N + NCH - 1) / NCH;
-    const int s = 1 + chn * m;
-    Chunk<T, M> ch;
-    ch.cnt = active ? max(0, min(m, N - s + 1)) : 0;
-    const bool last_real = ch.cnt > 0 && s + ch.cnt - 1 == N;
+    with_chunk_len<M>(chunk_len(N), [&](auto cc) {
+        constexpr int C = decltype(cc)::value;
+        const int s = 1 + chn * C;
+        Chunk<T, C> ch;
+        ch.cnt = active ? max(0, min(C, N - s + 1)) : 0;
+        const bool last_real = ch.cnt > 0 && s + ch.cnt - 1 == N;
+        if (ch.cnt > 0) {
 #pragma unroll
-    for (int t = 0; t < M; ++t)
-        if (t < ch.cnt)
-            ch.put(t, eq(s + t));
-    ch.bwd();
-    {
-        T PF, QF, RF;
-        ch.first_rel(PF, QF, RF);
-        sh[1][0][chn][col] = PF;
-        sh[1][1][chn][col] = QF;
-        sh[1][2][chn][col] = RF;
-    }
-    __syncthreads();
-    T nPF = (T)0, nQF = (T)0, nRF = (T)0;
-    if (!last_real && chn + 1 < NCH) {
-        nPF = sh[1][0][chn + 1][col];
-        nQF = sh[1][1][chn + 1][col];
-        nRF = sh[1][2][chn + 1][col];
-    }
-    Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
-    int buf = 0;
-    sh[0][0][chn][col] = e.a;
-    sh[0][1][chn][col] = fast_rcp(e.b);
-    sh[0][2][chn][col] = e.c;
-    sh[0][3][chn][col] = e.r;
-    __syncthreads();
-#pragma unroll
-    for (int d = 1; d < NCH; d <<= 1) {
-        Eq<T> l = identity_eq<T>(), h = identity_eq<T>();
-        if (chn >= d) {
-            l.a = sh[buf][0][chn - d][col];
-            l.b = sh[buf][1][chn - d][col];
-            l.c = sh[buf][2][chn - d][col];
-            l.r = sh[buf][3][chn - d][col];
+            for (int t = 0; t < C; ++t)
+                ch.put(t, eq(s + t));
+            ch.bwd();
         }
-        if (chn + d < NCH) {
-            h.a = sh[buf][0][chn + d][col];
-            h.b = sh[buf][1][chn + d][col];
-            h.c = sh[buf][2][chn + d][col];
-            h.r = sh[buf][3][chn + d][col];
+        {
+            T PF, QF, RF;
+            ch.first_rel(PF, QF, RF);
+            sh[1][0][chn][col] = PF;
+            sh[1][1][chn][col] = QF;
+            sh[1][2][chn][col] = RF;
         }
-        e = pcr_combine(e, l, h);
-        buf ^= 1;
-        sh[buf][0][chn][col] = e.a;
-        sh[buf][1][chn][col] = fast_rcp(e.b);
-        sh[buf][2][chn][col] = e.c;
-        sh[buf][3][chn][col] = e.r;
         __syncthreads();
-    }
-    const T xi = e.r * fast_rcp(e.b);
-    sh[buf ^ 1][0][chn][col] = xi;
-    __syncthreads();
-    const T xl = chn > 0 ? sh[buf ^ 1][0][chn - 1][col] : (T)0;
-    if (!active)
-        return;
+        T nPF = (T)0, nQF = (T)0, nRF = (T)0;
+        if (!last_real && chn + 1 < NCH) {
+            nPF = sh[1][0][chn + 1][col];
+            nQF = sh[1][1][chn + 1][col];
+            nRF = sh[1][2][chn + 1][col];
+        }
+        Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
+        int buf = 0;
+        sh[0][0][chn][col] = e.a;
+        sh[0][1][chn][col] = fast_rcp(e.b);
+        sh[0][2][chn][col] = e.c;
+        sh[0][3][chn][col] = e.r;
+        __syncthreads();
 #pragma unroll
-    for (int t = 0; t < M; ++t)
-        if (t < ch.cnt)
-            put(t, s + t, ch.value(t, xl, xi));
+        for (int d = 1; d < NCH; d <<= 1) {
+            Eq<T> l = identity_eq<T>(), h = identity_eq<T>();
+            if (chn >= d) {
+                l.a = sh[buf][0][chn - d][col];
+                l.b = sh[buf][1][chn - d][col];
+                l.c = sh[buf][2][chn - d][col];
+                l.r = sh[buf][3][chn - d][col];
+            }
+            if (chn + d < NCH) {
+                h.a = sh[buf][0][chn + d][col];
+                h.b = sh[buf][1][chn + d][col];
+                h.c = sh[buf][2][chn + d][col];
+                h.r = sh[buf][3][chn + d][col];
+            }
+            e = pcr_combine(e, l, h);
+            buf ^= 1;
+            sh[buf][0][chn][col] = e.a;
+            sh[buf][1][chn][col] = fast_rcp(e.b);
+            sh[buf][2][chn][col] = e.c;
+            sh[buf][3][chn][col] = e.r;
+            __syncthreads();
+        }
+        const T xi = e.r * fast_rcp(e.b);
+        sh[buf ^ 1][0][chn][col] = xi;
+        __syncthreads();
+        const T xl = chn > 0 ? sh[buf ^ 1][0][chn - 1][col] : (T)0;
+        if (!active)
+            return;
+#pragma unroll
+        for (int t = 0; t < C; ++t)
+            if (t < ch.cnt)
+                put(t, s + t, ch.value(t, xl, xi));
+    });
+}
+
+// First point and point count of the chunk of thread position `pos` (lane or chunk index) of a line
+// of N interior unknowns, as the solvers above partition it.
+__device__ __forceinline__ void chunk_span(int N, int pos, int &s, int &cnt)
+{
+    const int c = chunk_len(N);
+    s = 1 + pos * c;
+    cnt = max(0, min(c, N - s + 1));
 }
 
 }  // namespace lssamd

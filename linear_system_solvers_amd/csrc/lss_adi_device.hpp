@@ -56,6 +56,15 @@ template <typename T> struct GAt {
     uint32_t q;
     __device__ __forceinline__ T operator[](int d) const { return ldg(b, q + (uint32_t)d); }
 };
+// view at element k of a row starting at `row`, every access clamped to element kmax of the row
+// (for points past a line's end, whose values are discarded); within the line it equals GAt
+template <typename T> struct GAtC {
+    gptr<const T> b;
+    uint32_t row;
+    int k;
+    uint32_t kmax;
+    __device__ __forceinline__ T operator[](int d) const { return ldg(b, row + min((uint32_t)(k + d), kmax)); }
+};
 
 template <typename T> struct Prob {
     int32_t nx, ny, steps, scheme;

@@ -150,6 +150,7 @@ __global__ void __launch_bounds__(TILE) k_xsweep(const Prob<T> *__restrict__ pro
     const gptr<T> scr = G(p.scr);
     const uint32_t jj = p.idx2[j];
     const T lo = u[jj], hi = u[(int64_t)(nx - 1) * ny + jj];
+    auto at = [&](gptr<const T> b, int64_t q) { return PAt<gptr<const T>>{b + q, ny}; };
     T gprev = (T)0, sprev = (T)0;
     for (int t = 1; t <= end; ++t) {
         const int64_t q = (int64_t)t * ny + j;
@@ -157,13 +158,13 @@ __global__ void __launch_bounds__(TILE) k_xsweep(const Prob<T> *__restrict__ pro
         const T low = (-theta * s.M), diag = (one + theta * s.Z), high = (-theta * s.P);
         T r;
         if (kind == 0)
-            r = rhs_intermed_1(p, s, in + q, ny);
+            r = rhs_intermed_1(p, s, at(in, q));
         else if (kind == 1)
-            r = rhs_intermed_3<T, false>(p, s, in + q, inh + q, inhn + q, ny);
+            r = rhs_intermed_3<T, false>(p, s, at(in, q), at(inh, q), at(inhn, q));
         else if (kind == 2)
-            r = rhs_intermed_3<T, true>(p, s, in + q, inh + q, inhn + q, ny);
+            r = rhs_intermed_3<T, true>(p, s, at(in, q), at(inh, q), at(inhn, q));
         else
-            r = rhs_intermed_4(p, s, in + q, inh + q, ny);
+            r = rhs_intermed_4(p, s, at(in, q), at(inh, q));
         T g, sv;
         fwd_step(tri, t == 1, t == end, low, diag, high, r, lo, hi, gprev, sprev, g, sv);
         scr[q] = g;
@@ -267,7 +268,8 @@ __global__ void __launch_bounds__(TILE) k_hv_y3(const Prob<T> *__restrict__ prob
     for (int c = 1 + threadIdx.x; c < ny - 1; c += blockDim.x) {
         const int64_t q = (int64_t)r * ny + c;
         const Stencil<T> s = stencil_at(p, cs, r, c);
-        const T val = hv_y3_at(p, s, u + q, y1 + q, y2 + q, ny);
+        const PAt<gptr<const T>> in{u + q, ny}, inh{y1 + q, ny}, inhn{y2 + q, ny};
+        const T val = hv_y3_at(p, s, in, inh, inhn);
         y3[q] = val;
     }
 }

@@ -9,7 +9,8 @@
 //                   The HV stage-2 sweep evaluates the explicit Y3 stage pointwise inside its
 //                   right-hand side, so Y3 is never stored.
 // W and VR are the tile/lane mappings the plan's autotuner chooses between; they do not change the
-// arithmetic of a line.
+// arithmetic of a line.  (One wavefront per S-line, as in the v-sweep, spreads each access over 64
+// rows; measured on the Heston book it ran the S-sweeps at about half the speed of W = 8.)
 //
 // The coefficients and right-hand sides of each point follow the reference's formulas (multiply-adds
 // may be fused here); the elimination order of the line solve differs from the reference's
@@ -62,13 +63,14 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
     const gptr<T> out = G(out0);
     const uint32_t ii = p.idx1[i];
     const T lo = G(p.colA0)[ii], hi = G(p.colA1)[ii];
+    const uint32_t jmax = (uint32_t)(N + 1);  // points past the line's end read column N + 1
     auto eq = [&](int jj) -> Eq<T> {
-        const Stencil<T> st = stencil_at<T, PAT>(p, cs, i, jj);
+        const Stencil<T> st = stencil_at<T, PAT>(p, cs, i, (int)min((uint32_t)jj, jmax));
         Eq<T> q;
         q.a = (-theta * st.Mt);
         q.b = (one + theta * st.W);
         q.c = (-theta * st.Pt);
-        q.r = rhs_v(p, st, GAt<T>{in, row + jj}, GAt<T>{inh, row + jj});
+        q.r = rhs_v(p, st, GAtC<T>{in, row, jj, jmax}, GAtC<T>{inh, row, jj, jmax});
         if (jj == 1) {  // fold the Dirichlet ends into the first / last equation
             q.r = q.r - q.a * lo;
             q.a = (T)0;
@@ -92,45 +94,52 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
 // HV rhs_intermed_4(Y2, Y3) with Y3 evaluated in place from (u, Y1, Y2) -> Y4 (= y4).
 // Ends u(0, idx2[j]), u(nx-1, idx2[j]) (implicit_kernel.hpp:86-101).
 // -------------------------------------------------------------------------------------------------
-template <typename T, int M, int STAGE, int W, int PAT>
-__global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__ probs,
-                                                     const Tile *__restrict__ tiles, int step, int swz)
-{
-    __shared__ T sh[2][4][NCH][W];
-    const Tile tl = tiles[swz ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
-    const Prob<T> &p = probs[tl.prob];
-    if (step >= p.steps || (STAGE == 2 && p.scheme == SCHEME_DR))
-        return;  // uniform over the workgroup
-    const int col = threadIdx.x % W, chn = threadIdx.x / W;
-    const bool colok = col < tl.count;
-    const int j = tl.first + (colok ? col : 0);
-    const int nx = p.nx, ny = p.ny, N = nx - 2;
-    const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
-    const GArr<T> cs{coeff_set(p, step)};
-    const gptr<const T> y1 = G((const T *)p.y1), y2 = G((const T *)p.y2);
-    const T one = (T)1.0, theta = p.theta;
-    const int kind = (STAGE == 1) ? 0 : (p.scheme == SCHEME_CS ? 1 : (p.scheme == SCHEME_MCS ? 2 : 3));
-    const gptr<T> out = G((STAGE == 1) ? p.y1 : (kind == 3 ? p.y4 : p.y3));
-    const uint32_t jj = p.idx2[j];
-    const T lo = ldg(u, jj), hi = ldg(u, (uint32_t)(nx - 1) * (uint32_t)ny + jj);
-    auto eq = [&](int k) -> Eq<T> {
-        const uint32_t q = (uint32_t)k * (uint32_t)ny + (uint32_t)j;
-        const Stencil<T> st = stencil_at<T, PAT>(p, cs, k, j);
+// KIND 0: stage 1 (every scheme); 1: CS, 2: MCS, 3: HV stage 2 (compile time, so that the loads of
+// neighbouring points are shared and issued together)
+template <typename T, int KIND, int PAT> struct XLine {
+    const Prob<T> &p;
+    gptr<const T> u, y1, y2;
+    GArr<T> cs;
+    gptr<T> out;
+    int N, ny;
+    uint32_t j;
+    T lo, hi;
+
+    __device__ __forceinline__ XLine(const Prob<T> &p_, int step, int j_) : p(p_)
+    {
+        ny = p.ny;
+        N = p.nx - 2;
+        j = (uint32_t)j_;
+        u = G((const T *)((step & 1) ? p.b : p.a));
+        cs = GArr<T>{coeff_set(p, step)};
+        y1 = G((const T *)p.y1);
+        y2 = G((const T *)p.y2);
+        out = G(KIND == 0 ? p.y1 : (KIND == 3 ? p.y4 : p.y3));
+        const uint32_t jj = p.idx2[j];
+        lo = ldg(u, jj);
+        hi = ldg(u, (uint32_t)(p.nx - 1) * (uint32_t)ny + jj);
+    }
+
+    __device__ __forceinline__ Eq<T> operator()(int k) const
+    {
+        const T one = (T)1.0, theta = p.theta;
+        const uint32_t kmax = (uint32_t)(N + 1);  // points past the line's end read row N + 1
+        const Stencil<T> st = stencil_at<T, PAT>(p, cs, (int)min((uint32_t)k, kmax), (int)j);
         Eq<T> e;
         e.a = (-theta * st.M);
         e.b = (one + theta * st.Z);
         e.c = (-theta * st.P);
-        const GAt<T> uq{u, q};
-        if (STAGE == 1) {
-            e.r = rhs_intermed_1(p, st, uq, ny);
-        } else if (kind == 1) {
-            e.r = rhs_intermed_3<T, false>(p, st, uq, GAt<T>{y1, q}, GAt<T>{y2, q}, ny);
-        } else if (kind == 2) {
-            e.r = rhs_intermed_3<T, true>(p, st, uq, GAt<T>{y1, q}, GAt<T>{y2, q}, ny);
+        const GRC<T> uq{u, k, j, (uint32_t)ny, kmax}, y1q{y1, k, j, (uint32_t)ny, kmax},
+            y2q{y2, k, j, (uint32_t)ny, kmax};
+        if constexpr (KIND == 0) {
+            e.r = rhs_intermed_1(p, st, uq);
+        } else if constexpr (KIND == 1) {
+            e.r = rhs_intermed_3<T, false>(p, st, uq, y1q, y2q);
+        } else if constexpr (KIND == 2) {
+            e.r = rhs_intermed_3<T, true>(p, st, uq, y1q, y2q);
         } else {
-            const GAt<T> y2q{y2, q};
-            const T y3 = hv_y3_at(p, st, uq, GAt<T>{y1, q}, y2q, ny);
-            e.r = (-theta * st.M * y2q[-ny]) + (theta * st.Z * y2q[0]) - (theta * st.P * y2q[ny]) + y3;
+            const T y3 = hv_y3_at(p, st, uq, y1q, y2q);
+            e.r = (-theta * st.M * y2q(-1, 0)) + (theta * st.Z * y2q(0, 0)) - (theta * st.P * y2q(1, 0)) + y3;
         }
         if (k == 1) {
             e.r = e.r - e.a * lo;
@@ -141,13 +150,46 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
             e.c = (T)0;
         }
         return e;
-    };
-    auto put = [&](int, int k, T x) { stg(out, (uint32_t)k * (uint32_t)ny + (uint32_t)j, x); };
-    block_solve<T, M, W>(sh, N, col, chn, colok, eq, put);
-    if (colok && chn == 0) {
-        stg(out, (uint32_t)j, lo);
-        stg(out, (uint32_t)(nx - 1) * (uint32_t)ny + (uint32_t)j, hi);
     }
+
+    __device__ __forceinline__ void store(int k, T x) const { stg(out, (uint32_t)k * (uint32_t)ny + j, x); }
+    __device__ __forceinline__ void store_ends() const
+    {
+        stg(out, j, lo);
+        stg(out, (uint32_t)(N + 1) * (uint32_t)ny + j, hi);
+    }
+};
+
+template <typename T, int M, int W, int KIND, int PAT>
+__device__ __forceinline__ void xsweep_lines(T (*sh)[4][NCH][W], const Prob<T> &p, const Tile &tl, int step)
+{
+    const int col = threadIdx.x % W, chn = threadIdx.x / W;
+    const bool colok = col < tl.count;
+    const XLine<T, KIND, PAT> ln(p, step, tl.first + (colok ? col : 0));
+    auto put = [&](int, int k, T x) { ln.store(k, x); };
+    block_solve<T, M, W>(sh, ln.N, col, chn, colok, ln, put);
+    if (colok && chn == 0)
+        ln.store_ends();
+}
+
+// W S-lines per workgroup, their chunks spread over the workgroup (block_solve)
+template <typename T, int M, int STAGE, int W, int PAT>
+__global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__ probs,
+                                                     const Tile *__restrict__ tiles, int step, int swz)
+{
+    __shared__ T sh[2][4][NCH][W];
+    const Tile tl = tiles[swz ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    if (step >= p.steps || (STAGE == 2 && p.scheme == SCHEME_DR))
+        return;  // uniform over the workgroup
+    if constexpr (STAGE == 1)
+        xsweep_lines<T, M, W, 0, PAT>(sh, p, tl, step);
+    else if (p.scheme == SCHEME_CS)
+        xsweep_lines<T, M, W, 1, PAT>(sh, p, tl, step);
+    else if (p.scheme == SCHEME_MCS)
+        xsweep_lines<T, M, W, 2, PAT>(sh, p, tl, step);
+    else
+        xsweep_lines<T, M, W, 3, PAT>(sh, p, tl, step);
 }
 
 #define LSS_INST_P(T, M, PAT)                                                                          \

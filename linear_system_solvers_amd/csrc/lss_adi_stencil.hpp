@@ -26,6 +26,27 @@ template <> struct FPat<PAT_SABR0> {
                                   0, -1, 1, -1, 2, -1, -2, 3, 4, -1, -1, -1};
 };
 
+// Derivative-term tables of the reference models' parameters in the packed form the plan stores
+// (Adj::tp, tq, fx, fy; fx/fy: -2 one, k >= 0 packed factor vector k): Heston (kappa, theta_v,
+// sigma, rho, r) and SABR (nu, rho, beta), in the order the plan generates them.  The adjoint
+// kernels use them at compile time when every problem of the plan has exactly these tables and the
+// same factor pattern.
+template <int PAT> struct TPat;
+template <> struct TPat<PAT_HESTON> {
+    static constexpr int n = 8;
+    static constexpr int tp[8] = {0, 1, 2, 2, 2, 3, 4, 4};
+    static constexpr int tq[8] = {4, 4, 1, 2, 4, 2, 3, 5};
+    static constexpr int fx[8] = {-2, -2, -2, 0, -2, 0, 0, -2};
+    static constexpr int fy[8] = {0, 1, 2, 3, 4, 5, -2, 6};
+};
+template <> struct TPat<PAT_SABR0> {
+    static constexpr int n = 7;
+    static constexpr int tp[7] = {0, 0, 0, 1, 2, 2, 2};
+    static constexpr int tq[7] = {1, 2, 4, 2, 0, 2, 3};
+    static constexpr int fx[7] = {-2, 0, -2, 0, 1, 2, 3};
+    static constexpr int fy[7] = {0, 1, 2, 3, 4, 5, 4};
+};
+
 // factor slot SLOT of a packed SEPARABLE block B (length-n vectors) at index i
 template <typename T, int PAT, int SLOT, typename P>
 __device__ __forceinline__ T factor(const Prob<T> &p, P B, int n, int i)
@@ -82,19 +103,19 @@ __device__ __forceinline__ Stencil<T> stencil_at(const Prob<T> &p, P cs, int i, 
 }
 
 // D, E, F on the v = 0 row at zeta_i (used by boundary stage A).
-template <typename T, typename P>
+template <typename T, int PAT = PAT_GENERIC, typename P>
 __device__ __forceinline__ void def_row0(const Prob<T> &p, P cs, int i, T &D, T &E, T &F)
 {
-    if (p.coeff_mode == COEFF_FIELDS) {
+    if (PAT == PAT_GENERIC && p.coeff_mode == COEFF_FIELDS) {
         const P r = cs + 7 * (int)p.nx * p.ny;
         D = r[i];
         E = r[p.nx + i];
         F = r[2 * p.nx + i];
     } else {
         const P X = cs, Y = cs + 12 * (int)p.nx;
-        D = sep<T, PAT_GENERIC, 3>(p, X, Y, i, 0);
-        E = sep<T, PAT_GENERIC, 4>(p, X, Y, i, 0);
-        F = sep<T, PAT_GENERIC, 5>(p, X, Y, i, 0);
+        D = sep<T, PAT, 3>(p, X, Y, i, 0);
+        E = sep<T, PAT, 4>(p, X, Y, i, 0);
+        F = sep<T, PAT, 5>(p, X, Y, i, 0);
     }
 }
 
@@ -107,43 +128,62 @@ template <typename T> __device__ __forceinline__ gptr<const T> bnd_set(const Pro
     return G(p.bnd + (p.bnd_sets > 1 ? (int64_t)step : 0) * p.bnd_stride);
 }
 
+// Two-dimensional point accessors: in(di, dj) is the value at (i + di, j + dj) of the point (i, j).
+// PAt: a pointer at the point and the row stride.  GRC: a global base, the point's row k and column
+// j; every access forms the offset of its row, min(k + di, kmax) * ny, so that neighbouring points
+// of one line read their common rows through the same addresses (one load per value).
+template <typename P> struct PAt {
+    P b;
+    int64_t ny;
+    __device__ __forceinline__ auto operator()(int di, int dj) const { return b[di * ny + dj]; }
+};
+template <typename T> struct GRC {
+    gptr<const T> b;
+    int k;
+    uint32_t j, ny, kmax;  // rows are clamped to kmax (for points past a line's end)
+    __device__ __forceinline__ T operator()(int di, int dj) const
+    {
+        return ldg(b, min((uint32_t)(k + di), kmax) * ny + (j + (uint32_t)dj));
+    }
+};
+
 // -------------------------------------------------------------------------------------------------
 // Right-hand sides (operand order of the reference's rhs functions)
 // -------------------------------------------------------------------------------------------------
-// rhs_intermed_1 (lss_heat_douglas_rachford_method.hpp:37-68); `in` points at (t, j), row stride ny
+// rhs_intermed_1 (lss_heat_douglas_rachford_method.hpp:37-68); in(di, dj) is the value at (t + di, j + dj)
 template <typename T, typename P>
-__device__ __forceinline__ T rhs_intermed_1(const Prob<T> &p, const Stencil<T> &s, P in, int ny)
+__device__ __forceinline__ T rhs_intermed_1(const Prob<T> &p, const Stencil<T> &s, P in)
 {
     const T one = (T)1.0, gamma = p.gamma, theta = p.theta;
-    return (gamma * s.C * in[-ny - 1]) + ((one - theta) * s.M * in[-ny]) - (gamma * s.C * in[-ny + 1]) +
-           (s.Mt * in[-1]) + ((one - s.W - (one - theta) * s.Z) * in[0]) + (s.Pt * in[1]) -
-           (gamma * s.C * in[ny - 1]) + ((one - theta) * s.P * in[ny]) + (gamma * s.C * in[ny + 1]);
+    return (gamma * s.C * in(-1, -1)) + ((one - theta) * s.M * in(-1, 0)) - (gamma * s.C * in(-1, 1)) +
+           (s.Mt * in(0, -1)) + ((one - s.W - (one - theta) * s.Z) * in(0, 0)) + (s.Pt * in(0, 1)) -
+           (gamma * s.C * in(1, -1)) + ((one - theta) * s.P * in(1, 0)) + (gamma * s.C * in(1, 1));
 }
 
 // CS rhs_intermed_3 (lss_heat_craig_sneyd_method.hpp:128-156), MCS adds the (zeta - theta) term
 // of lss_heat_modified_craig_sneyd_method.hpp:158-163 (with its "- P")
 template <typename T, bool MCS, typename P>
-__device__ __forceinline__ T rhs_intermed_3(const Prob<T> &p, const Stencil<T> &s, P in, P inh, P inhn, int ny)
+__device__ __forceinline__ T rhs_intermed_3(const Prob<T> &p, const Stencil<T> &s, P in, P inh, P inhn)
 {
     const T gamma = p.gamma, theta = p.theta, zeta = p.zeta;
-    T v = (-theta * s.M * in[-ny]) + (theta * s.Z * in[0]) - (theta * s.P * in[ny]) + (inh[0]) +
+    T v = (-theta * s.M * in(-1, 0)) + (theta * s.Z * in(0, 0)) - (theta * s.P * in(1, 0)) + (inh(0, 0)) +
           (zeta * gamma * s.C *
-           ((inhn[ny + 1] - in[ny + 1]) - (inhn[ny - 1] - in[ny - 1]) - (inhn[-ny + 1] - in[-ny + 1]) +
-            (inhn[-ny - 1] - in[-ny - 1])));
+           ((inhn(1, 1) - in(1, 1)) - (inhn(1, -1) - in(1, -1)) - (inhn(-1, 1) - in(-1, 1)) +
+            (inhn(-1, -1) - in(-1, -1))));
     if (MCS) {
-        v = v + ((zeta - theta) * (s.M * (inhn[-ny] - in[-ny]) - (s.Z + s.W) * (inhn[0] - in[0]) -
-                                   s.P * (inhn[ny] - in[ny]) + s.Mt * (inhn[-1] - in[-1]) +
-                                   s.Pt * (inhn[1] - in[1])));
+        v = v + ((zeta - theta) * (s.M * (inhn(-1, 0) - in(-1, 0)) - (s.Z + s.W) * (inhn(0, 0) - in(0, 0)) -
+                                   s.P * (inhn(1, 0) - in(1, 0)) + s.Mt * (inhn(0, -1) - in(0, -1)) +
+                                   s.Pt * (inhn(0, 1) - in(0, 1))));
     }
     return v;
 }
 
 // HV rhs_intermed_4 (lss_heat_hundsdorfer_verwer_method.hpp:174-194)
 template <typename T, typename P>
-__device__ __forceinline__ T rhs_intermed_4(const Prob<T> &p, const Stencil<T> &s, P in, P inh, int ny)
+__device__ __forceinline__ T rhs_intermed_4(const Prob<T> &p, const Stencil<T> &s, P in, P inh)
 {
     const T theta = p.theta;
-    return (-theta * s.M * in[-ny]) + (theta * s.Z * in[0]) - (theta * s.P * in[ny]) + inh[0];
+    return (-theta * s.M * in(-1, 0)) + (theta * s.Z * in(0, 0)) - (theta * s.P * in(1, 0)) + inh(0, 0);
 }
 
 // v-direction rhs shared by every scheme (e.g. lss_heat_douglas_rachford_method.hpp:106-126)
@@ -155,18 +195,19 @@ __device__ __forceinline__ T rhs_v(const Prob<T> &p, const Stencil<T> &s, P in, 
 }
 
 // HV explicit stage value Y3 at a point (lss_heat_hundsdorfer_verwer_method.hpp:146-168);
-// in = u, inh = Y1, inhn = Y2, each pointing at the point.
+// in = u, inh = Y1, inhn = Y2, each an accessor at the point.
 template <typename T, typename P>
-__device__ __forceinline__ T hv_y3_at(const Prob<T> &p, const Stencil<T> &s, P in, P inh, P inhn, int ny)
+__device__ __forceinline__ T hv_y3_at(const Prob<T> &p, const Stencil<T> &s, P in, P inh, P inhn)
 {
     const T one = (T)1.0, gamma = p.gamma, theta = p.theta, zeta = p.zeta;
-    return (-theta * s.M * inh[-ny]) + ((one + theta * s.Z) * inh[0]) - (theta * s.P * inh[ny]) +
-           (theta * s.M * in[-ny]) - (theta * s.Z * in[0]) + (theta * s.P * in[ny]) +
+    return (-theta * s.M * inh(-1, 0)) + ((one + theta * s.Z) * inh(0, 0)) - (theta * s.P * inh(1, 0)) +
+           (theta * s.M * in(-1, 0)) - (theta * s.Z * in(0, 0)) + (theta * s.P * in(1, 0)) +
            (zeta * gamma * s.C *
-            ((inhn[ny + 1] - in[ny + 1]) - (inhn[ny - 1] - in[ny - 1]) - (inhn[-ny + 1] - in[-ny + 1]) +
-             (inhn[-ny - 1] - in[-ny - 1]))) +
-           (zeta * (s.M * (inhn[-ny] - in[-ny]) - (s.W + s.Z) * (inhn[0] - in[0]) + s.P * (inhn[ny] - in[ny]) +
-                    s.Mt * (inhn[-1] - in[-1]) + s.Pt * (inhn[1] - in[1])));
+            ((inhn(1, 1) - in(1, 1)) - (inhn(1, -1) - in(1, -1)) - (inhn(-1, 1) - in(-1, 1)) +
+             (inhn(-1, -1) - in(-1, -1)))) +
+           (zeta * (s.M * (inhn(-1, 0) - in(-1, 0)) - (s.W + s.Z) * (inhn(0, 0) - in(0, 0)) +
+                    s.P * (inhn(1, 0) - in(1, 0)) + s.Mt * (inhn(0, -1) - in(0, -1)) +
+                    s.Pt * (inhn(0, 1) - in(0, 1))));
 }
 
 }  // namespace lssamd

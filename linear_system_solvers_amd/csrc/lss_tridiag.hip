@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(64 * TRI_WAVES) k_tridiag_rows(int64_t B, int 
     __syncthreads();
     if (live) {
         auto eq = [&](int kk) -> Eq<T> {
-            const int i = kk - 1;
+            const int i = min(kk, n) - 1;  // points past the end repeat the last equation
             Eq<T> e;
             e.a = i == 0 ? (T)0 : sa[i];
             e.b = sb[i];
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(XTJ *NCH) k_tridiag_cols(int64_t B, int n, con
     const int64_t k = (int64_t)blockIdx.x * XTJ + col;
     const bool live = k < B;
     auto eq = [&](int kk) -> Eq<T> {
-        const int i = kk - 1;
+        const int i = min(kk, n) - 1;  // points past the end repeat the last equation
         const int64_t q = (int64_t)i * B + k;
         Eq<T> e;
         e.a = i == 0 ? (T)0 : a[q];

@@ -186,3 +186,28 @@ def test_backward_without_forward_is_an_error():
     lo = _dev(np.zeros((34, 20)))
     with pytest.raises(L.LssError):
         plan.backward([lo.data_ptr()])
+
+
+@pytest.mark.parametrize("model", ["heston", "sabr"])
+def test_specialized_adjoint_kernels_match_the_generic_ones(model):
+    """Separable Heston / SABR plans run the adjoint kernels with compile-time factor and derivative-term
+    tables; the same problem with FIELDS coefficients runs the generic kernels (tables read at run time).
+    Both must give the same gradients, up to the rounding of the two coefficient forms."""
+    if model == "heston":
+        mk, x0, y0, want = (lambda **o: _heston(2, nx=40, ny=24, steps=12, **o)), 100.0, 0.04, 1
+    else:
+        mk, x0, y0, want = (lambda **o: _sabr(**o)), 0.03, 0.05, 2
+    res = {}
+    for mode in (L.COEFF_SEPARABLE, L.COEFF_FIELDS):
+        plan = L.Plan.from_models([mk(coeff_mode=mode)])
+        assert plan.kernel_config()["adjoint_pattern"] == -1  # not set up before the first forward()
+        w = _price_weights(plan, 0, x0, y0)
+        _, g, dp = _grad(plan, [plan.initial(0)], [w])
+        res[mode] = (plan.kernel_config()["adjoint_pattern"], g[0], dp[0])
+    assert res[L.COEFF_SEPARABLE][0] == want and res[L.COEFF_FIELDS][0] == 0
+    gs, gf = res[L.COEFF_SEPARABLE][1], res[L.COEFF_FIELDS][1]
+    ds, df = res[L.COEFF_SEPARABLE][2], res[L.COEFF_FIELDS][2]
+    print(f"{model}: surface gradient diff {np.abs(gs - gf).max() / np.abs(gf).max():.2e}, "
+          f"parameters {ds[:5]} vs {df[:5]}", flush=True)
+    assert np.abs(gs - gf).max() <= 1e-9 * np.abs(gf).max()
+    np.testing.assert_allclose(ds, df, rtol=1e-8, atol=1e-12 * np.abs(df).max())
