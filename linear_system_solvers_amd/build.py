@@ -14,7 +14,10 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "liblss_amd.so")
-SOURCES = ["lss_adi_kernels.hip", "lss_adi_partitioned.hip", "lss_adi_adjoint.hip", "lss_tridiag.hip", "lss_adi_capi.hip"]
+# the heaviest translation units first, so that a parallel build ends soonest
+SOURCES = ([f"lss_adi_partitioned_{t}_{p}.hip" for t in ("f64", "f32") for p in ("generic", "heston", "sabr")] +
+           [f"lss_adi_adjoint_{t}_{p}.hip" for t in ("f64", "f32") for p in ("generic", "heston", "sabr")] +
+           ["lss_adi_kernels.hip", "lss_tridiag.hip", "lss_adi_capi.hip"])
 ARCH = os.environ.get("LSS_OFFLOAD_ARCH", "gfx950")
 
 
@@ -41,7 +44,7 @@ def build(verbose: bool = False) -> str:
     if verbose:
         for cmd in cmds:
             print(" ".join(cmd), flush=True)
-    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "4")), os.cpu_count() or 1))
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "8")), os.cpu_count() or 1))
     with cf.ThreadPoolExecutor(jobs) as ex:
         for r in ex.map(lambda c: subprocess.run(c), cmds):
             if r.returncode != 0:

@@ -1,4 +1,4 @@
-// lss_adi_adjoint.hip -- reverse mode of the ADI time step for gfx950 (hand-written adjoint).
+// lss_adi_adjoint.hpp -- reverse mode of the ADI time step for gfx950 (hand-written adjoint).
 //
 // One forward step maps the level U_s to U_{s+1} through the stages of its scheme (boundary stage A,
 // S-sweep X1, v-sweep V1, [HV explicit H], S-sweep X2, v-sweep V2, boundary stage B; DR stops after
@@ -24,6 +24,7 @@
 
 // The fast path may fuse multiply-adds (the library is built with -ffp-contract=off so that the exact
 // line solver keeps the reference's rounding; this translation unit opts back in).
+#pragma once
 #pragma clang fp contract(fast)
 #include "lss_adi_chunk.hpp"
 #include "lss_adi_stencil.hpp"
@@ -708,23 +709,16 @@ template <typename T> __global__ void __launch_bounds__(256) k_copy_jobs(const C
         dst[k] = src[k];
 }
 
+}  // namespace lssamd
+
+// Instantiations (one translation unit per precision and pattern): LSS_INST_ADJ(T, PAT).
 #define LSS_INST_A(T, M, PAT)                                                                                  \
     template __global__ void k_adj_v<T, M, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);    \
     template __global__ void k_adj_x<T, M, 16, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *); \
     template __global__ void k_adj_x<T, M, 8, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
-#define LSS_INST_AP(T, PAT)                                                                                    \
+#define LSS_INST_ADJ(T, PAT)                                                                                   \
+    namespace lssamd {                                                                                         \
     LSS_INST_A(T, 1, PAT) LSS_INST_A(T, 2, PAT) LSS_INST_A(T, 4, PAT) LSS_INST_A(T, 8, PAT) LSS_INST_A(T, 16, PAT) \
     template __global__ void k_adj_g<T, 0, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);       \
-    template __global__ void k_adj_g<T, 1, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
-LSS_INST_AP(double, 0)
-LSS_INST_AP(double, 1)
-LSS_INST_AP(double, 2)
-LSS_INST_AP(float, 0)
-LSS_INST_AP(float, 1)
-LSS_INST_AP(float, 2)
-#undef LSS_INST_AP
-#undef LSS_INST_A
-template __global__ void k_copy_jobs<double>(const CopyJob *);
-template __global__ void k_copy_jobs<float>(const CopyJob *);
-
-}  // namespace lssamd
+    template __global__ void k_adj_g<T, 1, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);       \
+    }

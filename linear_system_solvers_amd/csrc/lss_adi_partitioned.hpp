@@ -1,4 +1,4 @@
-// lss_adi_partitioned.hip -- bandwidth-oriented line sweeps for gfx950: partitioned tridiagonal
+// lss_adi_partitioned.hpp -- bandwidth-oriented line sweeps for gfx950: partitioned tridiagonal
 // solves (lss_adi_chunk.hpp) whose intermediates stay in registers and LDS.  Each input is read
 // once and each output written once.
 //
@@ -18,6 +18,7 @@
 
 // The fast path may fuse multiply-adds (the library is built with -ffp-contract=off so that the exact
 // line solver keeps the reference's rounding; this translation unit opts back in).
+#pragma once
 #pragma clang fp contract(fast)
 #include "lss_adi_chunk.hpp"
 #include "lss_adi_stencil.hpp"
@@ -192,27 +193,25 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
         xsweep_lines<T, M, W, 3, PAT>(sh, p, tl, step);
 }
 
-#define LSS_INST_P(T, M, PAT)                                                                          \
-    template __global__ void k_vsweep_p<T, M, 4, PAT>(const Prob<T> *, const Tile *, int, int, int);    \
-    template __global__ void k_vsweep_p<T, M, 16, PAT>(const Prob<T> *, const Tile *, int, int, int);   \
-    template __global__ void k_xsweep_p<T, M, 1, 16, PAT>(const Prob<T> *, const Tile *, int, int);     \
-    template __global__ void k_xsweep_p<T, M, 2, 16, PAT>(const Prob<T> *, const Tile *, int, int);     \
-    template __global__ void k_xsweep_p<T, M, 1, 8, PAT>(const Prob<T> *, const Tile *, int, int);      \
-    template __global__ void k_xsweep_p<T, M, 2, 8, PAT>(const Prob<T> *, const Tile *, int, int);      \
-    template __global__ void k_xsweep_p<T, M, 1, 4, PAT>(const Prob<T> *, const Tile *, int, int);      \
-    template __global__ void k_xsweep_p<T, M, 2, 4, PAT>(const Prob<T> *, const Tile *, int, int);
-#define LSS_INST_PM(T, M) LSS_INST_P(T, M, 0) LSS_INST_P(T, M, 1) LSS_INST_P(T, M, 2)
-LSS_INST_PM(double, 1)
-LSS_INST_PM(double, 2)
-LSS_INST_PM(double, 4)
-LSS_INST_PM(double, 8)
-LSS_INST_P(double, 16, 0)
-LSS_INST_PM(float, 1)
-LSS_INST_PM(float, 2)
-LSS_INST_PM(float, 4)
-LSS_INST_PM(float, 8)
-LSS_INST_P(float, 16, 0)
-#undef LSS_INST_PM
-#undef LSS_INST_P
-
 }  // namespace lssamd
+
+// Instantiations (one translation unit per precision and coefficient pattern, so that the library
+// builds in parallel): LSS_INST_PART(T, PAT) instantiates every sweep of precision T and pattern
+// PAT for the chunk bounds the plans use (16 only with generic coefficients).
+#define LSS_INST_P(T, M, PAT)                                                                                  \
+    template __global__ void k_vsweep_p<T, M, 4, PAT>(const Prob<T> *, const Tile *, int, int, int);            \
+    template __global__ void k_vsweep_p<T, M, 16, PAT>(const Prob<T> *, const Tile *, int, int, int);           \
+    template __global__ void k_xsweep_p<T, M, 1, 16, PAT>(const Prob<T> *, const Tile *, int, int);             \
+    template __global__ void k_xsweep_p<T, M, 2, 16, PAT>(const Prob<T> *, const Tile *, int, int);             \
+    template __global__ void k_xsweep_p<T, M, 1, 8, PAT>(const Prob<T> *, const Tile *, int, int);              \
+    template __global__ void k_xsweep_p<T, M, 2, 8, PAT>(const Prob<T> *, const Tile *, int, int);              \
+    template __global__ void k_xsweep_p<T, M, 1, 4, PAT>(const Prob<T> *, const Tile *, int, int);              \
+    template __global__ void k_xsweep_p<T, M, 2, 4, PAT>(const Prob<T> *, const Tile *, int, int);
+#define LSS_INST_PART(T, PAT)                                                                                  \
+    namespace lssamd {                                                                                         \
+    LSS_INST_P(T, 1, PAT) LSS_INST_P(T, 2, PAT) LSS_INST_P(T, 4, PAT) LSS_INST_P(T, 8, PAT)                   \
+    }
+#define LSS_INST_PART16(T) \
+    namespace lssamd {     \
+    LSS_INST_P(T, 16, 0)   \
+    }

@@ -1,0 +1,5 @@
+// Partitioned sweep kernels (float, sabr coefficient pattern); the kernels are in
+// lss_adi_partitioned.hpp.
+#include "lss_adi_partitioned.hpp"
+
+LSS_INST_PART(float, 2)

@@ -1,0 +1,5 @@
+// Partitioned sweep kernels (double, sabr coefficient pattern); the kernels are in
+// lss_adi_partitioned.hpp.
+#include "lss_adi_partitioned.hpp"
+
+LSS_INST_PART(double, 2)
