@@ -75,7 +75,8 @@ template <typename T> __device__ __forceinline__ T pick6(int q, const T (&g)[6])
 // thread's column j is fixed) sums g X_t[i] over the thread's points and multiplies by Y_t[j] once
 // in finish(); FIX 2 (v-line kernels: row i fixed) the other way round; FIX 0 applies both factors
 // at each point.  PAT 1, 2: the term table is TPat<PAT>, fixed at compile time; PAT 0: the
-// problem's own (Adj::tp, tq, fx, fy).  Sums are kept in double.
+// problem's own (Adj::tp, tq, fx, fy), applied term by term at each point in a loop (compact code
+// for the rarely used run-time tables), s[p] summing parameter p.  Sums are kept in double.
 template <typename T, int PAT, int FIX> struct TermSums {
     double s[NTERM];
     gptr<const T> fX, fY;
@@ -108,18 +109,20 @@ template <typename T, int PAT, int FIX> struct TermSums {
     // one point (i, j) with d J / d (a, b, c, d, e, f) = g
     __device__ __forceinline__ void add(const Adj<T> &A, int i, int j, const T (&g)[6])
     {
+        if constexpr (PAT == PAT_GENERIC) {  // compact code: s[p] sums parameter p directly
+#pragma unroll 1
+            for (int t = 0; t < n; ++t) {
+                const double v = (double)pick6(A.tq[t], g) * yf(A.fy[t], j) * xf(A.fx[t], i);
+                const int k = A.tp[t];
+#pragma unroll
+                for (int z = 0; z < NPAR; ++z)
+                    s[z] += (k == z) ? v : 0.0;
+            }
+            return;
+        }
         static_for<0, NTERM>([&](auto tt) {
             constexpr int t = decltype(tt)::value;
-            if constexpr (PAT == PAT_GENERIC) {
-                if (t < n) {
-                    double v = (double)pick6(A.tq[t], g);
-                    if (FIX != 1)
-                        v *= yf(A.fy[t], j);
-                    if (FIX != 2)
-                        v *= xf(A.fx[t], i);
-                    s[t] += v;
-                }
-            } else {
+            if constexpr (PAT != PAT_GENERIC) {
                 using TP = TPat<PAT>;
                 if constexpr (t < TP::n) {
                     if constexpr (FIX == 0 || term_rep<PAT, FIX>(t) == t) {
@@ -138,21 +141,15 @@ template <typename T, int PAT, int FIX> struct TermSums {
     // acc[p] += the sums of parameter p, with the fixed factor of each term applied at (i, j)
     __device__ __forceinline__ void finish(const Adj<T> &A, int i, int j, double *acc) const
     {
+        if constexpr (PAT == PAT_GENERIC) {
+#pragma unroll
+            for (int z = 0; z < NPAR; ++z)
+                acc[z] += s[z];
+            return;
+        }
         static_for<0, NTERM>([&](auto tt) {
             constexpr int t = decltype(tt)::value;
-            if constexpr (PAT == PAT_GENERIC) {
-                if (t < n) {
-                    double v = s[t];
-                    if (FIX == 1)
-                        v *= yf(A.fy[t], j);
-                    if (FIX == 2)
-                        v *= xf(A.fx[t], i);
-                    const int k = A.tp[t];
-#pragma unroll
-                    for (int z = 0; z < NPAR; ++z)
-                        acc[z] += (k == z) ? v : 0.0;
-                }
-            } else {
+            if constexpr (PAT != PAT_GENERIC) {
                 using TP = TPat<PAT>;
                 if constexpr (t < TP::n) {
                     double v = s[FIX == 0 ? t : term_rep<PAT, FIX>(t)];
@@ -370,30 +367,33 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
             }
             return v;
         };
+        // column k + d, clamped to N + 1 (points past the line's end are discarded)
+        const uint32_t jmax = (uint32_t)(N + 1);
+        auto rc = [&](int k, int d) { return (int)min((uint32_t)(k + d), jmax); };
         auto eq = [&](int k) -> Eq<T> {
-            k = min(k, N);  // points past the line's end repeat the last equation
             Eq<T> e;
-            e.b = one + theta * stencil_at<T, PAT>(p, cs, i, k).W;
-            e.a = (k >= 2) ? -theta * stencil_at<T, PAT>(p, cs, i, k - 1).Pt : (T)0;
-            e.c = (k <= N - 1) ? -theta * stencil_at<T, PAT>(p, cs, i, k + 1).Mt : (T)0;
-            e.r = lam_out(k);
+            e.b = one + theta * stencil_at<T, PAT>(p, cs, i, rc(k, 0)).W;
+            e.a = (k >= 2) ? -theta * stencil_at<T, PAT>(p, cs, i, rc(k, -1)).Pt : (T)0;
+            e.c = (k <= N - 1) ? -theta * stencil_at<T, PAT>(p, cs, i, rc(k, 1)).Mt : (T)0;
+            e.r = lam_out(rc(k, 0));
             return e;
         };
         // the solution stays in registers until every load of the parameter terms is issued
         T xs[M];
         auto put = [&](int t, int k, T x) {
             xs[t] = x;
+            if (ts.n == 0)
+                return;
+            const T xv = k <= N ? x : (T)0;  // a discarded point contributes nothing (x may be any value)
             SG<T> g = sg_zero<T>();
-            g.Mt = theta * x * (yrow[k - 1] - irow[k - 1]);
-            g.W = -theta * x * (yrow[k] - irow[k]);
-            g.Pt = theta * x * (yrow[k + 1] - irow[k + 1]);
-            if (ts.n > 0) {
-                T gq[6];
-                sg_to_q(p, g, gq);
-                ts.add(A, i, k, gq);
-            }
+            g.Mt = theta * xv * (yrow[rc(k, -1)] - irow[rc(k, -1)]);
+            g.W = -theta * xv * (yrow[rc(k, 0)] - irow[rc(k, 0)]);
+            g.Pt = theta * xv * (yrow[rc(k, 1)] - irow[rc(k, 1)]);
+            T gq[6];
+            sg_to_q(p, g, gq);
+            ts.add(A, i, rc(k, 0), gq);
         };
-        wave_solve<T, M>(N, lane, eq, put);
+        wave_solve<T, M, true>(N, lane, eq, put);
         int s0, cnt;
         chunk_span(N, lane, s0, cnt);
 #pragma unroll
@@ -413,10 +413,121 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
 // Adjoint of an S-sweep (XTJ columns x NCH chunks per workgroup).  stage 2: X2 of CS/MCS/HV
 // (with HV's explicit stage H, whose multiplier equals X2's); stage 1: X1.
 // -------------------------------------------------------------------------------------------------
-template <typename T, int M, int W, int PAT>
+// KIND (compile time, so that the loads of neighbouring points are shared and issued together):
+// stage 1 of DR 0, of CS/MCS 1, of HV 2; stage 2 of CS 3, of MCS 4, of HV 5.
+template <typename T, int M, int W, int PAT, int KIND>
+__device__ __forceinline__ void adj_x_lines(T (*sh)[4][NCH][W], const Prob<T> &p, const Adj<T> &A, const Tile &tl,
+                                            int step, double *acc)
+{
+    constexpr bool S2 = KIND >= 3, HV = KIND == 2 || KIND == 5, MCS = KIND == 4;
+    const int col = threadIdx.x % W, chn = threadIdx.x / W;
+    const bool colok = col < tl.count;
+    const int j = tl.first + (colok ? col : 0);
+    const int nx = p.nx, ny = p.ny, N = nx - 2;
+    const uint32_t kmax = (uint32_t)(N + 1), uj = (uint32_t)j, uny = (uint32_t)ny;
+    const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
+    const GArr<T> cs{coeff_set(p, step)};
+    const T one = (T)1, theta = p.theta, zeta = p.zeta, gamma = p.gamma;
+    TermSums<T, PAT, 1> ts(A, step, nx, ny);
+    // stage output: Y1, or Y3 (CS/MCS) / Y4 (HV)
+    const gptr<const T> y = G((const T *)(!S2 ? p.y1 : (HV ? p.y4 : p.y3)));
+    const gptr<T> L = G(S2 ? A.lx2 : A.lx1);
+    const gptr<const T> lv2 = G((const T *)A.lv2), lv1 = G((const T *)A.lv1), lx2 = G((const T *)A.lx2);
+    const gptr<const T> y1 = G((const T *)p.y1), y2 = G((const T *)p.y2);
+    // row k + d of column j + dc, rows clamped to N + 1 (points past the line's end are discarded)
+    auto at = [&](gptr<const T> b, int k, int d, int dc) {
+        return ldg(b, min((uint32_t)(k + d), kmax) * uny + (uj + (uint32_t)dc));
+    };
+    // lambda of the stage output at (k, j).  Stage 1 of HV: Y1 is also read by the explicit stage H,
+    // whose weights on Y1 in column j (-theta P_{k-1}, 1 + theta Z_k, -theta M_{k+1}; see w_h) are
+    // the coefficients of this row of the transposed system.
+    auto eq = [&](int k) -> Eq<T> {
+        const int r0 = (int)min((uint32_t)k, kmax), rm = (int)min((uint32_t)(k - 1), kmax),
+                  rp = (int)min((uint32_t)(k + 1), kmax);
+        Eq<T> e;
+        e.b = one + theta * stencil_at<T, PAT>(p, cs, r0, j).Z;
+        e.a = (k >= 2) ? -theta * stencil_at<T, PAT>(p, cs, rm, j).P : (T)0;
+        e.c = (k <= N - 1) ? -theta * stencil_at<T, PAT>(p, cs, rp, j).M : (T)0;
+        if constexpr (S2 || KIND == 0) {
+            e.r = at(lv2, k, 0, 0);
+        } else if constexpr (KIND == 1) {
+            e.r = at(lx2, k, 0, 0) + at(lv1, k, 0, 0);
+        } else {  // e.a = 0 at k = 1 and e.c = 0 at k >= N, so the clamped reads there are harmless
+            e.r = at(lv1, k, 0, 0) + e.b * at(lx2, k, 0, 0) + e.a * at(lx2, k, -1, 0) + e.c * at(lx2, k, 1, 0);
+        }
+        return e;
+    };
+    // the solution stays in registers until every load of the parameter terms is issued
+    T xs[M];
+    auto put = [&](int t, int k, T x) {
+        xs[t] = x;
+        if (ts.n == 0)
+            return;
+        const T xv = k <= N ? x : (T)0;  // a discarded point contributes nothing (x may be any value)
+        auto U = [&](int d, int dc) { return at(u, k, d, dc); };
+        auto Y2 = [&](int d, int dc) { return at(y2, k, d, dc); };
+        SG<T> g = sg_zero<T>();
+        // implicit rows [-theta M, 1 + theta Z, -theta P] applied to the stage output
+        g.M = theta * xv * at(y, k, -1, 0);
+        g.Z = -theta * xv * at(y, k, 0, 0);
+        g.P = theta * xv * at(y, k, 1, 0);
+        if constexpr (!S2) {  // rhs_intermed_1 on U
+            g.M += (one - theta) * xv * U(-1, 0);
+            g.Z += -(one - theta) * xv * U(0, 0);
+            g.W += -xv * U(0, 0);
+            g.P += (one - theta) * xv * U(1, 0);
+            g.Mt += xv * U(0, -1);
+            g.Pt += xv * U(0, 1);
+            g.C += xv * gamma * (U(-1, -1) - U(-1, 1) - U(1, -1) + U(1, 1));
+        } else {
+            auto D = [&](int d, int dc) { return Y2(d, dc) - U(d, dc); };
+            const T cornerD = D(1, 1) - D(1, -1) - D(-1, 1) + D(-1, -1);
+            if constexpr (!HV) {  // rhs_intermed_3
+                g.M += -theta * xv * U(-1, 0);
+                g.Z += theta * xv * U(0, 0);
+                g.P += -theta * xv * U(1, 0);
+                g.C += xv * zeta * gamma * cornerD;
+                if constexpr (MCS) {
+                    const T zt = zeta - theta;
+                    g.M += xv * zt * D(-1, 0);
+                    g.Z += -xv * zt * D(0, 0);
+                    g.W += -xv * zt * D(0, 0);
+                    g.P += -xv * zt * D(1, 0);
+                    g.Mt += xv * zt * D(0, -1);
+                    g.Pt += xv * zt * D(0, 1);
+                }
+            } else {  // rhs_intermed_4 on Y2, then the explicit stage H with the same multiplier
+                g.M += -theta * xv * Y2(-1, 0);
+                g.Z += theta * xv * Y2(0, 0);
+                g.P += -theta * xv * Y2(1, 0);
+                g.M += xv * (-theta * at(y1, k, -1, 0) + theta * U(-1, 0) + zeta * D(-1, 0));
+                g.Z += xv * (theta * at(y1, k, 0, 0) - theta * U(0, 0) - zeta * D(0, 0));
+                g.P += xv * (-theta * at(y1, k, 1, 0) + theta * U(1, 0) + zeta * D(1, 0));
+                g.W += -xv * zeta * D(0, 0);
+                g.Mt += xv * zeta * D(0, -1);
+                g.Pt += xv * zeta * D(0, 1);
+                g.C += xv * zeta * gamma * cornerD;
+            }
+        }
+        T gq[6];
+        sg_to_q(p, g, gq);
+        ts.add(A, (int)min((uint32_t)k, kmax), j, gq);
+    };
+    block_solve<T, M, W, true>(sh, N, col, chn, colok, eq, put);
+    if (colok) {
+        int s0, cnt;
+        chunk_span(N, chn, s0, cnt);
+#pragma unroll
+        for (int t = 0; t < M; ++t)
+            if (t < cnt)
+                L[(int)(s0 + t) * ny + j] = xs[t];
+    }
+    ts.finish(A, 0, j, acc);
+}
+
+template <typename T, int M, int W, int PAT, int STAGE>
 __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
-                                                  const Tile *__restrict__ tiles, int step, int stage,
-                                                  double *__restrict__ slots)
+                                                  const Tile *__restrict__ tiles, int step, double *__restrict__ slots)
 {
     __shared__ T sh[2][4][NCH][W];
     __shared__ double red[W * NCH / 64][NPAR];
@@ -427,111 +538,24 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
 #pragma unroll
     for (int z = 0; z < NPAR; ++z)
         acc[z] = 0.0;
-    if (step >= p.steps || (stage == 2 && p.scheme == SCHEME_DR))
-        return;  // uniform over the workgroup
-    const int col = threadIdx.x % W, chn = threadIdx.x / W;
-    const bool colok = col < tl.count;
-    const int j = tl.first + (colok ? col : 0);
-    const int nx = p.nx, ny = p.ny, N = nx - 2;
-    const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
-    const GArr<T> cs{coeff_set(p, step)};
-    const T one = (T)1, theta = p.theta, zeta = p.zeta, gamma = p.gamma;
-    TermSums<T, PAT, 1> ts(A, step, nx, ny);
     const int sch = p.scheme;
-    const bool hv = sch == SCHEME_HV, mcs = sch == SCHEME_MCS, dr = sch == SCHEME_DR;
-    // stage output: Y1, or Y3 (CS/MCS) / Y4 (HV)
-    const gptr<const T> y = G((const T *)((stage == 1) ? p.y1 : (hv ? p.y4 : p.y3)));
-    const gptr<T> L = G((stage == 2) ? A.lx2 : A.lx1);
-    const gptr<const T> lv2 = G((const T *)A.lv2), lv1 = G((const T *)A.lv1), lx2 = G((const T *)A.lx2);
-    const gptr<const T> y1 = G((const T *)p.y1), y2 = G((const T *)p.y2);
-    // lambda of the stage output at (k, j).  Stage 1 of HV: Y1 is also read by the explicit stage H,
-    // whose weights on Y1 in column j (-theta P_{k-1}, 1 + theta Z_k, -theta M_{k+1}; see w_h) are
-    // the coefficients of this row of the transposed system.
-    auto eq = [&](int k) -> Eq<T> {
-        k = min(k, N);  // points past the line's end repeat the last equation
-        const int q = (int)k * ny + j;
-        Eq<T> e;
-        e.b = one + theta * stencil_at<T, PAT>(p, cs, k, j).Z;
-        e.a = (k >= 2) ? -theta * stencil_at<T, PAT>(p, cs, k - 1, j).P : (T)0;
-        e.c = (k <= N - 1) ? -theta * stencil_at<T, PAT>(p, cs, k + 1, j).M : (T)0;
-        if (stage == 2 || dr) {
-            e.r = lv2[q];
-        } else if (!hv) {
-            e.r = lx2[q] + lv1[q];
-        } else {
-            T v = lv1[q] + e.b * lx2[q];
-            if (k >= 2)
-                v += e.a * lx2[q - ny];
-            if (k <= N - 1)
-                v += e.c * lx2[q + ny];
-            e.r = v;
-        }
-        return e;
-    };
-    // the solution stays in registers until every load of the parameter terms is issued
-    T xs[M];
-    auto put = [&](int t, int k, T x) {
-        const int q = (int)k * ny + j;
-        xs[t] = x;
-        if (ts.n == 0)
-            return;
-        SG<T> g = sg_zero<T>();
-        // implicit rows [-theta M, 1 + theta Z, -theta P] applied to the stage output
-        g.M = theta * x * y[q - ny];
-        g.Z = -theta * x * y[q];
-        g.P = theta * x * y[q + ny];
-        if (stage == 1) {  // rhs_intermed_1 on U
-            g.M += (one - theta) * x * u[q - ny];
-            g.Z += -(one - theta) * x * u[q];
-            g.W += -x * u[q];
-            g.P += (one - theta) * x * u[q + ny];
-            g.Mt += x * u[q - 1];
-            g.Pt += x * u[q + 1];
-            g.C += x * gamma * (u[q - ny - 1] - u[q - ny + 1] - u[q + ny - 1] + u[q + ny + 1]);
-        } else {
-            auto D = [&](int o) { return y2[q + o] - u[q + o]; };
-            const T cornerD = D(ny + 1) - D(ny - 1) - D(-ny + 1) + D(-ny - 1);
-            if (!hv) {  // rhs_intermed_3
-                g.M += -theta * x * u[q - ny];
-                g.Z += theta * x * u[q];
-                g.P += -theta * x * u[q + ny];
-                g.C += x * zeta * gamma * cornerD;
-                if (mcs) {
-                    const T zt = zeta - theta;
-                    g.M += x * zt * D(-ny);
-                    g.Z += -x * zt * D(0);
-                    g.W += -x * zt * D(0);
-                    g.P += -x * zt * D(ny);
-                    g.Mt += x * zt * D(-1);
-                    g.Pt += x * zt * D(1);
-                }
-            } else {  // rhs_intermed_4 on Y2, then the explicit stage H with the same multiplier
-                g.M += -theta * x * y2[q - ny];
-                g.Z += theta * x * y2[q];
-                g.P += -theta * x * y2[q + ny];
-                g.M += x * (-theta * y1[q - ny] + theta * u[q - ny] + zeta * D(-ny));
-                g.Z += x * (theta * y1[q] - theta * u[q] - zeta * D(0));
-                g.P += x * (-theta * y1[q + ny] + theta * u[q + ny] + zeta * D(ny));
-                g.W += -x * zeta * D(0);
-                g.Mt += x * zeta * D(-1);
-                g.Pt += x * zeta * D(1);
-                g.C += x * zeta * gamma * cornerD;
-            }
-        }
-        T gq[6];
-        sg_to_q(p, g, gq);
-        ts.add(A, k, j, gq);
-    };
-    block_solve<T, M, W>(sh, N, col, chn, colok, eq, put);
-    if (colok) {
-        int s0, cnt;
-        chunk_span(N, chn, s0, cnt);
-#pragma unroll
-        for (int t = 0; t < M; ++t)
-            if (t < cnt)
-                L[(int)(s0 + t) * ny + j] = xs[t];
+    if (step >= p.steps || (STAGE == 2 && sch == SCHEME_DR))
+        return;  // uniform over the workgroup
+    if constexpr (STAGE == 1) {
+        if (sch == SCHEME_DR)
+            adj_x_lines<T, M, W, PAT, 0>(sh, p, A, tl, step, acc);
+        else if (sch == SCHEME_HV)
+            adj_x_lines<T, M, W, PAT, 2>(sh, p, A, tl, step, acc);
+        else
+            adj_x_lines<T, M, W, PAT, 1>(sh, p, A, tl, step, acc);
+    } else {
+        if (sch == SCHEME_CS)
+            adj_x_lines<T, M, W, PAT, 3>(sh, p, A, tl, step, acc);
+        else if (sch == SCHEME_MCS)
+            adj_x_lines<T, M, W, PAT, 4>(sh, p, A, tl, step, acc);
+        else
+            adj_x_lines<T, M, W, PAT, 5>(sh, p, A, tl, step, acc);
     }
-    ts.finish(A, 0, j, acc);
     block_acc_store<W * NCH / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
 
@@ -712,13 +736,17 @@ template <typename T> __global__ void __launch_bounds__(256) k_copy_jobs(const C
 }  // namespace lssamd
 
 // Instantiations (one translation unit per precision and pattern): LSS_INST_ADJ(T, PAT).
-#define LSS_INST_A(T, M, PAT)                                                                                  \
-    template __global__ void k_adj_v<T, M, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);    \
-    template __global__ void k_adj_x<T, M, 16, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *); \
-    template __global__ void k_adj_x<T, M, 8, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
+#define LSS_INST_AX(T, M, W, PAT)                                                                              \
+    template __global__ void k_adj_x<T, M, W, PAT, 1>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *); \
+    template __global__ void k_adj_x<T, M, W, PAT, 2>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
+#define LSS_INST_AV(T, M, PAT) \
+    template __global__ void k_adj_v<T, M, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
+// S-lines per adjoint S-sweep workgroup: 16 for chunk bounds up to 8, 8 for 16 (the plan's rule)
 #define LSS_INST_ADJ(T, PAT)                                                                                   \
     namespace lssamd {                                                                                         \
-    LSS_INST_A(T, 1, PAT) LSS_INST_A(T, 2, PAT) LSS_INST_A(T, 4, PAT) LSS_INST_A(T, 8, PAT) LSS_INST_A(T, 16, PAT) \
+    LSS_INST_AV(T, 1, PAT) LSS_INST_AV(T, 2, PAT) LSS_INST_AV(T, 4, PAT) LSS_INST_AV(T, 8, PAT)                 \
+    LSS_INST_AV(T, 16, PAT) LSS_INST_AX(T, 1, 16, PAT) LSS_INST_AX(T, 2, 16, PAT) LSS_INST_AX(T, 4, 16, PAT)    \
+    LSS_INST_AX(T, 8, 16, PAT) LSS_INST_AX(T, 16, 8, PAT)                                                       \
     template __global__ void k_adj_g<T, 0, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);       \
     template __global__ void k_adj_g<T, 1, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);       \
     }

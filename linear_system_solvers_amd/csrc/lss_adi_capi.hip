@@ -39,8 +39,8 @@ __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int, int);
 template <typename T, int M, int VR, int PAT> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int, int);
 template <typename T, int M, int PAT>
 __global__ void k_adj_v(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
-template <typename T, int M, int W, int PAT>
-__global__ void k_adj_x(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
+template <typename T, int M, int W, int PAT, int STAGE>
+__global__ void k_adj_x(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
 template <typename T, int WHAT, int PAT>
 __global__ void k_adj_g(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
 template <typename T> __global__ void k_copy_jobs(const CopyJob *);
@@ -503,12 +503,10 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     }
     template <int M, int W> void launch_x_m(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
-        if constexpr (M <= 8) {
-            if (fpat == PAT_HESTON)
-                return launch_x_p<M, W, PAT_HESTON>(stage, s, st, pd, na);
-            if (fpat == PAT_SABR0)
-                return launch_x_p<M, W, PAT_SABR0>(stage, s, st, pd, na);
-        }
+        if (fpat == PAT_HESTON)
+            return launch_x_p<M, W, PAT_HESTON>(stage, s, st, pd, na);
+        if (fpat == PAT_SABR0)
+            return launch_x_p<M, W, PAT_SABR0>(stage, s, st, pd, na);
         launch_x_p<M, W, PAT_GENERIC>(stage, s, st, pd, na);
     }
     template <int W> void launch_x_w(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
@@ -539,12 +537,10 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     }
     template <int M, int VR> void launch_v_m(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
-        if constexpr (M <= 8) {
-            if (fpat == PAT_HESTON)
-                return launch_v_p<M, VR, PAT_HESTON>(stage, s, st, pd, na);
-            if (fpat == PAT_SABR0)
-                return launch_v_p<M, VR, PAT_SABR0>(stage, s, st, pd, na);
-        }
+        if (fpat == PAT_HESTON)
+            return launch_v_p<M, VR, PAT_HESTON>(stage, s, st, pd, na);
+        if (fpat == PAT_SABR0)
+            return launch_v_p<M, VR, PAT_SABR0>(stage, s, st, pd, na);
         launch_v_p<M, VR, PAT_GENERIC>(stage, s, st, pd, na);
     }
     template <int VR> void launch_v_r(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
@@ -995,6 +991,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         new (&lam_jobs_dev) dev_buffer();
         lam_jobs_dev.alloc(2 * n * sizeof(CopyJob));
         // parameter slots; S-lines of 16 chunks per point need the wider register budget of 8-line groups
+        // S-lines per adjoint S-sweep workgroup: 8 for 16-point chunks (register budget), else 16
+        // (measured faster than the forward's tuned width on the SABR book)
         axw = pmx >= 16 ? 8 : 16;
         slot_tb[1] = slot_tb[3] = axw == 8 ? TB_X8 : TB_AX;
         nslots = 0;
@@ -1034,15 +1032,20 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         else
             adj_v_p<M, PAT_GENERIC>(s, last, table, slot, st, grid);
     }
+    template <int M, int W, int PAT>
+    void adj_x_w(int s, int stage, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
+    {
+        const Adj<T> *ad = (const Adj<T> *)adj_dev.p;
+        const Tile *tx = (const Tile *)tiles_dev.p + tile_off[W == 8 ? TB_X8 : TB_AX];
+        if (stage == 1)
+            hipLaunchKernelGGL((k_adj_x<T, M, W, PAT, 1>), dim3(grid), dim3(W * P_NCH), 0, st, table, ad, tx, s, slot);
+        else
+            hipLaunchKernelGGL((k_adj_x<T, M, W, PAT, 2>), dim3(grid), dim3(W * P_NCH), 0, st, table, ad, tx, s, slot);
+    }
     template <int M, int PAT>
     void adj_x_p(int s, int stage, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
     {
-        if (axw == 8)
-            hipLaunchKernelGGL((k_adj_x<T, M, 8, PAT>), dim3(grid), dim3(8 * P_NCH), 0, st, table,
-                               (const Adj<T> *)adj_dev.p, (const Tile *)tiles_dev.p + tile_off[TB_X8], s, stage, slot);
-        else
-            hipLaunchKernelGGL((k_adj_x<T, M, 16, PAT>), dim3(grid), dim3(16 * P_NCH), 0, st, table,
-                               (const Adj<T> *)adj_dev.p, (const Tile *)tiles_dev.p + tile_off[TB_AX], s, stage, slot);
+        adj_x_w<M, (M >= 16 ? 8 : 16), PAT>(s, stage, table, slot, st, grid);  // = axw (M = pmx)
     }
     template <int M> void adj_x(int s, int stage, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
     {

@@ -183,7 +183,9 @@ template <typename T> __device__ __forceinline__ Eq<T> identity_eq()
 //   last chunk, whose equations are discarded): it must then read only in-bounds data.
 //   put(t, k, x) receives the solution of unknown k (k <= N), the t-th point of the calling
 //   thread's chunk (t is a compile-time index after unrolling, so a caller may keep the values in
-//   registers and store them later).
+//   registers and store them later).  With PUT_ALL, put is called for all C points of a non-empty
+//   chunk without per-point branches (so that the caller's loads can be issued together); then
+//   k > N marks a discarded point whose x is arbitrary (possibly not finite).
 // -------------------------------------------------------------------------------------------------
 __device__ __forceinline__ int chunk_len(int N)
 {
@@ -207,7 +209,7 @@ template <int M, typename F> __device__ __forceinline__ void with_chunk_len(int 
 
 // One line per wavefront; lane = chunk; the interface system is reduced with shuffles.
 // Every lane of the wavefront must call it; chunk_len(N) <= M.
-template <typename T, int M, typename EqF, typename PutF>
+template <typename T, int M, bool PUT_ALL = false, typename EqF, typename PutF>
 __device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put)
 {
     with_chunk_len<M>(chunk_len(N), [&](auto cc) {
@@ -253,10 +255,18 @@ __device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put
         T xl = __shfl_up(xi, 1);
         if (lane == 0)
             xl = (T)0;
+        if constexpr (PUT_ALL) {
+            if (ch.cnt > 0) {
 #pragma unroll
-        for (int t = 0; t < C; ++t)
-            if (t < ch.cnt)
-                put(t, s + t, ch.value(t, xl, xi));
+                for (int t = 0; t < C; ++t)
+                    put(t, s + t, ch.value(t, xl, xi));
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < C; ++t)
+                if (t < ch.cnt)
+                    put(t, s + t, ch.value(t, xl, xi));
+        }
     });
 }
 
@@ -264,7 +274,7 @@ __device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put
 // system is reduced through LDS `sh`.  Every thread of the workgroup must call it (it holds
 // barriers) with the same N (chunk_len(N) <= M); threads of lines that do not exist pass
 // active = false (their eq is not called).
-template <typename T, int M, int W, typename EqF, typename PutF>
+template <typename T, int M, int W, bool PUT_ALL = false, typename EqF, typename PutF>
 __device__ __forceinline__ void block_solve(T (*sh)[4][NCH][W], int N, int col, int chn, bool active, EqF &&eq,
                                             PutF &&put)
 {
@@ -330,10 +340,18 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NCH][W], int N, int col, 
         const T xl = chn > 0 ? sh[buf ^ 1][0][chn - 1][col] : (T)0;
         if (!active)
             return;
+        if constexpr (PUT_ALL) {
+            if (ch.cnt > 0) {
 #pragma unroll
-        for (int t = 0; t < C; ++t)
-            if (t < ch.cnt)
-                put(t, s + t, ch.value(t, xl, xi));
+                for (int t = 0; t < C; ++t)
+                    put(t, s + t, ch.value(t, xl, xi));
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < C; ++t)
+                if (t < ch.cnt)
+                    put(t, s + t, ch.value(t, xl, xi));
+        }
     });
 }
 

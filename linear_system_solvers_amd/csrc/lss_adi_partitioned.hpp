@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
 
 // Instantiations (one translation unit per precision and coefficient pattern, so that the library
 // builds in parallel): LSS_INST_PART(T, PAT) instantiates every sweep of precision T and pattern
-// PAT for the chunk bounds the plans use (16 only with generic coefficients).
+// PAT for the chunk bounds the plans use.
 #define LSS_INST_P(T, M, PAT)                                                                                  \
     template __global__ void k_vsweep_p<T, M, 4, PAT>(const Prob<T> *, const Tile *, int, int, int);            \
     template __global__ void k_vsweep_p<T, M, 16, PAT>(const Prob<T> *, const Tile *, int, int, int);           \
@@ -209,9 +209,5 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
     template __global__ void k_xsweep_p<T, M, 2, 4, PAT>(const Prob<T> *, const Tile *, int, int);
 #define LSS_INST_PART(T, PAT)                                                                                  \
     namespace lssamd {                                                                                         \
-    LSS_INST_P(T, 1, PAT) LSS_INST_P(T, 2, PAT) LSS_INST_P(T, 4, PAT) LSS_INST_P(T, 8, PAT)                   \
-    }
-#define LSS_INST_PART16(T) \
-    namespace lssamd {     \
-    LSS_INST_P(T, 16, 0)   \
+    LSS_INST_P(T, 1, PAT) LSS_INST_P(T, 2, PAT) LSS_INST_P(T, 4, PAT) LSS_INST_P(T, 8, PAT) LSS_INST_P(T, 16, PAT) \
     }

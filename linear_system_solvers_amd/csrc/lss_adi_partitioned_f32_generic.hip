@@ -3,4 +3,3 @@
 #include "lss_adi_partitioned.hpp"
 
 LSS_INST_PART(float, 0)
-LSS_INST_PART16(float)
