@@ -176,6 +176,26 @@ int lss_tridiag_batched(int32_t fp, int32_t layout, int64_t B, int64_t n, const 
                         const void *c, const void *d, void *x, void *stream);
 
 /*
+ * The same batched solve with end conditions folded into the first and last rows as the reference's
+ * thomas_lu_solver folds its boundary pair (lss_thomas_lu_boundary.hpp:114-205, with
+ * lss_dirichlet/neumann/robin_boundary.hpp conventions), the same for every system:
+ *   LSS_BC_DIRICHLET  the end value is val (x_0 or x_{n-1} = val; the system is solved for the rest);
+ *   LSS_BC_NEUMANN    u' + val = 0: the ghost point x_{-1} = x_1 + 2 h val (lower) or
+ *                     x_n = x_{n-2} - 2 h val (upper) is folded into the end row;
+ *   LSS_BC_ROBIN      u' + lin u + val = 0: ghost x_{-1} = x_1 + 2 h (lin x_0 + val), x_n likewise;
+ * h = 1 / (n - 1).  Replaces thomas_lu_solver::solve(boundary_pair, solution) per system.
+ */
+enum { LSS_BC_DIRICHLET = 0, LSS_BC_NEUMANN = 1, LSS_BC_ROBIN = 2 };
+typedef struct {
+    int32_t type; /* LSS_BC_* */
+    double lin;   /* Robin only */
+    double val;
+} lss_line_bc;
+int lss_tridiag_batched_bc(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b,
+                           const void *c, const void *d, const lss_line_bc *lower, const lss_line_bc *upper, void *x,
+                           void *stream);
+
+/*
  * Same time loop as lss_adi_run_dev (device pointers, no graph) with a hipEvent pair recorded on
  * `stream` around every launch; synchronous.  For each kernel kind k < LSS_K_COUNT it returns the
  * summed launch time ms[k], the summed algorithmic HBM bytes bytes[k] (compulsory reads and writes

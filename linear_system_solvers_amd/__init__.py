@@ -29,6 +29,7 @@ UNIFORM, NONUNIFORM = 0, 1
 COEFF_FIELDS, COEFF_SEPARABLE = 0, 1
 LINE_EXACT, LINE_PARTITIONED = 0, 1
 LAYOUT_CONTIGUOUS, LAYOUT_STRIDED = 0, 1
+BC_DIRICHLET, BC_NEUMANN, BC_ROBIN = 0, 1, 2
 MODEL_HESTON, MODEL_SABR = 0, 1
 
 # every symbol include/lss_amd.h declares
@@ -36,7 +37,7 @@ EXPORTS = ("lss_last_error", "lss_version", "lss_adi_plan_create", "lss_adi_plan
            "lss_adi_run_host", "lss_model_plan_create", "lss_model_initial", "lss_model_grid", "lss_plan_problem_shape",
            "lss_tridiag_batched", "lss_adi_run_profiled", "lss_adi_set_param_terms", "lss_plan_param_count",
            "lss_adi_forward", "lss_adi_backward", "lss_adi_autotune",
-           "lss_adi_set_mapping", "lss_plan_kernel_config")
+           "lss_adi_set_mapping", "lss_plan_kernel_config", "lss_tridiag_batched_bc")
 MAX_PARAMS = 6
 MAX_TERMS = 12
 HESTON_PARAMS = ("kappa", "theta_v", "sigma", "rho", "r")
@@ -156,6 +157,9 @@ def lib():
                                            C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.lss_tridiag_batched.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.lss_tridiag_batched_bc.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.POINTER(_LineBC), C.POINTER(_LineBC),
+                                             C.c_void_p, C.c_void_p]
         L.lss_adi_set_param_terms.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                               C.POINTER(C.c_int32), C.c_int64, C.c_void_p]
         L.lss_plan_param_count.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]
@@ -326,3 +330,20 @@ def tridiag_batched(fp: int, layout: int, B: int, n: int, a: int, b: int, c: int
     """Solve B plain tridiagonal systems of length n (device pointers; a[0], c[n-1] ignored)."""
     _check(lib().lss_tridiag_batched(fp, layout, B, n, C.c_void_p(a), C.c_void_p(b), C.c_void_p(c),
                                      C.c_void_p(d), C.c_void_p(x), C.c_void_p(stream)))
+
+
+class _LineBC(C.Structure):
+    _fields_ = [("type", C.c_int32), ("lin", C.c_double), ("val", C.c_double)]
+
+
+def tridiag_batched_bc(fp: int, layout: int, B: int, n: int, a: int, b: int, c: int, d: int, x: int,
+                       lower: tuple, upper: tuple, stream: int = 0) -> None:
+    """Solve B tridiagonal systems of length n (device pointers) with the end conditions folded in as
+    the reference's thomas_lu_solver does; lower/upper = (BC_DIRICHLET, val), (BC_NEUMANN, val) or
+    (BC_ROBIN, lin, val)."""
+    def mk(e):
+        return _LineBC(int(e[0]), float(e[1]) if len(e) == 3 else 0.0, float(e[-1]))
+    lo, hi = mk(lower), mk(upper)
+    _check(lib().lss_tridiag_batched_bc(fp, layout, B, n, C.c_void_p(a), C.c_void_p(b), C.c_void_p(c),
+                                        C.c_void_p(d), C.byref(lo), C.byref(hi), C.c_void_p(x),
+                                        C.c_void_p(stream)))

@@ -330,6 +330,9 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
                                                       double *__restrict__ slots)
 {
     __shared__ double red[VROWS][NPAR];
+    // Y-block factor vectors of a compile-time pattern in LDS (read at every point of a v-line)
+    constexpr int NVY = fpat_nvec<PAT>(12);
+    __shared__ T ysh[NVY > 0 ? NVY * (NCH * M + 2) : 1];
     const Tile tl = tiles[blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
     const Adj<T> &A = adjs[tl.prob];
@@ -338,7 +341,22 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
 #pragma unroll
     for (int z = 0; z < NPAR; ++z)
         acc[z] = 0.0;
-    const bool run = step < p.steps && (last || p.scheme != SCHEME_DR) && wave < tl.count;
+    const bool wg_run = step < p.steps && (last || p.scheme != SCHEME_DR);  // uniform over the workgroup
+    if constexpr (NVY > 0) {
+        if (wg_run) {
+            const GArr<T> yb = GArr<T>{coeff_set(p, step)} + 12 * (uint32_t)p.nx;
+            for (int q = threadIdx.x; q < NVY * p.ny; q += blockDim.x)
+                ysh[q] = yb[(uint32_t)q];
+        }
+        __syncthreads();
+    }
+    auto stn = [&](const GArr<T> &cs, int i, int j) -> Stencil<T> {
+        if constexpr (NVY > 0)
+            return stencil_xy<T, PAT>(p, cs, LArr<T>{(lptr<const T>)ysh}, i, j);
+        else
+            return stencil_at<T, PAT>(p, cs, i, j);
+    };
+    const bool run = wg_run && wave < tl.count;
     if (run) {
         const int i = tl.first + wave;
         const int nx = p.nx, ny = p.ny, N = ny - 2, P = nx - 1;
@@ -372,9 +390,9 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
         auto rc = [&](int k, int d) { return (int)min((uint32_t)(k + d), jmax); };
         auto eq = [&](int k) -> Eq<T> {
             Eq<T> e;
-            e.b = one + theta * stencil_at<T, PAT>(p, cs, i, rc(k, 0)).W;
-            e.a = (k >= 2) ? -theta * stencil_at<T, PAT>(p, cs, i, rc(k, -1)).Pt : (T)0;
-            e.c = (k <= N - 1) ? -theta * stencil_at<T, PAT>(p, cs, i, rc(k, 1)).Mt : (T)0;
+            e.b = one + theta * stn(cs, i, rc(k, 0)).W;
+            e.a = (k >= 2) ? -theta * stn(cs, i, rc(k, -1)).Pt : (T)0;
+            e.c = (k <= N - 1) ? -theta * stn(cs, i, rc(k, 1)).Mt : (T)0;
             e.r = lam_out(rc(k, 0));
             return e;
         };
@@ -401,7 +419,7 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
             if (t < cnt)
                 L[s0 + t] = xs[t];
         if (lane == 0) {
-            const T a1 = -theta * stencil_at<T, PAT>(p, cs, i, 1).Mt;
+            const T a1 = -theta * stn(cs, i, 1).Mt;
             G(last ? A.elo2 : A.elo1)[i] = lam_out(0) - a1 * xs[0];
         }
         ts.finish(A, i, 0, acc);
@@ -417,7 +435,7 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
 // stage 1 of DR 0, of CS/MCS 1, of HV 2; stage 2 of CS 3, of MCS 4, of HV 5.
 template <typename T, int M, int W, int PAT, int KIND>
 __device__ __forceinline__ void adj_x_lines(T (*sh)[4][NCH][W], const Prob<T> &p, const Adj<T> &A, const Tile &tl,
-                                            int step, double *acc)
+                                            int step, double *acc, lptr<const T> xsh)
 {
     constexpr bool S2 = KIND >= 3, HV = KIND == 2 || KIND == 5, MCS = KIND == 4;
     const int col = threadIdx.x % W, chn = threadIdx.x / W;
@@ -441,13 +459,20 @@ __device__ __forceinline__ void adj_x_lines(T (*sh)[4][NCH][W], const Prob<T> &p
     // lambda of the stage output at (k, j).  Stage 1 of HV: Y1 is also read by the explicit stage H,
     // whose weights on Y1 in column j (-theta P_{k-1}, 1 + theta Z_k, -theta M_{k+1}; see w_h) are
     // the coefficients of this row of the transposed system.
+    // stencils: the X-block factors of a compile-time pattern come from LDS
+    auto stn = [&](int r) -> Stencil<T> {
+        if constexpr (PAT != PAT_GENERIC)
+            return stencil_xy<T, PAT>(p, LArr<T>{xsh}, cs + 12 * (uint32_t)nx, r, j);
+        else
+            return stencil_at<T, PAT>(p, cs, r, j);
+    };
     auto eq = [&](int k) -> Eq<T> {
         const int r0 = (int)min((uint32_t)k, kmax), rm = (int)min((uint32_t)(k - 1), kmax),
                   rp = (int)min((uint32_t)(k + 1), kmax);
         Eq<T> e;
-        e.b = one + theta * stencil_at<T, PAT>(p, cs, r0, j).Z;
-        e.a = (k >= 2) ? -theta * stencil_at<T, PAT>(p, cs, rm, j).P : (T)0;
-        e.c = (k <= N - 1) ? -theta * stencil_at<T, PAT>(p, cs, rp, j).M : (T)0;
+        e.b = one + theta * stn(r0).Z;
+        e.a = (k >= 2) ? -theta * stn(rm).P : (T)0;
+        e.c = (k <= N - 1) ? -theta * stn(rp).M : (T)0;
         if constexpr (S2 || KIND == 0) {
             e.r = at(lv2, k, 0, 0);
         } else if constexpr (KIND == 1) {
@@ -531,6 +556,9 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
 {
     __shared__ T sh[2][4][NCH][W];
     __shared__ double red[W * NCH / 64][NPAR];
+    // X-block factor vectors of a compile-time pattern in LDS (read at every point of an S-line)
+    constexpr int NVX = fpat_nvec<PAT>(0);
+    __shared__ T xsh[NVX > 0 ? NVX * (NCH * M + 2) : 1];
     const Tile tl = tiles[blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
     const Adj<T> &A = adjs[tl.prob];
@@ -541,20 +569,27 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
     const int sch = p.scheme;
     if (step >= p.steps || (STAGE == 2 && sch == SCHEME_DR))
         return;  // uniform over the workgroup
+    if constexpr (NVX > 0) {
+        const GArr<T> xb{coeff_set(p, step)};
+        for (int q = threadIdx.x; q < NVX * p.nx; q += blockDim.x)
+            xsh[q] = xb[(uint32_t)q];
+        __syncthreads();
+    }
+    const lptr<const T> xs = (lptr<const T>)xsh;
     if constexpr (STAGE == 1) {
         if (sch == SCHEME_DR)
-            adj_x_lines<T, M, W, PAT, 0>(sh, p, A, tl, step, acc);
+            adj_x_lines<T, M, W, PAT, 0>(sh, p, A, tl, step, acc, xs);
         else if (sch == SCHEME_HV)
-            adj_x_lines<T, M, W, PAT, 2>(sh, p, A, tl, step, acc);
+            adj_x_lines<T, M, W, PAT, 2>(sh, p, A, tl, step, acc, xs);
         else
-            adj_x_lines<T, M, W, PAT, 1>(sh, p, A, tl, step, acc);
+            adj_x_lines<T, M, W, PAT, 1>(sh, p, A, tl, step, acc, xs);
     } else {
         if (sch == SCHEME_CS)
-            adj_x_lines<T, M, W, PAT, 3>(sh, p, A, tl, step, acc);
+            adj_x_lines<T, M, W, PAT, 3>(sh, p, A, tl, step, acc, xs);
         else if (sch == SCHEME_MCS)
-            adj_x_lines<T, M, W, PAT, 4>(sh, p, A, tl, step, acc);
+            adj_x_lines<T, M, W, PAT, 4>(sh, p, A, tl, step, acc, xs);
         else
-            adj_x_lines<T, M, W, PAT, 5>(sh, p, A, tl, step, acc);
+            adj_x_lines<T, M, W, PAT, 5>(sh, p, A, tl, step, acc, xs);
     }
     block_acc_store<W * NCH / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }

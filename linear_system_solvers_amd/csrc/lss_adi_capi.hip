@@ -33,7 +33,7 @@ template <typename T> __global__ void k_xsweep(const Prob<T> *, const Tile *, in
 template <typename T> __global__ void k_vsweep(const Prob<T> *, const Tile *, int, int);
 template <typename T> __global__ void k_hv_y3(const Prob<T> *, const Tile *, int);
 template <typename T>
-__global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *);
+__global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *, LineBC<T>);
 template <typename T, int M, int STAGE, int W, int PAT>
 __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int, int);
 template <typename T, int M, int VR, int PAT> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int, int);
@@ -45,9 +45,9 @@ template <typename T, int WHAT, int PAT>
 __global__ void k_adj_g(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
 template <typename T> __global__ void k_copy_jobs(const CopyJob *);
 template <typename T, int M>
-__global__ void k_tridiag_rows(int64_t, int, const T *, const T *, const T *, const T *, T *);
+__global__ void k_tridiag_rows(int64_t, int, const T *, const T *, const T *, const T *, T *, LineBC<T>);
 template <typename T, int M>
-__global__ void k_tridiag_cols(int64_t, int, const T *, const T *, const T *, const T *, T *);
+__global__ void k_tridiag_cols(int64_t, int, const T *, const T *, const T *, const T *, T *, LineBC<T>);
 constexpr int P_NCH = 64, P_XTJ = 16, P_VROWS = 4;
 
 }  // namespace lssamd
@@ -1553,31 +1553,31 @@ constexpr int TRI_WAVES_H = 4;  // systems per workgroup of k_tridiag_rows
 
 template <typename T, int M>
 void tridiag_part_m(int layout, int64_t B, int n, const void *a, const void *b, const void *c, const void *d, void *x,
-                    size_t lds_rows, hipStream_t st)
+                    size_t lds_rows, const LineBC<T> &bc, hipStream_t st)
 {
     const T *pa = (const T *)a, *pb = (const T *)b, *pc = (const T *)c, *pd = (const T *)d;
     T *px = (T *)x;
     if (layout == LSS_LAYOUT_CONTIGUOUS) {
         const unsigned blocks = (unsigned)((B + TRI_WAVES_H - 1) / TRI_WAVES_H);
         hipLaunchKernelGGL((k_tridiag_rows<T, M>), dim3(blocks), dim3(64 * TRI_WAVES_H), (unsigned)lds_rows, st, B, n,
-                           pa, pb, pc, pd, px);
+                           pa, pb, pc, pd, px, bc);
     } else {
         const unsigned blocks = (unsigned)((B + P_XTJ - 1) / P_XTJ);
         hipLaunchKernelGGL((k_tridiag_cols<T, M>), dim3(blocks), dim3(P_XTJ * P_NCH), 0, st, B, n, pa, pb, pc, pd,
-                           px);
+                           px, bc);
     }
 }
 
 template <typename T>
 void tridiag_part(int M, int layout, int64_t B, int n, const void *a, const void *b, const void *c, const void *d,
-                  void *x, size_t lds_rows, hipStream_t st)
+                  void *x, size_t lds_rows, const LineBC<T> &bc, hipStream_t st)
 {
     switch (M) {
-    case 1: tridiag_part_m<T, 1>(layout, B, n, a, b, c, d, x, lds_rows, st); break;
-    case 2: tridiag_part_m<T, 2>(layout, B, n, a, b, c, d, x, lds_rows, st); break;
-    case 4: tridiag_part_m<T, 4>(layout, B, n, a, b, c, d, x, lds_rows, st); break;
-    case 8: tridiag_part_m<T, 8>(layout, B, n, a, b, c, d, x, lds_rows, st); break;
-    default: tridiag_part_m<T, 16>(layout, B, n, a, b, c, d, x, lds_rows, st); break;
+    case 1: tridiag_part_m<T, 1>(layout, B, n, a, b, c, d, x, lds_rows, bc, st); break;
+    case 2: tridiag_part_m<T, 2>(layout, B, n, a, b, c, d, x, lds_rows, bc, st); break;
+    case 4: tridiag_part_m<T, 4>(layout, B, n, a, b, c, d, x, lds_rows, bc, st); break;
+    case 8: tridiag_part_m<T, 8>(layout, B, n, a, b, c, d, x, lds_rows, bc, st); break;
+    default: tridiag_part_m<T, 16>(layout, B, n, a, b, c, d, x, lds_rows, bc, st); break;
     }
 }
 
@@ -1606,6 +1606,63 @@ void *workspace(size_t bytes)
         w.second = bytes;
     }
     return w.first;
+}
+
+template <typename T> LineBC<T> line_bc(const lss_line_bc *lo, const lss_line_bc *hi, int64_t n)
+{
+    LineBC<T> bc;
+    bc.lo = lo ? lo->type : BC_PLAIN;
+    bc.hi = hi ? hi->type : BC_PLAIN;
+    bc.lo_lin = lo ? (T)lo->lin : (T)0;
+    bc.lo_val = lo ? (T)lo->val : (T)0;
+    bc.hi_lin = hi ? (T)hi->lin : (T)0;
+    bc.hi_val = hi ? (T)hi->val : (T)0;
+    const T step = (T)1 / (T)(n - 1);  // the solver's h (thomas_lu_boundary.hpp: two * step * value)
+    bc.h2 = (T)2 * step;
+    return bc;
+}
+
+// lss_tridiag_batched(_bc): lower/upper null = plain rows
+void tridiag_run(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b, const void *c,
+                 const void *d, const lss_line_bc *lo, const lss_line_bc *hi, void *x, hipStream_t st)
+{
+    require(fp == LSS_FP64 || fp == LSS_FP32, "fp must be LSS_FP64 or LSS_FP32");
+    require(layout == LSS_LAYOUT_CONTIGUOUS || layout == LSS_LAYOUT_STRIDED, "unknown layout");
+    require(B >= 0 && n >= 4 && n < (1 << 30), "need B >= 0 and n >= 4");
+    require(a && b && c && d && x, "null array");
+    for (const lss_line_bc *e : {lo, hi})
+        require(!e || e->type == LSS_BC_DIRICHLET || e->type == LSS_BC_NEUMANN || e->type == LSS_BC_ROBIN,
+                "end condition type must be LSS_BC_DIRICHLET, LSS_BC_NEUMANN or LSS_BC_ROBIN");
+    if (B == 0)
+        return;
+    const size_t es = fp == LSS_FP64 ? 8 : 4;
+    // unknowns of the folded system
+    const int64_t nf = n - ((lo && lo->type == LSS_BC_DIRICHLET) ? 1 : 0) - ((hi && hi->type == LSS_BC_DIRICHLET) ? 1 : 0);
+    // partitioned solvers for lines of up to NCH * 16 unknowns (rows: while the staged systems fit
+    // 64 KiB of LDS), otherwise one lane per system with a scratch workspace
+    int M = 1;
+    while (M * P_NCH < nf && M < 32)
+        M *= 2;
+    const size_t lds_rows = (size_t)TRI_WAVES_H * 4 * (size_t)n * es;
+    const bool part = M <= 16 && (layout == LSS_LAYOUT_STRIDED || lds_rows <= 65536);
+    if (part) {
+        if (fp == LSS_FP64)
+            tridiag_part<double>(M, layout, B, (int)n, a, b, c, d, x, lds_rows, line_bc<double>(lo, hi, n), st);
+        else
+            tridiag_part<float>(M, layout, B, (int)n, a, b, c, d, x, lds_rows, line_bc<float>(lo, hi, n), st);
+    } else {
+        void *scr = workspace((size_t)B * (size_t)n * es);
+        const unsigned blocks = (unsigned)((B + 255) / 256);
+        if (fp == LSS_FP64)
+            hipLaunchKernelGGL(k_tridiag_lane<double>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
+                               (const double *)a, (const double *)b, (const double *)c, (const double *)d,
+                               (double *)x, (double *)scr, line_bc<double>(lo, hi, n));
+        else
+            hipLaunchKernelGGL(k_tridiag_lane<float>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
+                               (const float *)a, (const float *)b, (const float *)c, (const float *)d,
+                               (float *)x, (float *)scr, line_bc<float>(lo, hi, n));
+    }
+    HIPCHK(hipGetLastError());
 }
 
 }  // namespace
@@ -1795,40 +1852,16 @@ int lss_adi_backward(lss_adi_plan *plan, const void *const *lam_out, void *const
 int lss_tridiag_batched(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b,
                         const void *c, const void *d, void *x, void *stream)
 {
+    return guarded([&] { tridiag_run(fp, layout, B, n, a, b, c, d, nullptr, nullptr, x, (hipStream_t)stream); });
+}
+
+int lss_tridiag_batched_bc(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b,
+                           const void *c, const void *d, const lss_line_bc *lower, const lss_line_bc *upper, void *x,
+                           void *stream)
+{
     return guarded([&] {
-        require(fp == LSS_FP64 || fp == LSS_FP32, "fp must be LSS_FP64 or LSS_FP32");
-        require(layout == LSS_LAYOUT_CONTIGUOUS || layout == LSS_LAYOUT_STRIDED, "unknown layout");
-        require(B >= 0 && n >= 4 && n < (1 << 30), "need B >= 0 and n >= 4");
-        require(a && b && c && d && x, "null array");
-        if (B == 0)
-            return;
-        const size_t es = fp == LSS_FP64 ? 8 : 4;
-        hipStream_t st = (hipStream_t)stream;
-        // partitioned solvers for lines of up to NCH * 16 unknowns (rows: while the staged systems fit
-        // 64 KiB of LDS), otherwise one lane per system with a scratch workspace
-        int M = 1;
-        while (M * P_NCH < n && M < 32)
-            M *= 2;
-        const size_t lds_rows = (size_t)TRI_WAVES_H * 4 * (size_t)n * es;
-        const bool part = M <= 16 && (layout == LSS_LAYOUT_STRIDED || lds_rows <= 65536);
-        if (part) {
-            if (fp == LSS_FP64)
-                tridiag_part<double>(M, layout, B, (int)n, a, b, c, d, x, lds_rows, st);
-            else
-                tridiag_part<float>(M, layout, B, (int)n, a, b, c, d, x, lds_rows, st);
-        } else {
-            void *scr = workspace((size_t)B * (size_t)n * es);
-            const unsigned blocks = (unsigned)((B + 255) / 256);
-            if (fp == LSS_FP64)
-                hipLaunchKernelGGL(k_tridiag_lane<double>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
-                                   (const double *)a, (const double *)b, (const double *)c, (const double *)d,
-                                   (double *)x, (double *)scr);
-            else
-                hipLaunchKernelGGL(k_tridiag_lane<float>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
-                                   (const float *)a, (const float *)b, (const float *)c, (const float *)d,
-                                   (float *)x, (float *)scr);
-        }
-        HIPCHK(hipGetLastError());
+        require(lower && upper, "null end condition");
+        tridiag_run(fp, layout, B, n, a, b, c, d, lower, upper, x, (hipStream_t)stream);
     });
 }
 

@@ -8,8 +8,9 @@
 //     x_k = p_k + q_k xi_{l-1} + r_k xi_l            (k = s..e-1, held in registers)
 // Substituting the last interior point of chunk l and the first interior point of chunk l+1 into the
 // equation of point e gives one tridiagonal equation per chunk in xi_{l-1}, xi_l, xi_{l+1}; that
-// 64-unknown interface system is solved by parallel cyclic reduction (wave shuffles or LDS), after
-// which every chunk finishes its points.
+// 64-unknown interface system is solved by parallel cyclic reduction (wave shuffles or LDS; each step
+// exchanges the neighbours' rows divided by their diagonals), after which every chunk finishes its
+// points.
 #pragma once
 
 #include "lss_adi_device.hpp"
@@ -149,15 +150,35 @@ template <typename T, int M> struct Chunk {
     }
 };
 
-// one cyclic-reduction step; lo.b and hi.b carry the reciprocals of the neighbours' diagonals
-template <typename T> __device__ __forceinline__ Eq<T> pcr_combine(const Eq<T> &me, const Eq<T> &lo, const Eq<T> &hi)
+// A row divided by its diagonal, (a, c, r) / b: what a cyclic-reduction step needs of a neighbour
+// (three values to exchange instead of four).
+template <typename T> struct NEq {
+    T a, c, r;
+};
+template <typename T> __device__ __forceinline__ NEq<T> normalized(const Eq<T> &e)
 {
-    const T al = me.a * lo.b, ga = me.c * hi.b;
+    const T rb = fast_rcp(e.b);
+    NEq<T> n;
+    n.a = e.a * rb;
+    n.c = e.c * rb;
+    n.r = e.r * rb;
+    return n;
+}
+template <typename T> __device__ __forceinline__ NEq<T> zero_neq()
+{
+    NEq<T> n;
+    n.a = n.c = n.r = (T)0;
+    return n;
+}
+
+// one cyclic-reduction step: eliminate the neighbours' unknowns (lo, hi normalised; zero when absent)
+template <typename T> __device__ __forceinline__ Eq<T> pcr_step(const Eq<T> &me, const NEq<T> &lo, const NEq<T> &hi)
+{
     Eq<T> n;
-    n.a = -al * lo.a;
-    n.c = -ga * hi.c;
-    n.b = me.b - al * lo.c - ga * hi.a;
-    n.r = me.r - al * lo.r - ga * hi.r;
+    n.a = -me.a * lo.a;
+    n.c = -me.c * hi.c;
+    n.b = me.b - me.a * lo.c - me.c * hi.a;
+    n.r = me.r - me.a * lo.r - me.c * hi.r;
     return n;
 }
 
@@ -235,21 +256,19 @@ __device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put
         Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
 #pragma unroll
         for (int d = 1; d < NCH; d <<= 1) {
-            const T rb = fast_rcp(e.b);
-            Eq<T> l, h;
-            l.a = __shfl_up(e.a, d);
-            l.b = __shfl_up(rb, d);
-            l.c = __shfl_up(e.c, d);
-            l.r = __shfl_up(e.r, d);
-            h.a = __shfl_down(e.a, d);
-            h.b = __shfl_down(rb, d);
-            h.c = __shfl_down(e.c, d);
-            h.r = __shfl_down(e.r, d);
+            const NEq<T> m = normalized(e);
+            NEq<T> l, h;
+            l.a = __shfl_up(m.a, d);
+            l.c = __shfl_up(m.c, d);
+            l.r = __shfl_up(m.r, d);
+            h.a = __shfl_down(m.a, d);
+            h.c = __shfl_down(m.c, d);
+            h.r = __shfl_down(m.r, d);
             if (lane < d)
-                l = identity_eq<T>();
+                l = zero_neq<T>();
             if (lane + d >= NCH)
-                h = identity_eq<T>();
-            e = pcr_combine(e, l, h);
+                h = zero_neq<T>();
+            e = pcr_step(e, l, h);
         }
         const T xi = e.r * fast_rcp(e.b);
         T xl = __shfl_up(xi, 1);
@@ -306,32 +325,32 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NCH][W], int N, int col, 
         }
         Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
         int buf = 0;
-        sh[0][0][chn][col] = e.a;
-        sh[0][1][chn][col] = fast_rcp(e.b);
-        sh[0][2][chn][col] = e.c;
-        sh[0][3][chn][col] = e.r;
+        {
+            const NEq<T> m = normalized(e);
+            sh[0][0][chn][col] = m.a;
+            sh[0][1][chn][col] = m.c;
+            sh[0][2][chn][col] = m.r;
+        }
         __syncthreads();
 #pragma unroll
         for (int d = 1; d < NCH; d <<= 1) {
-            Eq<T> l = identity_eq<T>(), h = identity_eq<T>();
+            NEq<T> l = zero_neq<T>(), h = zero_neq<T>();
             if (chn >= d) {
                 l.a = sh[buf][0][chn - d][col];
-                l.b = sh[buf][1][chn - d][col];
-                l.c = sh[buf][2][chn - d][col];
-                l.r = sh[buf][3][chn - d][col];
+                l.c = sh[buf][1][chn - d][col];
+                l.r = sh[buf][2][chn - d][col];
             }
             if (chn + d < NCH) {
                 h.a = sh[buf][0][chn + d][col];
-                h.b = sh[buf][1][chn + d][col];
-                h.c = sh[buf][2][chn + d][col];
-                h.r = sh[buf][3][chn + d][col];
+                h.c = sh[buf][1][chn + d][col];
+                h.r = sh[buf][2][chn + d][col];
             }
-            e = pcr_combine(e, l, h);
+            e = pcr_step(e, l, h);
             buf ^= 1;
-            sh[buf][0][chn][col] = e.a;
-            sh[buf][1][chn][col] = fast_rcp(e.b);
-            sh[buf][2][chn][col] = e.c;
-            sh[buf][3][chn][col] = e.r;
+            const NEq<T> m = normalized(e);
+            sh[buf][0][chn][col] = m.a;
+            sh[buf][1][chn][col] = m.c;
+            sh[buf][2][chn][col] = m.r;
             __syncthreads();
         }
         const T xi = e.r * fast_rcp(e.b);

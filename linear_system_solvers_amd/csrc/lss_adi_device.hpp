@@ -51,6 +51,12 @@ template <typename T> struct GArr {
     __device__ __forceinline__ T operator[](uint32_t i) const { return ldg(b, i); }
     __device__ __forceinline__ GArr operator+(uint32_t o) const { return GArr{b + o}; }
 };
+// read-only array in LDS (address space 3, so that accesses are ds_read, not flat)
+template <typename T> using lptr = __attribute__((address_space(3))) T *;
+template <typename T> struct LArr {
+    lptr<const T> b;
+    __device__ __forceinline__ T operator[](uint32_t i) const { return b[i]; }
+};
 template <typename T> struct GAt {
     gptr<const T> b;
     uint32_t q;
@@ -132,6 +138,49 @@ template <typename T> struct Adj {
     int64_t dfac_stride;
     int32_t dfac_sets, nterm, nfx, nfy;
     int32_t tp[NTERM], tq[NTERM], fx[NTERM], fy[NTERM];
+};
+
+// End conditions of a batched line solve, the same for every system of the batch, folded into the
+// end rows as thomas_lu_solver_boundary does (lss_thomas_lu_boundary.hpp:114-205): Dirichlet (the end
+// value is val; the unknowns run from 1 or to n - 2), Neumann (u' = -val, a ghost point folded into
+// the first / last row) and Robin (u' + lin u + val = 0), with h2 = 2 h = 2 / (n - 1).  PLAIN keeps
+// the rows as given (a[0] and c[n-1] ignored).
+enum : int32_t { BC_DIRICHLET = 0, BC_NEUMANN = 1, BC_ROBIN = 2, BC_PLAIN = 3 };
+template <typename T> struct LineBC {
+    int32_t lo, hi;
+    T lo_lin, lo_val, hi_lin, hi_val, h2;
+    __host__ __device__ int start() const { return lo == BC_DIRICHLET ? 1 : 0; }
+    __host__ __device__ int end(int n) const { return hi == BC_DIRICHLET ? n - 2 : n - 1; }
+    // folds the first (row start) or last (row end) equation; operand order of the reference
+    __device__ __forceinline__ void fold(bool first, bool last, T &a, T &b, T &c, T &r) const
+    {
+        if (first) {
+            if (lo == BC_DIRICHLET) {
+                r = r - a * lo_val;
+            } else if (lo == BC_NEUMANN) {
+                c = a + c;
+                r = r - a * (h2 * lo_val);
+            } else if (lo == BC_ROBIN) {
+                b = b + a * (h2 * lo_lin);
+                c = a + c;
+                r = r - a * (h2 * lo_val);
+            }
+            a = (T)0;
+        }
+        if (last) {
+            if (hi == BC_DIRICHLET) {
+                r = r - c * hi_val;
+            } else if (hi == BC_NEUMANN) {
+                a = a + c;
+                r = r + c * (h2 * hi_val);
+            } else if (hi == BC_ROBIN) {
+                a = a + c;
+                b = b - c * (h2 * hi_lin);
+                r = r + c * (h2 * hi_val);
+            }
+            c = (T)0;
+        }
+    }
 };
 
 // one device-to-device copy of n elements (checkpoint and lambda transfers)

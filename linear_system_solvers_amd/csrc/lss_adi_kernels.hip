@@ -275,47 +275,55 @@ __global__ void __launch_bounds__(TILE) k_hv_y3(const Prob<T> *__restrict__ prob
 }
 
 // -------------------------------------------------------------------------------------------------
-// Batched plain tridiagonal systems (one lane per system).  a[0] and c[n-1] are ignored, which is
-// the reference's Thomas LU with Neumann(0)/Neumann(0) ends (thomas_lu_boundary.hpp:136-144, 184-191).
+// Batched tridiagonal systems (one lane per system), the reference's Thomas LU in its operation
+// order with the end rows folded by `bc` (LineBC; PLAIN ignores a[0] and c[n-1]).
 // Layout: contiguous x[k*n + i] (strided == 0) or interleaved x[i*B + k] (strided == 1).
 // -------------------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) k_tridiag_lane(int64_t B, int n, int strided, const T *__restrict__ a,
                                                       const T *__restrict__ b, const T *__restrict__ c,
-                                                      const T *__restrict__ d, T *__restrict__ x, T *__restrict__ scr)
+                                                      const T *__restrict__ d, T *__restrict__ x, T *__restrict__ scr,
+                                                      LineBC<T> bc)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= B)
         return;
     const int64_t base = strided ? k : k * n, st = strided ? B : 1;
-    // t = 0: beta = b0, gamma = (0 + c0) / beta, r = d0 - 0 * 0
-    T beta = b[base];
-    T g = c[base] / beta;
-    T s = d[base] / beta;
-    scr[base] = g;
-    x[base] = s;
-    for (int t = 1; t < n - 1; ++t) {
+    const int s0 = bc.start(), e0 = bc.end(n);
+    // row t with its end folded (thomas_lu_solver.hpp:125-191, thomas_lu_boundary.hpp:114-205)
+    auto row = [&](int t, T &at, T &bt, T &ct, T &rt) {
         const int64_t q = base + t * st;
-        const T at = a[q];
-        beta = b[q] - (at * g);
-        g = c[q] / beta;
-        s = (d[q] - (at * s)) / beta;
-        scr[q] = g;
-        x[q] = s;
-    }
-    {
-        const int64_t q = base + (int64_t)(n - 1) * st;
-        const T at = a[q];
-        beta = b[q] - (at * g);
-        s = (d[q] - (at * s)) / beta;
-        x[q] = s;
+        at = a[q];
+        bt = b[q];
+        ct = c[q];
+        rt = d[q];
+        bc.fold(t == s0, t == e0, at, bt, ct, rt);
+    };
+    T at, bt, ct, rt;
+    row(s0, at, bt, ct, rt);
+    T beta = bt;
+    T g = ct / beta;
+    T s = rt / beta;
+    scr[base + s0 * st] = g;
+    x[base + s0 * st] = s;
+    for (int t = s0 + 1; t <= e0; ++t) {
+        row(t, at, bt, ct, rt);
+        beta = bt - (at * g);
+        g = ct / beta;
+        s = (rt - (at * s)) / beta;
+        scr[base + t * st] = g;
+        x[base + t * st] = s;
     }
     T xv = s;
-    for (int t = n - 2; t >= 0; --t) {
+    for (int t = e0 - 1; t >= s0; --t) {
         const int64_t q = base + t * st;
         xv = x[q] - (scr[q] * xv);
         x[q] = xv;
     }
+    if (bc.lo == BC_DIRICHLET)
+        x[base] = bc.lo_val;
+    if (bc.hi == BC_DIRICHLET)
+        x[base + (int64_t)(n - 1) * st] = bc.hi_val;
 }
 
 // -------------------------------------------------------------------------------------------------
@@ -328,7 +336,7 @@ __global__ void __launch_bounds__(256) k_tridiag_lane(int64_t B, int n, int stri
     template __global__ void k_vsweep<T>(const Prob<T> *, const Tile *, int, int);                            \
     template __global__ void k_hv_y3<T>(const Prob<T> *, const Tile *, int);                                  \
     template __global__ void k_tridiag_lane<T>(int64_t, int, int, const T *, const T *, const T *, const T *, \
-                                               T *, T *);
+                                               T *, T *, LineBC<T>);
 LSS_INST(double)
 LSS_INST(float)
 #undef LSS_INST

@@ -35,14 +35,16 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
 {
     const Tile tl = tiles[swz ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
+    if (step >= p.steps || (stage == 2 && p.scheme == SCHEME_DR))
+        return;  // uniform over the workgroup
+    const GArr<T> cs{coeff_set(p, step)};
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (step >= p.steps || (stage == 2 && p.scheme == SCHEME_DR) || wave >= tl.count)
+    if (wave >= tl.count)
         return;
     const int i = tl.first + wave;
     const int ny = p.ny, N = ny - 2;
     const T *u = (step & 1) ? p.b : p.a;
     T *nxt = (step & 1) ? p.a : p.b;
-    const GArr<T> cs{coeff_set(p, step)};
     const T one = (T)1.0, theta = p.theta;
     const T *in0, *inh0;
     T *out0;
@@ -66,6 +68,7 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
     const T lo = G(p.colA0)[ii], hi = G(p.colA1)[ii];
     const uint32_t jmax = (uint32_t)(N + 1);  // points past the line's end read column N + 1
     auto eq = [&](int jj) -> Eq<T> {
+        // (the Y factors are read from the caches: a copy in LDS per workgroup cost what it saved)
         const Stencil<T> st = stencil_at<T, PAT>(p, cs, i, (int)min((uint32_t)jj, jmax));
         Eq<T> q;
         q.a = (-theta * st.Mt);
@@ -101,13 +104,15 @@ template <typename T, int KIND, int PAT> struct XLine {
     const Prob<T> &p;
     gptr<const T> u, y1, y2;
     GArr<T> cs;
+    LArr<T> xl;  // the X-block factor vectors in LDS (compile-time patterns)
     gptr<T> out;
     int N, ny;
     uint32_t j;
     T lo, hi;
 
-    __device__ __forceinline__ XLine(const Prob<T> &p_, int step, int j_) : p(p_)
+    __device__ __forceinline__ XLine(const Prob<T> &p_, int step, int j_, lptr<const T> xsh) : p(p_)
     {
+        xl = LArr<T>{xsh};
         ny = p.ny;
         N = p.nx - 2;
         j = (uint32_t)j_;
@@ -125,7 +130,11 @@ template <typename T, int KIND, int PAT> struct XLine {
     {
         const T one = (T)1.0, theta = p.theta;
         const uint32_t kmax = (uint32_t)(N + 1);  // points past the line's end read row N + 1
-        const Stencil<T> st = stencil_at<T, PAT>(p, cs, (int)min((uint32_t)k, kmax), (int)j);
+        Stencil<T> st;
+        if constexpr (PAT != PAT_GENERIC)
+            st = stencil_xy<T, PAT>(p, xl, cs + 12 * (uint32_t)(N + 2), (int)min((uint32_t)k, kmax), (int)j);
+        else
+            st = stencil_at<T, PAT>(p, cs, (int)min((uint32_t)k, kmax), (int)j);
         Eq<T> e;
         e.a = (-theta * st.M);
         e.b = (one + theta * st.Z);
@@ -162,11 +171,12 @@ template <typename T, int KIND, int PAT> struct XLine {
 };
 
 template <typename T, int M, int W, int KIND, int PAT>
-__device__ __forceinline__ void xsweep_lines(T (*sh)[4][NCH][W], const Prob<T> &p, const Tile &tl, int step)
+__device__ __forceinline__ void xsweep_lines(T (*sh)[4][NCH][W], const Prob<T> &p, const Tile &tl, int step,
+                                             lptr<const T> xsh)
 {
     const int col = threadIdx.x % W, chn = threadIdx.x / W;
     const bool colok = col < tl.count;
-    const XLine<T, KIND, PAT> ln(p, step, tl.first + (colok ? col : 0));
+    const XLine<T, KIND, PAT> ln(p, step, tl.first + (colok ? col : 0), xsh);
     auto put = [&](int, int k, T x) { ln.store(k, x); };
     block_solve<T, M, W>(sh, ln.N, col, chn, colok, ln, put);
     if (colok && chn == 0)
@@ -179,18 +189,29 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
                                                      const Tile *__restrict__ tiles, int step, int swz)
 {
     __shared__ T sh[2][4][NCH][W];
+    // the X-block factor vectors of a compile-time pattern, copied to LDS once per workgroup (every
+    // point of an S-line reads them at its own row; the Y factors are the same for the whole line)
+    constexpr int NVX = fpat_nvec<PAT>(0);
+    __shared__ T xsh[NVX > 0 ? NVX * (NCH * M + 2) : 1];
     const Tile tl = tiles[swz ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
     if (step >= p.steps || (STAGE == 2 && p.scheme == SCHEME_DR))
         return;  // uniform over the workgroup
+    if constexpr (NVX > 0) {
+        const GArr<T> xb{coeff_set(p, step)};
+        for (int q = threadIdx.x; q < NVX * p.nx; q += blockDim.x)
+            xsh[q] = xb[(uint32_t)q];
+        __syncthreads();
+    }
+    const lptr<const T> xs = (lptr<const T>)xsh;
     if constexpr (STAGE == 1)
-        xsweep_lines<T, M, W, 0, PAT>(sh, p, tl, step);
+        xsweep_lines<T, M, W, 0, PAT>(sh, p, tl, step, xs);
     else if (p.scheme == SCHEME_CS)
-        xsweep_lines<T, M, W, 1, PAT>(sh, p, tl, step);
+        xsweep_lines<T, M, W, 1, PAT>(sh, p, tl, step, xs);
     else if (p.scheme == SCHEME_MCS)
-        xsweep_lines<T, M, W, 2, PAT>(sh, p, tl, step);
+        xsweep_lines<T, M, W, 2, PAT>(sh, p, tl, step, xs);
     else
-        xsweep_lines<T, M, W, 3, PAT>(sh, p, tl, step);
+        xsweep_lines<T, M, W, 3, PAT>(sh, p, tl, step, xs);
 }
 
 }  // namespace lssamd

@@ -63,12 +63,47 @@ __device__ __forceinline__ T factor(const Prob<T> &p, P B, int n, int i)
     }
 }
 
+// number of packed vectors a pattern reads in its X block (first = 0) or Y block (first = 12); 0 for
+// the run-time pattern
+template <int PAT> __host__ __device__ constexpr int fpat_nvec(int first)
+{
+    if constexpr (PAT == PAT_GENERIC) {
+        return 0;
+    } else {
+        int m = -1;
+        for (int s = first; s < first + 12; ++s)
+            m = FPat<PAT>::c[s] > m ? FPat<PAT>::c[s] : m;
+        return m + 1;
+    }
+}
+
 // q(i, j) = X[2K][i] Y[2K][j] + X[2K+1][i] Y[2K+1][j]
-template <typename T, int PAT, int K, typename P>
-__device__ __forceinline__ T sep(const Prob<T> &p, P X, P Y, int i, int j)
+template <typename T, int PAT, int K, typename PX, typename PY>
+__device__ __forceinline__ T sep(const Prob<T> &p, PX X, PY Y, int i, int j)
 {
     return factor<T, PAT, 2 * K>(p, X, p.nx, i) * factor<T, PAT, 12 + 2 * K>(p, Y, p.ny, j) +
            factor<T, PAT, 2 * K + 1>(p, X, p.nx, i) * factor<T, PAT, 12 + 2 * K + 1>(p, Y, p.ny, j);
+}
+
+// Stencil coefficients at (zeta_i, eta_j) of SEPARABLE data whose X and Y blocks are read through
+// the accessors X and Y (global memory, or a copy in LDS): general_heston_equation_coefficients
+// (coefficients.hpp:89-95).
+template <typename T, int PAT, typename PX, typename PY>
+__device__ __forceinline__ Stencil<T> stencil_xy(const Prob<T> &p, PX X, PY Y, int i, int j)
+{
+    Stencil<T> s;
+    const T a = sep<T, PAT, 0>(p, X, Y, i, j), b = sep<T, PAT, 1>(p, X, Y, i, j);
+    const T c = sep<T, PAT, 2>(p, X, Y, i, j), d = sep<T, PAT, 3>(p, X, Y, i, j);
+    const T e = sep<T, PAT, 4>(p, X, Y, i, j), f = sep<T, PAT, 5>(p, X, Y, i, j);
+    const T two = (T)2.0, half = (T)0.5;
+    s.M = (p.alpha * a - p.delta * d);
+    s.Mt = (p.beta * b - p.ni * e);
+    s.P = (p.alpha * a + p.delta * d);
+    s.Pt = (p.beta * b + p.ni * e);
+    s.Z = (two * p.alpha * a - half * p.rho * f);
+    s.W = (two * p.beta * b - half * p.rho * f);
+    s.C = c;
+    return s;
 }
 
 // Stencil coefficients at (zeta_i, eta_j): general_heston_equation_coefficients (coefficients.hpp:89-95).
@@ -86,18 +121,7 @@ __device__ __forceinline__ Stencil<T> stencil_at(const Prob<T> &p, P cs, int i, 
         s.W = cs[5 * nxy + q];
         s.C = cs[6 * nxy + q];
     } else {
-        const P X = cs, Y = cs + 12 * p.nx;
-        const T a = sep<T, PAT, 0>(p, X, Y, i, j), b = sep<T, PAT, 1>(p, X, Y, i, j);
-        const T c = sep<T, PAT, 2>(p, X, Y, i, j), d = sep<T, PAT, 3>(p, X, Y, i, j);
-        const T e = sep<T, PAT, 4>(p, X, Y, i, j), f = sep<T, PAT, 5>(p, X, Y, i, j);
-        const T two = (T)2.0, half = (T)0.5;
-        s.M = (p.alpha * a - p.delta * d);
-        s.Mt = (p.beta * b - p.ni * e);
-        s.P = (p.alpha * a + p.delta * d);
-        s.Pt = (p.beta * b + p.ni * e);
-        s.Z = (two * p.alpha * a - half * p.rho * f);
-        s.W = (two * p.beta * b - half * p.rho * f);
-        s.C = c;
+        s = stencil_xy<T, PAT>(p, cs, cs + 12 * p.nx, i, j);
     }
     return s;
 }
