@@ -106,12 +106,15 @@ template <typename T, int PAT, int FIX> struct TermSums {
         return code >= 0 ? (double)fY[code * ny + j] : 1.0;
     }
 
-    // one point (i, j) with d J / d (a, b, c, d, e, f) = g
-    __device__ __forceinline__ void add(const Adj<T> &A, int i, int j, const T (&g)[6])
+    // one point (i, j) with d J / d (a, b, c, d, e, f) = g; QM: bit q set when g[q] may be nonzero
+    // (the caller's equations do not depend on the other coefficients, whose terms are skipped)
+    template <int QM = 63> __device__ __forceinline__ void add(const Adj<T> &A, int i, int j, const T (&g)[6])
     {
         if constexpr (PAT == PAT_GENERIC) {  // compact code: s[p] sums parameter p directly
 #pragma unroll 1
             for (int t = 0; t < n; ++t) {
+                if (!((QM >> A.tq[t]) & 1))
+                    continue;
                 const double v = (double)pick6(A.tq[t], g) * yf(A.fy[t], j) * xf(A.fx[t], i);
                 const int k = A.tp[t];
 #pragma unroll
@@ -124,7 +127,7 @@ template <typename T, int PAT, int FIX> struct TermSums {
             constexpr int t = decltype(tt)::value;
             if constexpr (PAT != PAT_GENERIC) {
                 using TP = TPat<PAT>;
-                if constexpr (t < TP::n) {
+                if constexpr (t < TP::n && ((QM >> TP::tq[t]) & 1)) {
                     if constexpr (FIX == 0 || term_rep<PAT, FIX>(t) == t) {
                         double v = (double)g[TP::tq[t]];
                         if constexpr (FIX != 1 && TP::fy[t] >= 0)
@@ -288,33 +291,36 @@ template <typename T> __device__ __forceinline__ bool inside(const Prob<T> &p, i
 
 // lambda(Y2) at (i, j) (interior row i, any column): gathered from the X2 equations (CS/MCS: Y2 in
 // the Delta terms; HV: rhs_intermed_4 and the explicit stage H, whose multiplier is also lx2) and,
-// for HV, from the V2 equations (in = Y2).  S(di, dj) gives the stencil at (i + di, j + dj).
-template <typename T, typename SF>
+// for HV, from the V2 equations (in = Y2).  S(di, dj) gives the stencil at (i + di, j + dj).  SCH is
+// the scheme (compile time).  Every neighbour is read (at clamped indices) and weighted by whether
+// it is an equation point, so that all loads are issued together.
+template <int SCH, typename T, typename SF>
 __device__ __forceinline__ T lam_y2(const Prob<T> &p, const Adj<T> &A, SF &&S, int i, int j)
 {
-    const int ny = p.ny;
+    constexpr bool hv = SCH == SCHEME_HV, mcs = SCH == SCHEME_MCS;
+    const int ny = p.ny, P = p.nx - 1;
     const gptr<const T> lx2 = G((const T *)A.lx2), lv2 = G((const T *)A.lv2);
-    const bool hv = p.scheme == SCHEME_HV, mcs = p.scheme == SCHEME_MCS;
     T v = (T)0;
 #pragma unroll
     for (int di = -1; di <= 1; ++di) {
 #pragma unroll
         for (int dj = -1; dj <= 1; ++dj) {
             const int ei = i + di, ej = j + dj;
-            if (!inside(p, ei, ej))
-                continue;
-            const Stencil<T> s = S(di, dj);
-            const int q = (int)ei * ny + ej;
-            if (hv) {
+            const bool in = inside(p, ei, ej);
+            const int q = min(max(ei, 0), P) * ny + min(max(ej, 0), ny - 1);
+            const Stencil<T> s = S(di, dj);  // not defined outside the equation points: selected away
+            T t;
+            if constexpr (hv) {
                 T w = w_h(p, s, -di, -dj, 2);
                 if (dj == 0)
                     w += w_r4(p, s, -di);
-                v += w * lx2[q];
+                t = w * lx2[q];
                 if (di == 0)
-                    v += w_rv(p, s, -dj) * lv2[q];
+                    t += w_rv(p, s, -dj) * lv2[q];
             } else {
-                v += w_r3(p, s, -di, -dj, mcs, false) * lx2[q];
+                t = w_r3(p, s, -di, -dj, mcs, false) * lx2[q];
             }
+            v += in ? t : (T)0;
         }
     }
     return v;
@@ -409,7 +415,8 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
             g.Pt = theta * xv * (yrow[rc(k, 1)] - irow[rc(k, 1)]);
             T gq[6];
             sg_to_q(p, g, gq);
-            ts.add(A, i, rc(k, 0), gq);
+            // the v-rows depend on M~, W, P~ only: on b, e and f
+            ts.template add<(1 << 1) | (1 << 4) | (1 << 5)>(A, i, rc(k, 0), gq);
         };
         wave_solve<T, M, true>(N, lane, eq, put);
         int s0, cnt;
@@ -536,7 +543,9 @@ __device__ __forceinline__ void adj_x_lines(T (*sh)[4][NCH][W], const Prob<T> &p
         }
         T gq[6];
         sg_to_q(p, g, gq);
-        ts.add(A, (int)min((uint32_t)k, kmax), j, gq);
+        // CS stage 2 reads M, Z, P and C only: a, c, d and f
+        constexpr int QM = KIND == 3 ? (1 << 0) | (1 << 2) | (1 << 3) | (1 << 5) : 63;
+        ts.template add<QM>(A, (int)min((uint32_t)k, kmax), j, gq);
     };
     block_solve<T, M, W, true>(sh, N, col, chn, colok, eq, put);
     if (colok) {
@@ -651,10 +660,19 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
         st.C = sst[6][lr][lc];
         return st;
     };
+    // the scheme as a compile-time constant (uniform over the workgroup)
+    auto with_scheme = [&](auto &&f) {
+        switch (p.scheme) {
+        case SCHEME_DR: f(std::integral_constant<int, SCHEME_DR>{}); break;
+        case SCHEME_CS: f(std::integral_constant<int, SCHEME_CS>{}); break;
+        case SCHEME_MCS: f(std::integral_constant<int, SCHEME_MCS>{}); break;
+        default: f(std::integral_constant<int, SCHEME_HV>{}); break;
+        }
+    };
     if (run && tr < tl.count && c < ny) {
         if (WHAT == 1) {
             if (r >= 1 && r <= P - 1)
-                G(A.ly2)[(int)r * ny + c] = lam_y2(p, A, S, r, c);
+                with_scheme([&](auto sc) { G(A.ly2)[(int)r * ny + c] = lam_y2<decltype(sc)::value>(p, A, S, r, c); });
         } else {
             const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
             const gptr<T> lam = G((step & 1) ? A.lb : A.la);
@@ -664,8 +682,7 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
             const gptr<const uint32_t> inv1o = G(A.inv1o), inv1i = G(A.inv1i), inv2o = G(A.inv2o),
                                        inv2i = G(A.inv2i);
             const T theta = p.theta, delta = p.delta, ni = p.ni, rho = p.rho;
-            const int sch = p.scheme;
-            const bool dr = sch == SCHEME_DR, hv = sch == SCHEME_HV, mcs = sch == SCHEME_MCS;
+            const bool dr = p.scheme == SCHEME_DR, hv = p.scheme == SCHEME_HV;
             // lambda of colA0[i]: every v-line whose lower end read it (through idx1)
             auto lamA = [&](int i) -> T {
                 T v = (T)0;
@@ -676,27 +693,33 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
                 }
                 return v;
             };
+            // the stencils of the right-hand sides that read U_s at (r, c); every neighbour is read
+            // (at clamped indices) and weighted by whether it is an equation point
             T v = (T)0;
+            with_scheme([&](auto sc) {
+                constexpr int SCH = decltype(sc)::value;
+                constexpr bool DR = SCH == SCHEME_DR, HV = SCH == SCHEME_HV, MCS = SCH == SCHEME_MCS;
 #pragma unroll
-            for (int di = -1; di <= 1; ++di) {
+                for (int di = -1; di <= 1; ++di) {
 #pragma unroll
-                for (int dj = -1; dj <= 1; ++dj) {
-                    const int ei = r + di, ej = c + dj;
-                    if (!inside(p, ei, ej))
-                        continue;
-                    const Stencil<T> st = S(di, dj);
-                    const int q = (int)ei * ny + ej;
-                    if (di == 0 && !hv)
-                        v += w_rv(p, st, -dj) * lv2[q];
-                    if (di == 0 && !dr)
-                        v += w_rv(p, st, -dj) * lv1[q];
-                    if (!dr && !hv)
-                        v += w_r3(p, st, -di, -dj, mcs, true) * lx2[q];
-                    if (hv)
-                        v += w_h(p, st, -di, -dj, 1) * lx2[q];
-                    v += w_r1(p, st, -di, -dj) * lx1[q];
+                    for (int dj = -1; dj <= 1; ++dj) {
+                        const int ei = r + di, ej = c + dj;
+                        const bool in = inside(p, ei, ej);
+                        const int q = min(max(ei, 0), P) * ny + min(max(ej, 0), ny - 1);
+                        const Stencil<T> st = S(di, dj);  // not defined outside: selected away
+                        T t = w_r1(p, st, -di, -dj) * lx1[q];
+                        if (di == 0 && !HV)
+                            t += w_rv(p, st, -dj) * lv2[q];
+                        if (di == 0 && !DR)
+                            t += w_rv(p, st, -dj) * lv1[q];
+                        if constexpr (!DR && !HV)
+                            t += w_r3(p, st, -di, -dj, MCS, true) * lx2[q];
+                        if constexpr (HV)
+                            t += w_h(p, st, -di, -dj, 1) * lx2[q];
+                        v += in ? t : (T)0;
+                    }
                 }
-            }
+            });
             // Dirichlet ends of the S-lines j with idx2[j] == c (implicit_kernel.hpp:86-101)
             if (r == 0 || r == P) {
                 const int ei = (r == 0) ? 1 : P - 1;
@@ -745,7 +768,7 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
                     const T ge = la * ni * (-(T)3 * ur[0] + (T)4 * ur[1] - ur[2]);
                     const T gf = la * rho * ur[0];
                     const T gq[6] = {(T)0, (T)0, (T)0, gd, ge, gf};
-                    ts.add(A, r, 0, gq);
+                    ts.template add<(1 << 3) | (1 << 4) | (1 << 5)>(A, r, 0, gq);
                     ts.finish(A, r, 0, acc);
                 }
             }
