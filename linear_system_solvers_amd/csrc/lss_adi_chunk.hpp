@@ -208,10 +208,10 @@ template <typename T> __device__ __forceinline__ Eq<T> identity_eq()
 //   chunk without per-point branches (so that the caller's loads can be issued together); then
 //   k > N marks a discarded point whose x is arbitrary (possibly not finite).
 // -------------------------------------------------------------------------------------------------
-__device__ __forceinline__ int chunk_len(int N)
+__device__ __forceinline__ int chunk_len(int N, int nch = NCH)
 {
     int c = 1;
-    while (c * NCH < N)
+    while (c * nch < N)
         c <<= 1;
     return c;
 }
@@ -289,15 +289,15 @@ __device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put
     });
 }
 
-// W lines per workgroup of W * NCH threads (thread = (chunk chn, line col)); the interface
-// system is reduced through LDS `sh`.  Every thread of the workgroup must call it (it holds
-// barriers) with the same N (chunk_len(N) <= M); threads of lines that do not exist pass
+// W lines per workgroup of W * NC threads (thread = (chunk chn, line col)), NC chunks per line; the
+// interface system is reduced through LDS `sh`.  Every thread of the workgroup must call it (it
+// holds barriers) with the same N (chunk_len(N, NC) <= M); threads of lines that do not exist pass
 // active = false (their eq is not called).
-template <typename T, int M, int W, bool PUT_ALL = false, typename EqF, typename PutF>
-__device__ __forceinline__ void block_solve(T (*sh)[4][NCH][W], int N, int col, int chn, bool active, EqF &&eq,
+template <typename T, int M, int W, bool PUT_ALL = false, int NC = NCH, typename EqF, typename PutF>
+__device__ __forceinline__ void block_solve(T (*sh)[4][NC][W], int N, int col, int chn, bool active, EqF &&eq,
                                             PutF &&put)
 {
-    with_chunk_len<M>(chunk_len(N), [&](auto cc) {
+    with_chunk_len<M>(chunk_len(N, NC), [&](auto cc) {
         constexpr int C = decltype(cc)::value;
         const int s = 1 + chn * C;
         Chunk<T, C> ch;
@@ -318,7 +318,7 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NCH][W], int N, int col, 
         }
         __syncthreads();
         T nPF = (T)0, nQF = (T)0, nRF = (T)0;
-        if (!last_real && chn + 1 < NCH) {
+        if (!last_real && chn + 1 < NC) {
             nPF = sh[1][0][chn + 1][col];
             nQF = sh[1][1][chn + 1][col];
             nRF = sh[1][2][chn + 1][col];
@@ -333,14 +333,14 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NCH][W], int N, int col, 
         }
         __syncthreads();
 #pragma unroll
-        for (int d = 1; d < NCH; d <<= 1) {
+        for (int d = 1; d < NC; d <<= 1) {
             NEq<T> l = zero_neq<T>(), h = zero_neq<T>();
             if (chn >= d) {
                 l.a = sh[buf][0][chn - d][col];
                 l.c = sh[buf][1][chn - d][col];
                 l.r = sh[buf][2][chn - d][col];
             }
-            if (chn + d < NCH) {
+            if (chn + d < NC) {
                 h.a = sh[buf][0][chn + d][col];
                 h.c = sh[buf][1][chn + d][col];
                 h.r = sh[buf][2][chn + d][col];
@@ -376,9 +376,9 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NCH][W], int N, int col, 
 
 // First point and point count of the chunk of thread position `pos` (lane or chunk index) of a line
 // of N interior unknowns, as the solvers above partition it.
-__device__ __forceinline__ void chunk_span(int N, int pos, int &s, int &cnt)
+__device__ __forceinline__ void chunk_span(int N, int pos, int &s, int &cnt, int nch = NCH)
 {
-    const int c = chunk_len(N);
+    const int c = chunk_len(N, nch);
     s = 1 + pos * c;
     cnt = max(0, min(c, N - s + 1));
 }

@@ -170,25 +170,25 @@ template <typename T, int KIND, int PAT> struct XLine {
     }
 };
 
-template <typename T, int M, int W, int KIND, int PAT>
-__device__ __forceinline__ void xsweep_lines(T (*sh)[4][NCH][W], const Prob<T> &p, const Tile &tl, int step,
+template <typename T, int M, int W, int KIND, int PAT, int NC>
+__device__ __forceinline__ void xsweep_lines(T (*sh)[4][NC][W], const Prob<T> &p, const Tile &tl, int step,
                                              lptr<const T> xsh)
 {
     const int col = threadIdx.x % W, chn = threadIdx.x / W;
     const bool colok = col < tl.count;
     const XLine<T, KIND, PAT> ln(p, step, tl.first + (colok ? col : 0), xsh);
     auto put = [&](int, int k, T x) { ln.store(k, x); };
-    block_solve<T, M, W>(sh, ln.N, col, chn, colok, ln, put);
+    block_solve<T, M, W, false, NC>(sh, ln.N, col, chn, colok, ln, put);
     if (colok && chn == 0)
         ln.store_ends();
 }
 
-// W S-lines per workgroup, their chunks spread over the workgroup (block_solve)
-template <typename T, int M, int STAGE, int W, int PAT>
-__global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__ probs,
-                                                     const Tile *__restrict__ tiles, int step, int swz)
+// W S-lines per workgroup of NC chunks each, the chunks spread over the workgroup (block_solve)
+template <typename T, int M, int STAGE, int W, int PAT, int NC>
+__global__ void __launch_bounds__(W *NC) k_xsweep_p(const Prob<T> *__restrict__ probs,
+                                                    const Tile *__restrict__ tiles, int step, int swz)
 {
-    __shared__ T sh[2][4][NCH][W];
+    __shared__ T sh[2][4][NC][W];
     // the X-block factor vectors of a compile-time pattern, copied to LDS once per workgroup (every
     // point of an S-line reads them at its own row; the Y factors are the same for the whole line)
     constexpr int NVX = fpat_nvec<PAT>(0);
@@ -205,13 +205,13 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
     }
     const lptr<const T> xs = (lptr<const T>)xsh;
     if constexpr (STAGE == 1)
-        xsweep_lines<T, M, W, 0, PAT>(sh, p, tl, step, xs);
+        xsweep_lines<T, M, W, 0, PAT, NC>(sh, p, tl, step, xs);
     else if (p.scheme == SCHEME_CS)
-        xsweep_lines<T, M, W, 1, PAT>(sh, p, tl, step, xs);
+        xsweep_lines<T, M, W, 1, PAT, NC>(sh, p, tl, step, xs);
     else if (p.scheme == SCHEME_MCS)
-        xsweep_lines<T, M, W, 2, PAT>(sh, p, tl, step, xs);
+        xsweep_lines<T, M, W, 2, PAT, NC>(sh, p, tl, step, xs);
     else
-        xsweep_lines<T, M, W, 3, PAT>(sh, p, tl, step, xs);
+        xsweep_lines<T, M, W, 3, PAT, NC>(sh, p, tl, step, xs);
 }
 
 }  // namespace lssamd
@@ -219,16 +219,18 @@ __global__ void __launch_bounds__(W *NCH) k_xsweep_p(const Prob<T> *__restrict__
 // Instantiations (one translation unit per precision and coefficient pattern, so that the library
 // builds in parallel): LSS_INST_PART(T, PAT) instantiates every sweep of precision T and pattern
 // PAT for the chunk bounds the plans use.
+#define LSS_INST_X(T, M, S, W, PAT, NC) \
+    template __global__ void k_xsweep_p<T, M, S, W, PAT, NC>(const Prob<T> *, const Tile *, int, int);
 #define LSS_INST_P(T, M, PAT)                                                                                  \
     template __global__ void k_vsweep_p<T, M, 4, PAT>(const Prob<T> *, const Tile *, int, int, int);            \
     template __global__ void k_vsweep_p<T, M, 16, PAT>(const Prob<T> *, const Tile *, int, int, int);           \
-    template __global__ void k_xsweep_p<T, M, 1, 16, PAT>(const Prob<T> *, const Tile *, int, int);             \
-    template __global__ void k_xsweep_p<T, M, 2, 16, PAT>(const Prob<T> *, const Tile *, int, int);             \
-    template __global__ void k_xsweep_p<T, M, 1, 8, PAT>(const Prob<T> *, const Tile *, int, int);              \
-    template __global__ void k_xsweep_p<T, M, 2, 8, PAT>(const Prob<T> *, const Tile *, int, int);              \
-    template __global__ void k_xsweep_p<T, M, 1, 4, PAT>(const Prob<T> *, const Tile *, int, int);              \
-    template __global__ void k_xsweep_p<T, M, 2, 4, PAT>(const Prob<T> *, const Tile *, int, int);
+    LSS_INST_X(T, M, 1, 16, PAT, 64) LSS_INST_X(T, M, 2, 16, PAT, 64) LSS_INST_X(T, M, 1, 8, PAT, 64)          \
+    LSS_INST_X(T, M, 2, 8, PAT, 64) LSS_INST_X(T, M, 1, 4, PAT, 64) LSS_INST_X(T, M, 2, 4, PAT, 64)
+// stage 2 with 32 chunks per line (plans whose S-lines all have at most 512 points; chunk bound 2M)
+#define LSS_INST_P32(T, M, PAT) \
+    LSS_INST_X(T, M, 2, 16, PAT, 32) LSS_INST_X(T, M, 2, 8, PAT, 32) LSS_INST_X(T, M, 2, 4, PAT, 32)
 #define LSS_INST_PART(T, PAT)                                                                                  \
     namespace lssamd {                                                                                         \
     LSS_INST_P(T, 1, PAT) LSS_INST_P(T, 2, PAT) LSS_INST_P(T, 4, PAT) LSS_INST_P(T, 8, PAT) LSS_INST_P(T, 16, PAT) \
+    LSS_INST_P32(T, 2, PAT) LSS_INST_P32(T, 4, PAT) LSS_INST_P32(T, 8, PAT) LSS_INST_P32(T, 16, PAT)          \
     }
