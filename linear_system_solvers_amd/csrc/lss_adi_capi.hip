@@ -292,6 +292,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             while (pmy < my)
                 pmy *= 2;
             adj_ok = pmx <= 16 && pmy <= 16;
+            if (pmx >= 16)
+                xw = 8;  // long S-lines: at most 8 per workgroup
         }
         order.resize(n);
         std::iota(order.begin(), order.end(), 0);
@@ -491,20 +493,30 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     }
 
     // partitioned sweeps with the chosen tile/lane mapping (xw, vr) and chunk bound (pmx, pmy)
-    // Stage 2 cuts its S-lines into 32 chunks (of at most 2M points) when every S-line of the plan has
-    // at most 512 points: measured on the Heston book, 11 % faster than 64 chunks; stage 1 is faster
-    // with 64.
+    // Chunks per S-line.  Plans whose S-lines all have at most 512 points (M <= 8): 64 in stage 1 and
+    // 32 (of at most 2M points) in stage 2 -- measured on the Heston book, stage 2 is 11 % faster with
+    // 32, stage 1 faster with 64.  Plans with longer S-lines (M = 16): 128 chunks of at most 8 points
+    // in both stages, so that a single large problem has enough wavefronts (the 1024 x 512 grid: four
+    // per CU instead of two); such plans take 8 or 4 S-lines per workgroup.
     template <int M, int W, int PAT> void launch_x_p(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
         const int tb = W == 16 ? TB_X : (W == 8 ? TB_X8 : TB_X4);
         const dim3 g((unsigned)prefix[tb][na]);
         const Tile *tx = (const Tile *)tiles_dev.p + tile_off[tb];
-        if (stage == 1)
+        if constexpr (M >= 16) {
+            if constexpr (W <= 8) {
+                if (stage == 1)
+                    hipLaunchKernelGGL((k_xsweep_p<T, 8, 1, W, PAT, 128>), g, dim3(W * 128), 0, st, pd, tx, s, swz);
+                else
+                    hipLaunchKernelGGL((k_xsweep_p<T, 8, 2, W, PAT, 128>), g, dim3(W * 128), 0, st, pd, tx, s, swz);
+            } else {
+                throw lss_error(LSS_ERR_INVALID, "S-lines of more than 512 points take 8 or 4 lines per workgroup");
+            }
+        } else if (stage == 1) {
             hipLaunchKernelGGL((k_xsweep_p<T, M, 1, W, PAT, 64>), g, dim3(W * 64), 0, st, pd, tx, s, swz);
-        else if constexpr (M <= 8)
+        } else {
             hipLaunchKernelGGL((k_xsweep_p<T, 2 * M, 2, W, PAT, 32>), g, dim3(W * 32), 0, st, pd, tx, s, swz);
-        else
-            hipLaunchKernelGGL((k_xsweep_p<T, M, 2, W, PAT, 64>), g, dim3(W * 64), 0, st, pd, tx, s, swz);
+        }
     }
     template <int M, int W> void launch_x_m(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
@@ -605,6 +617,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     {
         require((x == 16 || x == 8 || x == 4) && (v == 4 || v == 16) && (z == 0 || z == 1),
                 "mapping must be (16, 8 or 4; 4 or 16; 0 or 1)");
+        require(x != 16 || pmx < 16, "S-lines of more than 512 points take 8 or 4 lines per workgroup");
         drop_graph();
         xw = x;
         vr = v;
@@ -624,10 +637,12 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         float best = 1e30f;
-        int bx = 16, bv = 4, bs = 0;
+        int bx = pmx >= 16 ? 8 : 16, bv = 4, bs = 0;
         for (int cz : {0, 1})
             for (int cx : {16, 8, 4})
                 for (int cv : {4, 16}) {
+                if (cx == 16 && pmx >= 16)
+                    continue;  // long S-lines: at most 8 per workgroup
                 xw = cx;
                 vr = cv;
                 swz = cz;

@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(W *NC) k_xsweep_p(const Prob<T> *__restrict__ 
     // the X-block factor vectors of a compile-time pattern, copied to LDS once per workgroup (every
     // point of an S-line reads them at its own row; the Y factors are the same for the whole line)
     constexpr int NVX = fpat_nvec<PAT>(0);
-    __shared__ T xsh[NVX > 0 ? NVX * (NCH * M + 2) : 1];
+    __shared__ T xsh[NVX > 0 ? NVX * (NC * M + 2) : 1];
     const Tile tl = tiles[swz ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
     if (step >= p.steps || (STAGE == 2 && p.scheme == SCHEME_DR))
@@ -229,8 +229,13 @@ __global__ void __launch_bounds__(W *NC) k_xsweep_p(const Prob<T> *__restrict__ 
 // stage 2 with 32 chunks per line (plans whose S-lines all have at most 512 points; chunk bound 2M)
 #define LSS_INST_P32(T, M, PAT) \
     LSS_INST_X(T, M, 2, 16, PAT, 32) LSS_INST_X(T, M, 2, 8, PAT, 32) LSS_INST_X(T, M, 2, 4, PAT, 32)
+// both stages with 128 chunks per line (plans with S-lines of more than 512 points; chunk bound 8)
+#define LSS_INST_P128(T, PAT) \
+    LSS_INST_X(T, 8, 1, 8, PAT, 128) LSS_INST_X(T, 8, 2, 8, PAT, 128) LSS_INST_X(T, 8, 1, 4, PAT, 128) \
+    LSS_INST_X(T, 8, 2, 4, PAT, 128)
 #define LSS_INST_PART(T, PAT)                                                                                  \
     namespace lssamd {                                                                                         \
     LSS_INST_P(T, 1, PAT) LSS_INST_P(T, 2, PAT) LSS_INST_P(T, 4, PAT) LSS_INST_P(T, 8, PAT) LSS_INST_P(T, 16, PAT) \
     LSS_INST_P32(T, 2, PAT) LSS_INST_P32(T, 4, PAT) LSS_INST_P32(T, 8, PAT) LSS_INST_P32(T, 16, PAT)          \
+    LSS_INST_P128(T, PAT)                                                                                      \
     }

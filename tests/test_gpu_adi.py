@@ -236,6 +236,26 @@ def test_every_tile_mapping_gives_identical_results():
         plan.set_mapping(32, 4)
 
 
+def test_long_s_lines_take_at_most_8_per_workgroup_and_every_mapping_agrees(oracle):
+    """S-lines of more than 512 points use 128 chunks; 16 lines per workgroup is refused."""
+    m = _stable(700, 40, steps=4, splitting=2, coeff_mode=L.COEFF_SEPARABLE, line_solver=L.LINE_PARTITIONED)
+    plan = L.Plan.from_models([m])
+    assert plan.autotune()[0] in (8, 4)
+    with pytest.raises(L.LssError):
+        plan.set_mapping(16, 4)
+    ref = None
+    for x in (8, 4):
+        for v in (4, 16):
+            plan.set_mapping(x, v, 1)
+            out = plan.run_host()[0]
+            if ref is None:
+                ref = out
+            np.testing.assert_array_equal(out, ref)
+    c = oracle_solve(oracle, m)
+    print(f"700x40 long S-lines: normwise {_normwise(ref, c):.2e}", flush=True)
+    assert _normwise(ref, c) <= 1e-12
+
+
 def test_reference_models_use_the_specialized_factor_patterns():
     """Heston (r != 0) and SABR (r = 0) separable data match the compile-time patterns exactly."""
     h = L.Plan.from_models([_stable(40, 20, coeff_mode=L.COEFF_SEPARABLE, line_solver=L.LINE_PARTITIONED)])
