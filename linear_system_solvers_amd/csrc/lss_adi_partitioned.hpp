@@ -41,7 +41,8 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (wave >= tl.count)
         return;
-    const int i = tl.first + wave;
+    // the row is the same for the whole wavefront: scalar registers, scalar loads of its X factors
+    const int i = __builtin_amdgcn_readfirstlane(tl.first + wave);
     const int ny = p.ny, N = ny - 2;
     const T *u = (step & 1) ? p.b : p.a;
     T *nxt = (step & 1) ? p.a : p.b;
