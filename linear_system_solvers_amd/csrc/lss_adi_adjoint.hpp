@@ -364,7 +364,7 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
     };
     const bool run = wg_run && wave < tl.count;
     if (run) {
-        const int i = tl.first + wave;
+        const int i = __builtin_amdgcn_readfirstlane(tl.first + wave);  // the same for the wavefront
         const int nx = p.nx, ny = p.ny, N = ny - 2, P = nx - 1;
         const T *u = (step & 1) ? p.b : p.a;
         const T *un = (step & 1) ? p.a : p.b;
