@@ -4,9 +4,10 @@
 Headline workload (one "step" = one full solve of the book, inputs resident in HBM):
   the book of 2048 Heston calls, seed 3: NS ~ U{64..512}, Nv ~ U{32..256}, steps ~ U{50..500},
   K ~ U[50,150], T ~ U[0.1,3], kappa ~ U[0.5,5], theta_v ~ U[0.02,0.4], sigma ~ U[0.1,1],
-  rho ~ U[-0.9,0], r ~ U[0,0.05], scheme ~ U{DR, CS, MCS, HV}; S in [0, 2K], v in [0, 1.2],
-  hints (K, 10, 50, Nonuniform), theta = 0.5, zeta = 0.5, fp64, separable coefficients, all problems
-  stepped together in one plan whose time loop is replayed from a hipGraph.
+  rho ~ U[-0.9,0], r ~ U[0,0.05], scheme ~ U{DR, CS, MCS, HV}; the call of the single-problem
+  case on each problem's own sizes: S in [0, 2K], v in [0, 1.2], uniform grid, theta = zeta = 0.5;
+  fp64, separable coefficients, all problems stepped together in one plan whose time loop is
+  replayed from a hipGraph.
 metric: ADI grid-point-steps/s (sum over problems of nx*ny*steps) of the batched book.
 
 Also reported: the plain per-problem stepper on a sample of the same book, the batched tridiagonal
@@ -70,8 +71,7 @@ def book_models(count: int = 2048, seed: int = 3, coeff_mode: int = L.COEFF_SEPA
         scheme = int(rng.integers(0, 4))
         models.append(L.Model(kind=L.MODEL_HESTON, strike=K, rate=r, sig_sig=sigma, sig_kappa=kappa,
                               sig_theta=theta_v, rho=rho, maturity=T, x_lo=0.0, x_hi=2.0 * K, y_lo=0.0, y_hi=1.2,
-                              t_lo=0.0, t_hi=T, nx=nx, ny=ny, nt=steps + 1, grid=L.NONUNIFORM,
-                              accumulation_point=K, alpha_scale=10.0, beta_scale=50.0, splitting=scheme,
+                              t_lo=0.0, t_hi=T, nx=nx, ny=ny, nt=steps + 1, grid=L.UNIFORM, splitting=scheme,
                               weighting=0.5, implicit_scheme=L.CRANK_NICOLSON, tridiagonal=L.THOMAS_LU,
                               direction=L.BACKWARD, coeff_mode=coeff_mode, line_solver=line_solver))
     return models
@@ -171,55 +171,149 @@ def bench_config0(torch, log):
 
 
 def parity_sample(models, outs, log, count=8):
+    """The smallest book problems of each scheme (so the CPU check stays short) against the oracle.
+    Problems whose oracle solution is not finite or grows past ten times S_max (the scheme is unstable
+    there at the drawn step count) are passed over; check_all covers them."""
     from oracle import oracle as O
     from tests.helpers import oracle_solve
-    # smallest problems of each scheme, so the CPU check stays short
     order = sorted(range(len(models)), key=lambda q: models[q].nx * models[q].ny * models[q].steps)
-    picked, seen = [], {}
+    worst, worst_norm, seen, picked, skipped = 0.0, 0.0, {}, 0, 0
     for q in order:
         s = models[q].splitting
-        if seen.get(s, 0) < count // 4:
-            picked.append(q)
-            seen[s] = seen.get(s, 0) + 1
-        if len(picked) >= count:
-            break
-    worst, worst_norm = 0.0, 0.0
-    for q in picked:
+        if seen.get(s, 0) >= count // 4:
+            continue
         c = oracle_solve(O, models[q])
-        worst = max(worst, rel_err(outs[q], c))
-        worst_norm = max(worst_norm, float(np.abs(outs[q] - c).max() / np.abs(c).max()))
-    log(f"parity sample: {len(picked)} book problems vs oracle, max per-value rel err {worst:.2e}, "
-        f"max normwise {worst_norm:.2e}")
-    return worst, worst_norm, len(picked)
+        if not np.isfinite(c).all() or np.abs(c).max() > 10.0 * models[q].x_hi:
+            skipped += 1
+            continue
+        g = outs[q]
+        worst = max(worst, rel_err(g, c)) if np.isfinite(g).all() else float("inf")
+        worst_norm = max(worst_norm, float(np.abs(g - c).max() / np.abs(c).max())) if np.isfinite(g).all() \
+            else float("inf")
+        seen[s] = seen.get(s, 0) + 1
+        picked += 1
+        if picked >= count:
+            break
+    log(f"parity sample: {picked} book problems vs oracle ({skipped} unstable ones passed over), "
+        f"max per-value rel err {worst:.2e}, max normwise {worst_norm:.2e}")
+    return worst, worst_norm, picked
 
 
-def check_all(models, outs, log, threads: int = 16):
+CONDITIONING_FACTOR = 100.0
+
+
+def check_all(models, outs, log, exact_outs=None, threads: int = 16, tol: float = 1e-10):
     """Every book problem against the oracle (cached coefficients; same arithmetic as the faithful
-    mode), on a pool of host threads (ctypes releases the GIL during each solve)."""
+    mode), on a pool of host threads (ctypes releases the GIL during each solve).
+
+    exact_outs, when given, are the book's surfaces from the exact GPU path (coefficient fields and
+    sequential Thomas lines, the oracle's own operation order): each must equal the oracle's surface
+    bit for bit (NaN where the oracle has NaN), whatever the problem's conditioning.
+
+    outs are the fast path's surfaces (separable coefficients, partitioned lines, fused multiply-
+    adds): each is held to tol normwise.  The book's parameter ranges include problems on which the
+    scheme itself is unstable at the drawn step count (the explicit v = 0 row and the explicit
+    mixed-derivative corrections at large steps; the solution then grows without bound): there any
+    change of rounding is amplified by orders of magnitude.  For every problem over tol the oracle is
+    run three more times, with the reference's other tridiagonal solver (double sweep instead of
+    Thomas LU) and with the strike and the rate perturbed by a relative 1e-15; the largest change
+    among them (the spread) measures what the problem does to a change of rounding, and the fast
+    path is accepted when its distance to the oracle is within CONDITIONING_FACTOR times the spread.
+    Where the oracle overflows (non-finite values) the fast path must overflow too.  A problem is
+    counted as unstable when the oracle overflows or grows past ten times S_max."""
     import concurrent.futures as cf
+    import dataclasses
     from oracle import oracle as O
     from tests.helpers import oracle_solve
     threads = max(1, min(threads, os.cpu_count() or 1))
     t0 = time.time()
 
-    def one(q):
-        c = oracle_solve(O, models[q])
-        g = outs[q]
-        return q, float(np.abs(g - c).max() / np.abs(c).max()), rel_err(g, c)
+    def normwise(g, c):
+        return float(np.abs(g - c).max() / np.abs(c).max())
 
-    worst_n, worst_v, over, done = 0.0, 0.0, [], 0
+    def one(q):
+        m = models[q]
+        c = oracle_solve(O, m)
+        g = outs[q]
+        exact = None if exact_outs is None else bool(np.array_equal(exact_outs[q], c, equal_nan=True))
+        if not np.isfinite(c).all():
+            return q, exact, True, None, None, None, bool(not np.isfinite(g).all())
+        unstable = bool(np.abs(c).max() > 10.0 * m.x_hi)
+        if not np.isfinite(g).all():
+            return q, exact, unstable, float("inf"), float("inf"), 0.0, True
+        en = normwise(g, c)
+        spread = None
+        if en > tol:
+            variants = (dict(tridiagonal=L.DOUBLE_SWEEP), dict(strike=m.strike * (1 + 1e-15)),
+                        dict(rate=m.rate * (1 + 1e-15)))
+            spread = max(normwise(oracle_solve(O, dataclasses.replace(m, **v)), c) for v in variants)
+        return q, exact, unstable, en, rel_err(g, c), spread, False
+
+    done, n_exact, exact_bad = 0, 0, []
+    st = {k: {"problems": 0, "within_tolerance": 0, "max_rel_err_normwise": 0.0, "max_rel_err_per_value": 0.0,
+              "within_conditioning": 0, "max_err_over_spread": 0.0} for k in ("stable", "unstable")}
+    overflow, failed, cond = 0, [], []
     order = sorted(range(len(models)), key=lambda q: -models[q].nx * models[q].ny * models[q].steps)
     with cf.ThreadPoolExecutor(threads) as ex:
-        for q, en, ev in ex.map(one, order):
-            worst_n, worst_v, done = max(worst_n, en), max(worst_v, ev), done + 1
-            if en > 1e-10:
-                over.append(q)
+        for q, exact, unstable, en, ev, spread, g_nonfinite in ex.map(one, order):
+            done += 1
+            if exact is not None:
+                n_exact += exact
+                if not exact:
+                    exact_bad.append(q)
+            d = st["unstable" if unstable else "stable"]
+            d["problems"] += 1
+            if en is None:  # the oracle overflowed
+                overflow += 1
+                if not g_nonfinite:
+                    failed.append((q, float("nan"), float("nan")))
+            elif spread is None:
+                d["within_tolerance"] += 1
+                d["max_rel_err_normwise"] = max(d["max_rel_err_normwise"], en)
+                d["max_rel_err_per_value"] = max(d["max_rel_err_per_value"], ev)
+            elif en <= CONDITIONING_FACTOR * spread:
+                d["within_conditioning"] += 1
+                d["max_err_over_spread"] = max(d["max_err_over_spread"], en / spread)
+                cond.append((q, en, spread))
+            else:
+                failed.append((q, en, spread))
             if done % 256 == 0:
-                log(f"check-all: {done}/{len(models)} problems, max normwise {worst_n:.2e} ({time.time() - t0:.0f}s)")
-    log(f"check-all: {done} problems in {time.time() - t0:.0f}s on {threads} threads, max normwise {worst_n:.2e}, "
-        f"max per-value {worst_v:.2e}, over 1e-10 normwise: {len(over)}")
-    return {"problems": done, "threads": threads, "max_rel_err_normwise": worst_n, "max_rel_err_per_value": worst_v,
-            "over_tolerance": over[:32], "tolerance_normwise": 1e-10, "seconds": round(time.time() - t0, 1)}
+                log(f"check-all: {done}/{len(models)} problems, {len(failed)} failed"
+                    + ("" if exact_outs is None else f", exact path bit-identical {n_exact}/{done}")
+                    + f" ({time.time() - t0:.0f}s)")
+    S, U = st["stable"], st["unstable"]
+    log(f"check-all: {done} problems in {time.time() - t0:.0f}s on {threads} threads"
+        + ("" if exact_outs is None else f"; exact path bit-identical to the oracle on {n_exact}/{done}")
+        + f"; fast path on the {S['problems']} stable problems: {S['within_tolerance']} within {tol:.0e} normwise "
+        f"(max {S['max_rel_err_normwise']:.2e}), {S['within_conditioning']} within conditioning (max "
+        f"{S['max_err_over_spread']:.1f}x spread); on the {U['problems']} unstable ones: {U['within_tolerance']} "
+        f"within {tol:.0e}, {U['within_conditioning']} within conditioning (max {U['max_err_over_spread']:.1f}x "
+        f"spread), {overflow} overflow in the oracle and on the GPU; {len(failed)} failed")
+    rnd = lambda v: float(f"{v:.3e}")  # noqa: E731
+    res = {"problems": done, "threads": threads, "tolerance_normwise": tol, "conditioning_factor": CONDITIONING_FACTOR,
+           "stable": S, "unstable": U, "overflow_in_oracle_and_gpu": overflow,
+           "within_conditioning_sample": [[q, rnd(e), rnd(s)] for q, e, s in sorted(cond)[:12]],
+           "failed": len(failed), "failed_sample": [[q, rnd(e), rnd(s)] for q, e, s in sorted(failed)[:12]],
+           "seconds": round(time.time() - t0, 1)}
+    if exact_outs is not None:
+        res["exact_path_bit_identical"] = n_exact
+        res["exact_path_mismatch_sample"] = sorted(exact_bad)[:12]
+    return res
+
+
+def exact_book(torch, log):
+    """The book through the exact path (coefficient fields, sequential Thomas lines): its surfaces."""
+    models = book_models(2048, coeff_mode=L.COEFF_FIELDS, line_solver=L.LINE_EXACT)
+    t0 = time.time()
+    plan = L.Plan.from_models(models, L.FP64)
+    ins = [torch.from_numpy(plan.initial(p)).cuda() for p in range(plan.count)]
+    outs = [torch.empty_like(u) for u in ins]
+    plan.run_dev([u.data_ptr() for u in ins], [u.data_ptr() for u in outs], torch.cuda.current_stream().cuda_stream,
+                 graph=False)
+    res = [o.cpu().numpy() for o in outs]
+    plan.close()
+    log(f"exact-path book solve (plan, solve, copy back) {time.time() - t0:.1f}s")
+    return res
 
 
 def cpu_baseline(models, log, budget_s: float = 15.0):
@@ -508,7 +602,8 @@ def main():
                      "whole_step_algo_GBs": round(total_algo / (total_ms / 1e3) / 1e9, 1)},
     }
     if args.check_all:
-        result["check_all"] = check_all(models, [o.cpu().numpy() for o in outs], log)
+        result["check_all"] = check_all(models, [o.cpu().numpy() for o in outs], log,
+                                        exact_outs=exact_book(torch, log))
     if not args.no_extras:
         from tests.helpers import rel_err as _re  # noqa: F401
         outs_np = [o.cpu().numpy() for o in outs]

@@ -120,9 +120,11 @@ int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v
 /* Kernel configuration of a plan (LSS_KERNEL_CONFIG_LEN values): line solver, S-lines and v-lines per
  * workgroup, XCD order, compile-time factor pattern of the coefficients (0 none, 1 Heston, 2 SABR with
  * r = 0), chunk bounds of S-lines and v-lines, whether the autotuner has run, the compile-time pattern
- * of the adjoint kernels (factor and derivative-term tables; -1 before the adjoint's first use) and
- * the S-lines per adjoint S-sweep workgroup (0 before first use). */
-enum { LSS_KERNEL_CONFIG_LEN = 10 };
+ * of the adjoint kernels (factor and derivative-term tables; -1 before the adjoint's first use), the
+ * S-lines per adjoint S-sweep workgroup (0 before first use) and whether the backward pass overlaps
+ * the recomputation of the next segment with the adjoint of the current one (two segment storage
+ * sets, for plans whose second set takes at most 2 GiB; -1 before first use). */
+enum { LSS_KERNEL_CONFIG_LEN = 11 };
 int lss_plan_kernel_config(const lss_adi_plan *plan, int32_t *config);
 int lss_adi_run_host(lss_adi_plan *plan, const void *const *u_in, void *const *u_out);
 

@@ -211,6 +211,11 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             (void)hipGraphDestroy(graph);
         if (cap_stream)
             (void)hipStreamDestroy(cap_stream);
+        if (aux_stream)
+            (void)hipStreamDestroy(aux_stream);
+        for (hipEvent_t e : {ev_fork, ev_ready[0], ev_ready[1], ev_free[0], ev_free[1]})
+            if (e)
+                (void)hipEventDestroy(e);
     }
 
     static void validate(const lss_adi_desc &d)
@@ -611,6 +616,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         cfg[7] = tuned ? 1 : 0;
         cfg[8] = ck > 0 ? apat : -1;
         cfg[9] = ck > 0 ? axw : 0;
+        cfg[10] = ck > 0 ? (dbl ? 1 : 0) : -1;
     }
 
     void set_mapping(int x, int v, int z) override
@@ -748,6 +754,13 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     int slot_tb[5] = {TB_AV, TB_AX, TB_AV, TB_AX, TB_U};
     int axw = 16;                                 // S-lines per adjoint x-sweep workgroup
     int apat = PAT_GENERIC;                       // compile-time factor and term tables of the adjoint
+    // Two segment storage sets (dbl): the backward pass then recomputes segment c - 1 on aux_stream
+    // while the adjoint steps of segment c run on the caller's stream.  Used when the second set
+    // takes at most SEG2_MAX_BYTES (plans of few problems, whose kernels alone leave CUs idle).
+    static constexpr size_t SEG2_MAX_BYTES = (size_t)2 << 30;
+    bool dbl = false;
+    hipStream_t aux_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_ready[2] = {}, ev_free[2] = {};
 
     void invalidate_adjoint() override
     {
@@ -755,7 +768,12 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         fwd_done = false;
     }
 
-    const Prob<T> *seg_table(int k) const { return (const Prob<T> *)seg_dev.p + (size_t)k * count; }
+    // problem table of step s0 + k of a segment stored in set `set`
+    const Prob<T> *seg_table(int k, int set = 0) const
+    {
+        return (const Prob<T> *)seg_dev.p + ((size_t)set * ck_len + k) * count;
+    }
+    int ck_len = 0;  // steps per segment of the tables in seg_dev
 
     // Derivative terms on the device: terms with an all-zero factor are dropped, all-ones factors are
     // not stored (code -2) and identical factor vectors are stored once (see Adj::dfac).
@@ -841,8 +859,16 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         const int64_t n = count;
         // sizes
         std::vector<size_t> off_T(n), n_T(n), off_U(n);
-        size_t total = 0;
+        size_t total = 0, seg_bytes = 0;
         max_nxy = 0;
+        for (int64_t q = 0; q < n; ++q) {
+            const Prob<T> &p = probs[q];
+            const size_t nxy = ((size_t)p.nx * p.ny + 63) / 64 * 64;
+            const bool dr = p.scheme == SCHEME_DR, hv = p.scheme == SCHEME_HV;
+            const size_t nys = (dr ? 1 : 3) + ((hv && line_solver == LINE_EXACT) ? 1 : 0);
+            seg_bytes += ((size_t)(K + 1) + (size_t)K * nys) * nxy * sizeof(T);
+        }
+        dbl = seg_bytes <= SEG2_MAX_BYTES;
         for (int64_t q = 0; q < n; ++q) {
             const Prob<T> &p = probs[q];
             const size_t nxy = (size_t)p.nx * p.ny, nx_ = (size_t)p.nx, ny_ = (size_t)p.ny;
@@ -853,8 +879,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             const Terms *tm = terms.empty() ? nullptr : &terms[order[q]];
             const size_t nfac = tm ? tm->fac.size() / sizeof(T) : 0;
             auto al = [](size_t k) { return (k + 63) / 64 * 64; };
-            size_t t = nl * al(nxy) + 2 * al(nx_) + nck * al(nxy) + (K + 1) * al(nxy) + (size_t)K * nys * al(nxy) +
-                       al(nfac);
+            size_t t = nl * al(nxy) + 2 * al(nx_) + nck * al(nxy) +
+                       (dbl ? 2 : 1) * ((K + 1) * al(nxy) + (size_t)K * nys * al(nxy)) + al(nfac);
             off_T[q] = total;
             n_T[q] = t;
             total += t * sizeof(T);
@@ -869,8 +895,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         char *base = (char *)adj_mem.p;
         adjs.assign(n, Adj<T>{});
         ckpt_base.assign(n, nullptr);
-        seg_lv_base.assign(n, nullptr);
-        std::vector<T *> seg_y_base(n, nullptr);
+        seg_lv_base.assign(2 * n, nullptr);  // set 1 at q + n
+        std::vector<T *> seg_y_base(2 * n, nullptr);
         for (int64_t q = 0; q < n; ++q) {
             const Prob<T> &p = probs[q];
             const size_t nxy = (size_t)p.nx * p.ny, nx_ = (size_t)p.nx, ny_ = (size_t)p.ny;
@@ -897,6 +923,10 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             ckpt_base[q] = take(nck * nxy);
             seg_lv_base[q] = take((K + 1) * nxy);
             seg_y_base[q] = take((size_t)K * nys * nxy);
+            if (dbl) {
+                seg_lv_base[q + n] = take((K + 1) * nxy);
+                seg_y_base[q + n] = take((size_t)K * nys * nxy);
+            }
             const Terms *tm = terms.empty() ? nullptr : &terms[order[q]];
             const size_t nfac = tm ? tm->fac.size() / sizeof(T) : 0;
             T *fac = take(nfac);
@@ -956,12 +986,14 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         adj_dev.alloc(n * sizeof(Adj<T>));
         HIPCHK(hipMemcpy(adj_dev.p, adjs.data(), n * sizeof(Adj<T>), hipMemcpyHostToDevice));
         // per-step problem tables of a segment: step s0 + k reads level slot k, writes slot k + 1
-        std::vector<Prob<T>> seg((size_t)K * n);
+        const int nset = dbl ? 2 : 1;
+        std::vector<Prob<T>> seg((size_t)nset * K * n);
+        for (int set = 0; set < nset; ++set)
         for (int k = 0; k < K; ++k)
             for (int64_t q = 0; q < n; ++q) {
                 Prob<T> t = probs[q];
                 const size_t nxy = (size_t)t.nx * t.ny;
-                T *lk = seg_lv_base[q] + (size_t)k * nxy, *lk1 = lk + nxy;
+                T *lk = seg_lv_base[q + set * n] + (size_t)k * nxy, *lk1 = lk + nxy;
                 if (k % 2 == 0) {
                     t.a = lk;
                     t.b = lk1;
@@ -971,13 +1003,14 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 }
                 const bool dr = t.scheme == SCHEME_DR, hv = t.scheme == SCHEME_HV;
                 const size_t nys = (dr ? 1 : 3) + ((hv && line_solver == LINE_EXACT) ? 1 : 0);
-                T *yb = seg_y_base[q] + (size_t)k * nys * nxy;
+                T *yb = seg_y_base[q + set * n] + (size_t)k * nys * nxy;
                 t.y1 = yb;
                 t.y2 = dr ? nullptr : yb + nxy;
                 t.y3 = dr ? nullptr : yb + 2 * nxy;
                 t.y4 = line_solver == LINE_PARTITIONED ? t.y3 : (hv ? yb + 3 * nxy : t.y1);
-                seg[(size_t)k * n + q] = t;
+                seg[((size_t)set * K + k) * n + q] = t;
             }
+        ck_len = K;
         seg_dev.~dev_buffer();
         new (&seg_dev) dev_buffer();
         seg_dev.alloc(seg.size() * sizeof(Prob<T>));
@@ -999,8 +1032,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             bwd_job_off[c] = jobs.size();
             for (int64_t q = 0; q < n; ++q)
                 if (probs[q].steps > c * K)
-                    jobs.push_back({ckpt_base[q] + (size_t)c * probs[q].nx * probs[q].ny, seg_lv_base[q],
-                                    (int64_t)probs[q].nx * probs[q].ny});
+                    jobs.push_back({ckpt_base[q] + (size_t)c * probs[q].nx * probs[q].ny,
+                                    seg_lv_base[q + (dbl ? (c & 1) : 0) * n], (int64_t)probs[q].nx * probs[q].ny});
             bwd_job_n[c] = (int)(jobs.size() - bwd_job_off[c]);
         }
         jobs_dev.~dev_buffer();
@@ -1182,13 +1215,50 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         launch_jobs(ljd, (int)nin, stream);
         const CopyJob *jd = (const CopyJob *)jobs_dev.p;
         const int nseg = (max_steps + K - 1) / K;
-        for (int c = nseg - 1; c >= 0; --c) {
+        // recomputation of segment c from its checkpoint into storage set `set`, keeping every stage
+        auto recompute = [&](int c, int set, hipStream_t st) {
             const int s0 = c * K, s1 = std::min(max_steps, s0 + K);
-            launch_jobs(jd + bwd_job_off[c], bwd_job_n[c], stream);
+            launch_jobs(jd + bwd_job_off[c], bwd_job_n[c], st);
             for (int s = s0; s < s1; ++s)
-                launch_step(s, stream, nullptr, seg_table(s - s0));
+                launch_step(s, st, nullptr, seg_table(s - s0, set));
+        };
+        auto adjoint = [&](int c, int set) {
+            const int s0 = c * K, s1 = std::min(max_steps, s0 + K);
             for (int s = s1 - 1; s >= s0; --s)
-                launch_adj_step(s, seg_table(s - s0), stream);
+                launch_adj_step(s, seg_table(s - s0, set), stream);
+        };
+        if (!dbl) {
+            for (int c = nseg - 1; c >= 0; --c) {
+                recompute(c, 0, stream);
+                adjoint(c, 0);
+            }
+        } else {
+            // segment c lives in set c & 1: aux_stream recomputes segment c - 1 (after the adjoint of
+            // segment c + 1 has released that set) while this stream runs the adjoint of segment c
+            if (!aux_stream) {
+                HIPCHK(hipStreamCreateWithFlags(&aux_stream, hipStreamNonBlocking));
+                HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+                for (int k = 0; k < 2; ++k) {
+                    HIPCHK(hipEventCreateWithFlags(&ev_ready[k], hipEventDisableTiming));
+                    HIPCHK(hipEventCreateWithFlags(&ev_free[k], hipEventDisableTiming));
+                }
+            }
+            HIPCHK(hipEventRecord(ev_fork, stream));
+            HIPCHK(hipStreamWaitEvent(aux_stream, ev_fork, 0));
+            recompute(nseg - 1, (nseg - 1) & 1, aux_stream);
+            HIPCHK(hipEventRecord(ev_ready[(nseg - 1) & 1], aux_stream));
+            for (int c = nseg - 1; c >= 0; --c) {
+                if (c >= 1) {
+                    const int set = (c - 1) & 1;
+                    if (c + 1 <= nseg - 1)  // the set held segment c + 1
+                        HIPCHK(hipStreamWaitEvent(aux_stream, ev_free[set], 0));
+                    recompute(c - 1, set, aux_stream);
+                    HIPCHK(hipEventRecord(ev_ready[set], aux_stream));
+                }
+                HIPCHK(hipStreamWaitEvent(stream, ev_ready[c & 1], 0));
+                adjoint(c, c & 1);
+                HIPCHK(hipEventRecord(ev_free[c & 1], stream));
+            }
         }
         launch_jobs(ljd + nin, (int)(lj.size() - nin), stream);
         HIPCHK(hipGetLastError());

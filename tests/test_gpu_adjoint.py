@@ -211,3 +211,27 @@ def test_specialized_adjoint_kernels_match_the_generic_ones(model):
           f"parameters {ds[:5]} vs {df[:5]}", flush=True)
     assert np.abs(gs - gf).max() <= 1e-9 * np.abs(gf).max()
     np.testing.assert_allclose(ds, df, rtol=1e-8, atol=1e-12 * np.abs(df).max())
+
+
+def test_overlapped_backward_matches_the_single_stream_one():
+    """A plan whose second segment storage set fits in 2 GiB recomputes segment c - 1 on a second
+    stream while the adjoint of segment c runs; one whose set does not runs both on the caller's
+    stream.  The checkpoint interval alone decides here (600 x 460 MCS, theta 0.5, 300 steps: 256
+    steps per segment need 2.3 GB per set, 32 steps 0.28 GB), and the gradients must not depend on
+    the schedule."""
+    m = _heston(2, nx=600, ny=460, steps=300, weighting=0.5)
+    plan = L.Plan.from_models([m])
+    u0 = plan.initial(0)
+    w = _price_weights(plan, 0, 100.0, 0.04)
+    res = {}
+    for ckpt in (256, 32):
+        out, g, dp = _grad(plan, [u0], [w], ckpt=ckpt)
+        res[ckpt] = (plan.kernel_config()["adjoint_overlap"], out[0], g[0], dp[0])
+    assert res[256][0] == 0 and res[32][0] == 1
+    assert np.isfinite(res[256][2]).all() and np.isfinite(res[256][3]).all()
+    np.testing.assert_array_equal(res[256][1], res[32][1])
+    gd = np.abs(res[256][2] - res[32][2]).max() / np.abs(res[256][2]).max()
+    print(f"overlapped vs single-stream backward: surface gradient {gd:.2e}, parameters {res[32][3][:5]} vs "
+          f"{res[256][3][:5]}", flush=True)
+    assert gd <= 1e-13
+    np.testing.assert_allclose(res[32][3], res[256][3], rtol=1e-12, atol=1e-14 * np.abs(res[256][3]).max())
