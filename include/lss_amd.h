@@ -219,7 +219,12 @@ int lss_adi_run_profiled(lss_adi_plan *plan, const void *const *u_in, void *cons
  *                     ckpt_every steps from its checkpoint and runs the adjoint steps backwards.  Writes
  *                     lam_in[p] = dJ/du_in[p] (device; a NULL entry or array skips it) and, on the host,
  *                     dparams[p * LSS_MAX_PARAMS + k] = dJ/dparam_k of problem p.  Synchronous.
- *                     Needs nx - 2 <= 1024 and ny - 2 <= 1024.
+ *                     Needs nx - 2 <= 1024 and ny - 2 <= 1024.  When a second segment store
+ *                     (ckpt_every + 1 levels and the stage outputs of ckpt_every steps, over all
+ *                     problems) takes at most 2 GiB, the plan keeps two, and the recomputation of
+ *                     the next segment runs on a stream of the plan's own (ordered after the work
+ *                     already queued on `stream`) while the adjoint of the current one runs on
+ *                     `stream`; every result is ready when the call returns.
  * Parameters enter through the transformed coefficients: a derivative term says
  *   d q(i, j) / d param_k = X_t[i] * Y_t[j]   for q one of a..f (LSS_Q_*),
  * i.e. the derivative of heat_data_transform's coefficient at grid point (i, j)

@@ -268,3 +268,37 @@ def test_reference_models_use_the_specialized_factor_patterns():
     assert s.kernel_config()["factor_pattern"] == 2
     mixed = L.Plan.from_models([_stable(40, 20, coeff_mode=L.COEFF_SEPARABLE, line_solver=L.LINE_PARTITIONED), sabr])
     assert mixed.kernel_config()["factor_pattern"] == 0
+
+
+def test_book_problems_unstable_in_the_reference_match_it_on_the_exact_path(oracle):
+    """Problems of the 2048-problem book (seed 3) that the scheme itself makes unstable at their drawn
+    step counts (the oracle's values grow past 10 S_max) still match the oracle bit for bit on the
+    exact path, stepped in one plan with stable ones; the stable ones also match on the fast path
+    to 1e-10 normwise.  The smallest such problems of the book are used."""
+    import dataclasses
+    import bench
+    book = bench.book_models(2048, coeff_mode=L.COEFF_FIELDS, line_solver=L.LINE_EXACT)
+    order = sorted(range(len(book)), key=lambda q: book[q].nx * book[q].ny * book[q].steps)
+    unstable, stable, ref = [], [], {}
+    for q in order:
+        c = oracle_solve(oracle, book[q])
+        blown = (not np.isfinite(c).all()) or np.abs(c).max() > 10.0 * book[q].x_hi
+        group = unstable if blown else stable
+        if len(group) < 4:
+            group.append(q)
+            ref[q] = c
+        if len(unstable) == 4 and len(stable) == 4:
+            break
+    assert len(unstable) == 4 and len(stable) == 4
+    picked = unstable + stable
+    exact = L.Plan.from_models([book[q] for q in picked]).run_host()
+    fast = L.Plan.from_models([dataclasses.replace(book[q], coeff_mode=L.COEFF_SEPARABLE,
+                                                   line_solver=L.LINE_PARTITIONED) for q in stable]).run_host()
+    for q, g in zip(picked, exact):
+        c = ref[q]
+        print(f"book problem {q}: {book[q].nx}x{book[q].ny}x{book[q].steps} scheme {book[q].splitting}, oracle max "
+              f"{np.abs(c).max():.3e}, exact path bit-identical {np.array_equal(g, c, equal_nan=True)}", flush=True)
+        assert np.array_equal(g, c, equal_nan=True)
+    for q, g in zip(stable, fast):
+        c = ref[q]
+        assert float(np.abs(g - c).max() / np.abs(c).max()) <= 1e-10
