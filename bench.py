@@ -413,6 +413,53 @@ def bench_adjoint(torch, log, reps=3, ckpt=32):
             "max_rel_diff_vs_fd_h1e-6": worst[1e-6], "max_rel_diff_vs_fd_h1e-4": worst[1e-4]}
 
 
+def explicit_models(count=1024, seed=5, coeff_mode=L.COEFF_SEPARABLE):
+    """The reference's explicit Euler Heston case (50 x 30 grid, 8000 steps, S in [50, 200], v in
+    [0, 1], hints (K, 3, 50), backward; unit_tests/common/lss_print_t.hpp:2599-2703) with the strike
+    drawn from U[80, 120] per problem."""
+    rng = np.random.default_rng(seed)
+    models = []
+    for _ in range(count):
+        K = float(rng.uniform(80.0, 120.0))
+        models.append(L.Model(kind=L.MODEL_HESTON, strike=K, rate=0.03, sig_sig=0.3, sig_kappa=2.0, sig_theta=0.2,
+                              rho=0.8, maturity=1.0, x_lo=50.0, x_hi=200.0, y_lo=0.0, y_hi=1.0, t_lo=0.0, t_hi=1.0,
+                              nx=50, ny=30, nt=8000, grid=L.NONUNIFORM, accumulation_point=K, alpha_scale=3.0,
+                              beta_scale=50.0, splitting=L.EXPLICIT_EULER, direction=L.BACKWARD,
+                              coeff_mode=coeff_mode, line_solver=L.LINE_PARTITIONED))
+    return models
+
+
+def bench_explicit(torch, log, count=1024, oracle_check=2):
+    """Explicit Euler book: per step bnd_a, euler, bnd_b over every problem, replayed from a hipGraph."""
+    from oracle import oracle as O
+    from tests.helpers import oracle_solve
+    models = explicit_models(count)
+    plan = L.Plan.from_models(models, L.FP64)
+    ins = [torch.from_numpy(plan.initial(p)).cuda() for p in range(plan.count)]
+    outs = [torch.empty_like(u) for u in ins]
+    pin, pout = [u.data_ptr() for u in ins], [u.data_ptr() for u in outs]
+    stream = torch.cuda.current_stream().cuda_stream
+    plan.run_dev(pin, pout, stream, graph=True)
+    torch.cuda.synchronize()
+    secs = time_stream(torch, lambda: plan.run_dev(pin, pout, stream, graph=True), 2) / 2
+    ps = point_steps(models)
+    prof = L.run_profiled(plan, pin, pout, stream)
+    e_ms, e_bytes, e_n = prof["euler"]
+    gbs = e_bytes / (e_ms / 1e3) / 1e9
+    worst = 0.0
+    for q in range(oracle_check):
+        c = oracle_solve(O, models[q])
+        worst = max(worst, float(np.abs(outs[q].cpu().numpy() - c).max() / np.abs(c).max()))
+    log(f"explicit Euler book: {count} problems 50x30x7999, {secs * 1e3:.1f} ms, {ps / secs:.3e} grid-point-steps/s; "
+        f"k_euler {e_ms / e_n * 1e3:.2f} us per launch, {gbs:.0f} GB/s; vs oracle (normwise, {oracle_check} "
+        f"problems) {worst:.2e}")
+    plan.close()
+    return {"workload": f"heston_explicit_euler_{count}x50x30x7999", "ms": round(secs * 1e3, 2),
+            "grid_point_steps_per_s": ps / secs, "euler_kernel_GBs": round(gbs, 1),
+            "euler_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "euler_avg_launch_us": round(e_ms / e_n * 1e3, 2),
+            "vs_oracle_normwise": worst, "problems_checked": oracle_check}
+
+
 def sabr_models(count=256, seed=4, nx=512, ny=256, steps=500):
     rng = np.random.default_rng(seed)
     models, a0s = [], []
@@ -520,7 +567,7 @@ def main():
     ap.add_argument("--counters", default="",
                     help="counter-collection run: S,V,X mapping (e.g. 8,4,1) set instead of autotuned; the book is "
                          "solved --steps times with no events and no timing (hipEvents are not used under --pmc)")
-    ap.add_argument("--only", choices=("adjoint", "sabr", "tridiag"), default="",
+    ap.add_argument("--only", choices=("adjoint", "sabr", "tridiag", "explicit"), default="",
                     help="run only this secondary workload (no book, no JSON bench line)")
     ap.add_argument("--check-all", action="store_true",
                     help="check every book problem against the oracle on host threads (about 12 min on 16 cores)")
@@ -548,7 +595,8 @@ def main():
         log("calibration: fill of 256 MiB, then 256 MiB read + 256 MiB written by one elementwise kernel")
         del x, y
     if args.only:  # one secondary workload alone (for kernel traces of it)
-        fn = {"adjoint": bench_adjoint, "sabr": bench_sabr, "tridiag": bench_tridiag}[args.only]
+        fn = {"adjoint": bench_adjoint, "sabr": bench_sabr, "tridiag": bench_tridiag,
+              "explicit": bench_explicit}[args.only]
         print(json.dumps({"only": args.only, "result": fn(torch, log)}), flush=True)
         return
     models = book_models(args.count)
@@ -622,6 +670,7 @@ def main():
         plan = None
         result["adjoint"] = bench_adjoint(torch, log)
         result["sabr_book"] = bench_sabr(torch, log)
+        result["explicit_euler_book"] = bench_explicit(torch, log)
     if not args.no_cpu:
         rate, nprob, ps = cpu_baseline(models, log)
         result["cpu_baseline"] = {"value": rate, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",

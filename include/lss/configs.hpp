@@ -353,6 +353,40 @@ struct heat_implicit_solver_config : public pde_implicit_solver_config {
 using pde_implicit_solver_config_ptr = sptr_t<pde_implicit_solver_config>;
 using heat_implicit_solver_config_ptr = sptr_t<heat_implicit_solver_config>;
 
+// explicit solver configs (lss_pde_solver_config.hpp:120-147, lss_heat_solver_config.hpp:56-78)
+struct pde_explicit_solver_config {
+  private:
+    memory_space_enum memory_space_;
+    traverse_direction_enum traverse_direction_;
+
+  public:
+    explicit pde_explicit_solver_config(memory_space_enum const &memory_space,
+                                        traverse_direction_enum const &traverse_direction)
+        : memory_space_{memory_space}, traverse_direction_{traverse_direction}
+    {
+    }
+    virtual ~pde_explicit_solver_config() {}
+    memory_space_enum memory_space() const { return memory_space_; }
+    traverse_direction_enum traverse_direction() const { return traverse_direction_; }
+};
+
+struct heat_explicit_solver_config : public pde_explicit_solver_config {
+  private:
+    explicit_pde_schemes_enum explicit_pde_scheme_;
+
+  public:
+    explicit heat_explicit_solver_config(memory_space_enum const &memory_space,
+                                         traverse_direction_enum const &traverse_direction,
+                                         explicit_pde_schemes_enum const &explicit_pde_scheme)
+        : pde_explicit_solver_config{memory_space, traverse_direction}, explicit_pde_scheme_{explicit_pde_scheme}
+    {
+    }
+    explicit_pde_schemes_enum explicit_pde_scheme() const { return explicit_pde_scheme_; }
+};
+
+using pde_explicit_solver_config_ptr = sptr_t<pde_explicit_solver_config>;
+using heat_explicit_solver_config_ptr = sptr_t<heat_explicit_solver_config>;
+
 // The reference's named default configurations (lss_heat_solver_config.hpp:84-239).
 namespace default_heat_solver_configs {
 #define LSS_DEFAULT_CFG(name, mem, dir, tri, fac, sch)                                                          \
@@ -397,6 +431,19 @@ LSS_DEFAULT_CFG(host_bwd_tlusolver_euler_solver_config_ptr, Host, Backward, Thom
 LSS_DEFAULT_CFG(host_bwd_tlusolver_cn_solver_config_ptr, Host, Backward, ThomasLUSolver, None, CrankNicolson)
 LSS_DEFAULT_CFG(host_bwd_tlusolver_o8_solver_config_ptr, Host, Backward, ThomasLUSolver, None, Theta_80)
 #undef LSS_DEFAULT_CFG
+// explicit ones (lss_heat_solver_config.hpp:244-266)
+#define LSS_DEFAULT_XCFG(name, mem, dir, sch)                                                                   \
+    inline const heat_explicit_solver_config_ptr name = std::make_shared<heat_explicit_solver_config>(       \
+        memory_space_enum::mem, traverse_direction_enum::dir, explicit_pde_schemes_enum::sch);
+LSS_DEFAULT_XCFG(dev_expl_fwd_euler_solver_config_ptr, Device, Forward, Euler)
+LSS_DEFAULT_XCFG(host_expl_fwd_euler_solver_config_ptr, Host, Forward, Euler)
+LSS_DEFAULT_XCFG(dev_expl_bwd_euler_solver_config_ptr, Device, Backward, Euler)
+LSS_DEFAULT_XCFG(host_expl_bwd_euler_solver_config_ptr, Host, Backward, Euler)
+LSS_DEFAULT_XCFG(host_expl_fwd_bc_solver_config_ptr, Host, Forward, ADEBarakatClark)
+LSS_DEFAULT_XCFG(host_expl_bwd_bc_solver_config_ptr, Host, Backward, ADEBarakatClark)
+LSS_DEFAULT_XCFG(host_expl_fwd_s_solver_config_ptr, Host, Forward, ADESaulyev)
+LSS_DEFAULT_XCFG(host_expl_bwd_s_solver_config_ptr, Host, Backward, ADESaulyev)
+#undef LSS_DEFAULT_XCFG
 }  // namespace default_heat_solver_configs
 
 }  // namespace lss_pde_solvers

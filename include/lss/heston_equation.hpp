@@ -183,5 +183,130 @@ template <typename fp_type, template <typename, typename> typename container = s
 using general_heston_equation_ptr = lss_utility::sptr_t<general_heston_equation<fp_type, container, allocator>>;
 
 }  // namespace implicit_solvers
+
+// -------------------------------------------------------------------------------------------------
+// Drop-in explicit_solvers::general_heston_equation: same class, constructor and
+// solve(container_2d<Row>&) as the reference (lss_2d_general_heston_equation.hpp:317-510).  The Euler
+// scheme runs on the GPU for both memory spaces (LSS_EXPLICIT_EULER; FIELDS coefficients, so the
+// result has the bits of the reference's host loop); the ADE schemes throw "Not currently supported"
+// as the reference's kernels do (explicit_kernel.hpp:88-99, 157-168).  A heat source leaves the
+// reference's explicit loop unrun (lss_heston_euler_solver_scheme.hpp:150-155), so the result is
+// then the zero surface, as there.
+// -------------------------------------------------------------------------------------------------
+namespace explicit_solvers {
+
+using lss_boundary::boundary_2d_pair;
+using lss_boundary::boundary_2d_ptr;
+using lss_containers::container_2d;
+using lss_enumerations::by_enum;
+using lss_enumerations::grid_enum;
+using lss_grids::grid_config_hints_2d_ptr;
+
+template <typename fp_type, template <typename, typename> typename container = std::vector,
+          typename allocator = std::allocator<fp_type>>
+class general_heston_equation {
+  private:
+    heat_data_config_2d_ptr<fp_type> heat_data_cfg_;
+    pde_discretization_config_2d_ptr<fp_type> discretization_cfg_;
+    boundary_2d_ptr<fp_type> vertical_upper_;
+    boundary_2d_pair<fp_type> horizontal_pair_;
+    grid_config_hints_2d_ptr<fp_type> grid_hints_;
+    heat_explicit_solver_config_ptr solver_cfg_;
+
+  public:
+    explicit general_heston_equation() = delete;
+    explicit general_heston_equation(heat_data_config_2d_ptr<fp_type> const &heat_data_config,
+                                     pde_discretization_config_2d_ptr<fp_type> const &discretization_config,
+                                     boundary_2d_ptr<fp_type> const &vertical_upper_boundary_ptr,
+                                     boundary_2d_pair<fp_type> const &horizontal_boundary_pair,
+                                     grid_config_hints_2d_ptr<fp_type> const &grid_config_hints,
+                                     heat_explicit_solver_config_ptr const &solver_config =
+                                         default_heat_solver_configs::dev_expl_fwd_euler_solver_config_ptr)
+        : heat_data_cfg_{heat_data_config}, discretization_cfg_{discretization_config},
+          vertical_upper_{vertical_upper_boundary_ptr}, horizontal_pair_{horizontal_boundary_pair},
+          grid_hints_{grid_config_hints}, solver_cfg_{solver_config}
+    {
+        LSS_VERIFY(heat_data_cfg_, "heat_data_config must not be null");
+        LSS_VERIFY(discretization_cfg_, "discretization_config must not be null");
+        LSS_VERIFY(solver_cfg_, "solver_config must not be null");
+        LSS_VERIFY(grid_hints_, "grid_config_hints must not be null");
+    }
+    general_heston_equation(general_heston_equation const &) = delete;
+    general_heston_equation(general_heston_equation &&) = delete;
+    general_heston_equation &operator=(general_heston_equation const &) = delete;
+    general_heston_equation &operator=(general_heston_equation &&) = delete;
+
+    /**
+     * Get the final solution of the PDE (heston_equation.hpp:430-503)
+     *
+     * \param solution - 2D container for the solution, nx rows (S) by ny columns (v)
+     */
+    void solve(container_2d<by_enum::Row, fp_type, container, allocator> &solution)
+    {
+        LSS_ASSERT((solution.rows()) > 0 && (solution.columns() > 0),
+                   "The input solution container must be initialized");
+        const auto np = discretization_cfg_->number_of_space_points();
+        LSS_ASSERT((solution.columns() == np.second) && (solution.rows() == np.first),
+                   "The input solution container must have the correct size");
+        if (solver_cfg_->explicit_pde_scheme() != explicit_pde_schemes_enum::Euler)
+            throw std::runtime_error("Not currently supported");
+        if (heat_data_cfg_->source_data_config()) {
+            for (std::size_t i = 0; i < np.first; ++i)
+                for (std::size_t j = 0; j < np.second; ++j)
+                    solution(i, j, fp_type{});
+            return;
+        }
+        // the implicit class's sampling, with the explicit scheme selected
+        lss_amd_detail::grid_spec<fp_type> g;
+        const auto sr = discretization_cfg_->space_range();
+        g.x_lo = sr.first.lower();
+        g.x_hi = sr.first.upper();
+        g.y_lo = sr.second.lower();
+        g.y_hi = sr.second.upper();
+        g.t_lo = discretization_cfg_->time_range().lower();
+        g.t_hi = discretization_cfg_->time_range().upper();
+        g.nx = np.first;
+        g.ny = np.second;
+        g.nt = discretization_cfg_->number_of_time_points();
+        g.accumulation_point = grid_hints_->accumulation_point();
+        g.alpha_scale = grid_hints_->alpha_scale();
+        g.beta_scale = grid_hints_->beta_scale();
+        g.nonuniform = grid_hints_->grid() == grid_enum::Nonuniform;
+        g.splitting = LSS_EXPLICIT_EULER;
+        g.weighting = fp_type(0.5);  // (unused by the explicit scheme)
+        g.implicit_scheme = LSS_CRANK_NICOLSON;
+        g.tridiagonal = LSS_THOMAS_LU;
+        g.direction = solver_cfg_->traverse_direction() == traverse_direction_enum::Forward ? LSS_FORWARD
+                                                                                          : LSS_BACKWARD;
+        lss_amd_detail::user_data<fp_type> u;
+        u.A = heat_data_cfg_->a_coefficient();
+        u.B = heat_data_cfg_->b_coefficient();
+        u.C = heat_data_cfg_->c_coefficient();
+        u.D = heat_data_cfg_->d_coefficient();
+        u.E = heat_data_cfg_->e_coefficient();
+        u.F = heat_data_cfg_->f_coefficient();
+        u.init = heat_data_cfg_->initial_condition();
+        auto lo = horizontal_pair_.first, hi = horizontal_pair_.second, vu = vertical_upper_;
+        u.h_lower = [lo](fp_type t, fp_type y) { return lo->value(t, y); };
+        u.h_upper = [hi](fp_type t, fp_type y) { return hi->value(t, y); };
+        u.v_upper = [vu](fp_type t, fp_type x) { return vu->value(t, x); };
+        lss_amd_detail::sampled<fp_type> s;
+        lss_amd_detail::sample_fields(u, g, s, lss_amd_detail::TIME_AUTO, lss_amd_detail::TIME_AUTO);
+        const lss_adi_desc d = s.desc(LSS_COEFF_FIELDS, LSS_LINE_EXACT);
+        lss_adi_plan *plan = nullptr;
+        implicit_solvers::detail::check(lss_adi_plan_create(&d, 1, &plan), "lss_adi_plan_create");
+        const void *in[1] = {s.init.data()};
+        void *out[1] = {solution.raw()};
+        const int rc = lss_adi_run_host(plan, in, out);
+        lss_adi_plan_destroy(plan);
+        implicit_solvers::detail::check(rc, "lss_adi_run_host");
+    }
+};
+
+template <typename fp_type, template <typename, typename> typename container = std::vector,
+          typename allocator = std::allocator<fp_type>>
+using general_heston_equation_ptr = lss_utility::sptr_t<general_heston_equation<fp_type, container, allocator>>;
+
+}  // namespace explicit_solvers
 }  // namespace two_dimensional
 }  // namespace lss_pde_solvers

@@ -27,8 +27,14 @@ typedef enum lss_status {
 } lss_status;
 
 enum { LSS_FP64 = 0, LSS_FP32 = 1 };
-/* splitting_method_enum */
-enum { LSS_DOUGLAS_RACHFORD = 0, LSS_CRAIG_SNEYD = 1, LSS_MODIFIED_CRAIG_SNEYD = 2, LSS_HUNDSDORFER_VERWER = 3 };
+/* splitting_method_enum; LSS_EXPLICIT_EULER selects the explicit solver instead
+ * (explicit_solvers::general_heston_equation with explicit_pde_schemes_enum::Euler,
+ * lss_2d_general_heston_equation.hpp:430-503, lss_heston_euler_solver_scheme.hpp:46-92): per step the
+ * Euler update of the interior (lss_heston_euler_solver_method.hpp:26-52), then the explicit boundary
+ * solves A and B shared with the ADI schemes.  A plan steps either explicit Euler problems only or
+ * ADI problems only; explicit plans have no adjoint. */
+enum { LSS_DOUGLAS_RACHFORD = 0, LSS_CRAIG_SNEYD = 1, LSS_MODIFIED_CRAIG_SNEYD = 2, LSS_HUNDSDORFER_VERWER = 3,
+       LSS_EXPLICIT_EULER = 4 };
 /* implicit_pde_schemes_enum */
 enum { LSS_EULER = 0, LSS_THETA_30 = 1, LSS_CRANK_NICOLSON = 2, LSS_THETA_80 = 3 };
 /* tridiagonal_method_enum: selects the line solver's operation order */
@@ -205,7 +211,7 @@ int lss_tridiag_batched_bc(int32_t fp, int32_t layout, int64_t B, int64_t n, con
  */
 enum {
     LSS_K_BND_A = 0, LSS_K_XSWEEP1 = 1, LSS_K_VSWEEP1 = 2, LSS_K_HV_Y3 = 3, LSS_K_XSWEEP2 = 4, LSS_K_VSWEEP2 = 5,
-    LSS_K_BND_B = 6, LSS_K_COUNT = 7
+    LSS_K_BND_B = 6, LSS_K_EULER = 7, LSS_K_COUNT = 8
 };
 int lss_adi_run_profiled(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, void *stream, double *ms,
                          double *bytes, int64_t *launches);

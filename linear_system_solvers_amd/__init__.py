@@ -22,6 +22,7 @@ LIB_PATH = os.path.join(PKG, "liblss_amd.so")
 
 FP64, FP32 = 0, 1
 DOUGLAS_RACHFORD, CRAIG_SNEYD, MODIFIED_CRAIG_SNEYD, HUNDSDORFER_VERWER = 0, 1, 2, 3
+EXPLICIT_EULER = 4  # the explicit solver (Euler scheme) instead of an ADI splitting
 EULER, THETA_30, CRANK_NICOLSON, THETA_80 = 0, 1, 2, 3
 CUDA_SOLVER, DOUBLE_SWEEP, SOR_SOLVER, THOMAS_LU = 0, 1, 2, 3
 FORWARD, BACKWARD = 0, 1
@@ -43,7 +44,7 @@ MAX_TERMS = 12
 HESTON_PARAMS = ("kappa", "theta_v", "sigma", "rho", "r")
 SABR_PARAMS = ("nu", "rho", "beta")
 KERNEL_CONFIG_LEN = 11
-KERNEL_KINDS = ("bnd_a", "xsweep1", "vsweep1", "hv_y3", "xsweep2", "vsweep2", "bnd_b")
+KERNEL_KINDS = ("bnd_a", "xsweep1", "vsweep1", "hv_y3", "xsweep2", "vsweep2", "bnd_b", "euler")
 
 
 class LssUnavailable(RuntimeError):

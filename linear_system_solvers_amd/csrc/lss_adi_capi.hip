@@ -32,6 +32,7 @@ template <typename T> __global__ void k_bnd_b(const Prob<T> *, const Tile *, int
 template <typename T> __global__ void k_xsweep(const Prob<T> *, const Tile *, int, int);
 template <typename T> __global__ void k_vsweep(const Prob<T> *, const Tile *, int, int);
 template <typename T> __global__ void k_hv_y3(const Prob<T> *, const Tile *, int);
+template <typename T> __global__ void k_euler(const Prob<T> *, const Tile *, int);
 template <typename T>
 __global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *, LineBC<T>);
 template <typename T, int M, int STAGE, int W, int PAT, int NC>
@@ -199,6 +200,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     int fpat = PAT_GENERIC;            // factor pattern shared by every problem (SEPARABLE), see FPat
     bool tuned = false;                // the autotuner has chosen xw and vr
     bool adj_ok = false;               // every line fits the partitioned solver (needed by the adjoint)
+    bool euler = false;                // explicit Euler plan (no line sweeps)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -218,6 +220,12 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 (void)hipEventDestroy(e);
     }
 
+    // schemes without the second pair of stages (no Y2, Y3)
+    static bool one_stage(const lss_adi_desc &d)
+    {
+        return d.splitting == LSS_DOUGLAS_RACHFORD || d.splitting == LSS_EXPLICIT_EULER;
+    }
+
     static void validate(const lss_adi_desc &d)
     {
         require(d.nx >= 4 && d.ny >= 4, "nx and ny must be >= 4");
@@ -225,7 +233,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         require(d.nx <= (1 << 20) && d.ny <= (1 << 20) && d.nx * d.ny <= (int64_t)1 << 28,
                 "grid too large for one problem (nx * ny <= 2^28)");
         require(d.steps >= 1 && d.steps < (1 << 30), "steps must be >= 1");
-        require(d.splitting >= 0 && d.splitting <= 3, "unknown splitting method");
+        require(d.splitting >= 0 && d.splitting <= LSS_EXPLICIT_EULER, "unknown splitting method");
         require(d.tridiagonal == LSS_THOMAS_LU || d.tridiagonal == LSS_DOUBLE_SWEEP ||
                     d.tridiagonal == LSS_CUDA_SOLVER,
                 "tridiagonal method must be ThomasLU, DoubleSweep or CUDASolver");
@@ -284,8 +292,11 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             require(descs[p].fp == (sizeof(T) == 8 ? LSS_FP64 : LSS_FP32), "all problems of a plan share fp");
             validate(descs[p]);
             require(descs[p].line_solver == descs[0].line_solver, "all problems of a plan share line_solver");
+            require((descs[p].splitting == LSS_EXPLICIT_EULER) == (descs[0].splitting == LSS_EXPLICIT_EULER),
+                    "a plan steps either explicit Euler problems or ADI problems, not both");
         }
         line_solver = descs[0].line_solver == LSS_LINE_PARTITIONED ? LINE_PARTITIONED : LINE_EXACT;
+        euler = descs[0].splitting == LSS_EXPLICIT_EULER;
         {
             int64_t mx = 1, my = 1;
             for (int64_t q = 0; q < n; ++q) {
@@ -296,7 +307,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 pmx *= 2;
             while (pmy < my)
                 pmy *= 2;
-            adj_ok = pmx <= 16 && pmy <= 16;
+            adj_ok = pmx <= 16 && pmy <= 16 && descs[0].splitting != LSS_EXPLICIT_EULER;
             if (pmx >= 16)
                 xw = 8;  // long S-lines: at most 8 per workgroup
         }
@@ -326,8 +337,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             L.off_a = take(nxy);
             L.off_b = take(nxy);
             L.off_y1 = take(nxy);
-            L.off_y2 = d.splitting == LSS_DOUGLAS_RACHFORD ? 0 : take(nxy);
-            L.off_y3 = d.splitting == LSS_DOUGLAS_RACHFORD ? 0 : take(nxy);
+            L.off_y2 = one_stage(d) ? 0 : take(nxy);
+            L.off_y3 = one_stage(d) ? 0 : take(nxy);
             L.off_scr = take(nxy);
             L.off_colA0 = take((size_t)d.nx);
             L.off_colA1 = take((size_t)d.nx);
@@ -369,8 +380,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             p.a = t + L.off_a;
             p.b = t + L.off_b;
             p.y1 = t + L.off_y1;
-            p.y2 = d.splitting == LSS_DOUGLAS_RACHFORD ? nullptr : t + L.off_y2;
-            p.y3 = d.splitting == LSS_DOUGLAS_RACHFORD ? nullptr : t + L.off_y3;
+            p.y2 = one_stage(d) ? nullptr : t + L.off_y2;
+            p.y3 = one_stage(d) ? nullptr : t + L.off_y3;
             p.y4 = line_solver == LINE_PARTITIONED ? p.y3 : p.y1;
             p.scr = t + L.off_scr;
             p.colA0 = t + L.off_colA0;
@@ -491,6 +502,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             case LSS_K_HV_Y3: b += hv ? 4 * nxy : 0; break;
             case LSS_K_XSWEEP2: b += dr ? 0 : ((hv && line_solver != LINE_PARTITIONED) ? 3 * nxy : 4 * nxy); break;
             case LSS_K_VSWEEP2: b += dr ? 0 : 3 * nxy; break;
+            case LSS_K_EULER: b += 2 * nxy; break;
             default: b += 4.0 * p.ny; break;
             }
         }
@@ -636,7 +648,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     void autotune(hipStream_t st)
     {
         tuned = true;
-        if (line_solver != LINE_PARTITIONED)
+        if (line_solver != LINE_PARTITIONED || euler)
             return;
         const int nst = std::min(max_steps, 8);
         hipEvent_t e0, e1;
@@ -699,6 +711,15 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         pre(LSS_K_BND_A);
         hipLaunchKernelGGL(k_bnd_a<T>, dim3(grid(TB_I)), dim3(TILE), 0, st, pd, td + tile_off[TB_I], s);
         post();
+        if (euler) {  // heston_euler_time_loop::run (lss_heston_euler_solver_scheme.hpp:63-67)
+            pre(LSS_K_EULER);
+            hipLaunchKernelGGL(k_euler<T>, dim3(grid(TB_U)), dim3(GTR * GTC), 0, st, pd, td + tile_off[TB_U], s);
+            post();
+            pre(LSS_K_BND_B);
+            hipLaunchKernelGGL(k_bnd_b<T>, dim3(grid(TB_J)), dim3(TILE), 0, st, pd, td + tile_off[TB_J], s);
+            post();
+            return;
+        }
         auto xs = [&](int stage) {
             if (line_solver == LINE_EXACT) {
                 hipLaunchKernelGGL(k_xsweep<T>, dim3(grid(TB_X)), dim3(TILE), 0, st, pd, td + tile_off[TB_X], s, stage);
@@ -852,6 +873,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
 
     void adj_setup(int K)
     {
+        require(!euler, "explicit Euler plans have no adjoint");
         require(adj_ok, "the adjoint needs nx - 2 <= 1024 and ny - 2 <= 1024 for every problem");
         require(K >= 2 && K % 2 == 0 && K <= 4096, "ckpt_every must be even, >= 2 and <= 4096");
         ck = 0;

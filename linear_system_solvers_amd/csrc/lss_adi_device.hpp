@@ -13,7 +13,7 @@
 namespace lssamd {
 
 // splitting_method_enum (common/lss_enumerations.hpp:35-41)
-enum : int32_t { SCHEME_DR = 0, SCHEME_CS = 1, SCHEME_MCS = 2, SCHEME_HV = 3 };
+enum : int32_t { SCHEME_DR = 0, SCHEME_CS = 1, SCHEME_MCS = 2, SCHEME_HV = 3, SCHEME_EULER = 4 };
 // operation order of the line solve: Thomas LU (lss_thomas_lu_solver.hpp:125-191) or
 // double sweep (lss_double_sweep_solver.hpp:112-155)
 enum : int32_t { TRI_THOMAS = 0, TRI_DSWEEP = 1 };

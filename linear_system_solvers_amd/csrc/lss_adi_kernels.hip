@@ -10,6 +10,7 @@
 //   xsweep  stage 2: CS/MCS rhs_intermed_3 -> Y3 ; HV rhs_intermed_4 -> Y4 (stored in Y1's buffer)
 //   vsweep  stage 2: v-direction rhs of (u, Y3) or (Y2, Y4)                          -> next
 //   bnd_b   boundary stage B: S_min row and S_max Neumann extrapolation (explicit_boundary.hpp:202-238)
+// The explicit Euler solver's step is bnd_a, euler (interior and the A columns), bnd_b.
 // Kernels are compiled with -ffp-contract=off and keep the reference's operand order, so with
 // FIELDS coefficients every value is rounded exactly as the reference's host code rounds it.
 //
@@ -275,6 +276,44 @@ __global__ void __launch_bounds__(TILE) k_hv_y3(const Prob<T> *__restrict__ prob
 }
 
 // -------------------------------------------------------------------------------------------------
+// Explicit Euler step of the explicit solver (lss_heston_euler_solver_scheme.hpp:63-67, 78-83): the
+// interior of next from u by explicit_heston_scheme::rhs (lss_heston_euler_solver_method.hpp:26-52),
+// whose C is gamma * c (lss_heston_euler_coefficients.hpp:50-51); columns 0 and ny - 1 of next take
+// boundary stage A's colA0 / colA1, which the reference writes into next for every row
+// (explicit_boundary.hpp:181-200); rows 0 and nx - 1 are left to boundary stage B.  Two-dimensional
+// tiles of GTR x GTC points, one thread per point, lanes along the contiguous v direction.
+// -------------------------------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(GTR *GTC) k_euler(const Prob<T> *__restrict__ probs,
+                                                    const Tile *__restrict__ tiles, int step)
+{
+    const Tile tl = tiles[blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    if (step >= p.steps)
+        return;
+    const int tr = threadIdx.x / GTC, tc = threadIdx.x % GTC;
+    const int i = tl.first + tr, j = tl.col + tc;
+    const int nx = p.nx, ny = p.ny;
+    if (tr >= tl.count || j >= ny)
+        return;
+    const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
+    const gptr<T> nxt = G((step & 1) ? p.a : p.b);
+    const int64_t q = (int64_t)i * ny + j;
+    if (j == 0 || j == ny - 1) {
+        nxt[q] = G((const T *)(j == 0 ? p.colA0 : p.colA1))[i];
+        return;
+    }
+    if (i == 0 || i == nx - 1)
+        return;
+    const Stencil<T> s = stencil_at(p, coeff_set(p, step), i, j);
+    const T one = (T)1.0;
+    const T C = p.gamma * s.C;
+    const gptr<const T> w = u + q;
+    nxt[q] = C * w[-ny - 1] + s.M * w[-ny] - C * w[-ny + 1] + s.Mt * w[-1] + (one - s.Z - s.W) * w[0] +
+             s.Pt * w[1] - C * w[ny - 1] + s.P * w[ny] + C * w[ny + 1];
+}
+
+// -------------------------------------------------------------------------------------------------
 // Batched tridiagonal systems (one lane per system), the reference's Thomas LU in its operation
 // order with the end rows folded by `bc` (LineBC; PLAIN ignores a[0] and c[n-1]).
 // Layout: contiguous x[k*n + i] (strided == 0) or interleaved x[i*B + k] (strided == 1).
@@ -335,6 +374,7 @@ __global__ void __launch_bounds__(256) k_tridiag_lane(int64_t B, int n, int stri
     template __global__ void k_xsweep<T>(const Prob<T> *, const Tile *, int, int);                            \
     template __global__ void k_vsweep<T>(const Prob<T> *, const Tile *, int, int);                            \
     template __global__ void k_hv_y3<T>(const Prob<T> *, const Tile *, int);                                  \
+    template __global__ void k_euler<T>(const Prob<T> *, const Tile *, int);                                  \
     template __global__ void k_tridiag_lane<T>(int64_t, int, int, const T *, const T *, const T *, const T *, \
                                                T *, T *, LineBC<T>);
 LSS_INST(double)

@@ -12,6 +12,11 @@
  *   pde_solvers/two_dimensional/heat_type/splitting_method/lss_heat_{douglas_rachford,craig_sneyd,...}_method.hpp           (DR/CS/MCS/HV)
  *   pde_solvers/two_dimensional/heat_type/implicit_coefficients/lss_2d_general_heston_equation_coefficients.hpp:43-99
  *   pde_solvers/two_dimensional/heat_type/lss_2d_general_heston_equation_explicit_boundary.hpp:39-238
+ * and the Host explicit Euler path (explicit_solvers::general_heston_equation):
+ *   pde_solvers/two_dimensional/heat_type/lss_2d_general_heston_equation.hpp:430-503      (solve)
+ *   pde_solvers/two_dimensional/heat_type/explicit_schemes/lss_heston_euler_solver_scheme.hpp:46-92 (loop)
+ *   pde_solvers/two_dimensional/heat_type/solver_method/lss_heston_euler_solver_method.hpp:26-52 (rhs)
+ *   pde_solvers/two_dimensional/heat_type/explicit_coefficients/lss_heston_euler_coefficients.hpp:34-51
  *   pde_solvers/two_dimensional/heat_type/transformation/lss_heston_boundary_transform.hpp:36-66
  *   pde_solvers/transformation/lss_heat_data_transform.hpp:148-221
  *   discretization/lss_grid_config.hpp:33-129, lss_grid_transform_config.hpp:103-188
@@ -37,6 +42,9 @@ extern "C" {
 
 /* enum values follow common/lss_enumerations.hpp:15-129 */
 enum { LSS_OR_DOUGLAS_RACHFORD = 0, LSS_OR_CRAIG_SNEYD = 1, LSS_OR_MOD_CRAIG_SNEYD = 2, LSS_OR_HUNDSDORFER_VERWER = 3 };
+/* not a splitting method: the explicit solver with explicit_pde_schemes_enum::Euler
+ * (lss_2d_general_heston_equation_explicit_kernel.hpp:153-160) */
+enum { LSS_OR_EXPLICIT_EULER = 4 };
 enum { LSS_OR_EULER = 0, LSS_OR_THETA_30 = 1, LSS_OR_CRANK_NICOLSON = 2, LSS_OR_THETA_80 = 3 };
 enum { LSS_OR_CUDA_SOLVER = 0, LSS_OR_DOUBLE_SWEEP = 1, LSS_OR_SOR_SOLVER = 2, LSS_OR_THOMAS_LU = 3 };
 enum { LSS_OR_FORWARD = 0, LSS_OR_BACKWARD = 1 };

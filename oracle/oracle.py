@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "liblss_oracle.so")
 
 # enum values of common/lss_enumerations.hpp
 DOUGLAS_RACHFORD, CRAIG_SNEYD, MOD_CRAIG_SNEYD, HUNDSDORFER_VERWER = 0, 1, 2, 3
+EXPLICIT_EULER = 4  # the explicit solver (Euler scheme), not a splitting method
 EULER, THETA_30, CRANK_NICOLSON, THETA_80 = 0, 1, 2, 3
 CUDA_SOLVER, DOUBLE_SWEEP, SOR_SOLVER, THOMAS_LU = 0, 1, 2, 3
 FORWARD, BACKWARD = 0, 1

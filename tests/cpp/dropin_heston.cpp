@@ -8,6 +8,7 @@
 //   case 2: unit_tests/common/lss_print_t.hpp:1976-2090 (SABR DS DR, time-dependent coefficients)
 //   case 3: unit_tests/common/lss_print_t.hpp:1724-1836 (Device cuSOLVER-QR config)
 //   case 4: HV with Host ThomasLU (unit_tests/common/lss_print_t.hpp:2470-2579)
+//   case 5: explicit Euler, Host, backward (unit_tests/common/lss_print_t.hpp:2599-2703)
 // usage: dropin_heston <case> <double|float>  (surface to stdout)
 #include <cmath>
 #include <cstdlib>
@@ -113,6 +114,51 @@ template <typename T> int run(int which)
         return 0;
     }
 
+    if (which == 5) {
+        using lss_pde_solvers::two_dimensional::explicit_solvers::general_heston_equation;
+        auto const &strike = 100.0;
+        auto const &maturity = 1.0;
+        auto const &rate = 0.03;
+        auto const &sig_sig = 0.3;
+        auto const &sig_kappa = 2.0;
+        auto const &sig_theta = 0.2;
+        auto const &rho = 0.8;
+        std::size_t const Sd = 50, Vd = 30, Td = 8000;
+        range<T> spacex_range(static_cast<T>(50.0), static_cast<T>(200.0));
+        range<T> spacey_range(static_cast<T>(0.0), static_cast<T>(1.0));
+        range<T> time_range(static_cast<T>(0.0), static_cast<T>(maturity));
+        auto const discretization_ptr =
+            std::make_shared<pde_discretization_config_2d<T>>(spacex_range, spacey_range, Sd, Vd, time_range, Td);
+        auto a = [=](T t, T s, T v) { return (0.5 * v * s * s); };
+        auto b = [=](T t, T s, T v) { return (0.5 * sig_sig * sig_sig * v); };
+        auto c = [=](T t, T s, T v) { return (rho * sig_sig * v * s); };
+        auto d = [=](T t, T s, T v) { return (rate * s); };
+        auto e = [=](T t, T s, T v) { return (sig_kappa * (sig_theta - v)); };
+        auto f = [=](T t, T s, T v) { return (-rate); };
+        auto const heat_coeffs_data_ptr = std::make_shared<heat_coefficient_data_config_2d<T>>(a, b, c, d, e, f);
+        auto terminal_condition = [=](T s, T v) { return std::max<T>(0.0, s - strike); };
+        auto const heat_init_data_ptr = std::make_shared<heat_initial_data_config_2d<T>>(terminal_condition);
+        auto const heat_data_ptr = std::make_shared<heat_data_config_2d<T>>(heat_coeffs_data_ptr, heat_init_data_ptr);
+        auto const &dirichlet_low = [=](T t, T v) { return 0.0; };
+        auto const &neumann_high = [=](T t, T s) { return -1.0; };
+        auto const &boundary_low_ptr = std::make_shared<dirichlet_boundary_2d<T>>(dirichlet_low);
+        auto const &boundary_high_ptr = std::make_shared<neumann_boundary_2d<T>>(neumann_high);
+        auto const &horizontal_boundary_pair = std::make_pair(boundary_low_ptr, boundary_high_ptr);
+        auto const &dirichlet_high = [=](T t, T s) { return s; };
+        auto const &vertical_upper_boundary_ptr = std::make_shared<dirichlet_boundary_2d<T>>(dirichlet_high);
+        auto const alpha_scale = static_cast<T>(3.);
+        auto const beta_scale = static_cast<T>(50.);
+        auto const &grid_config_hints_ptr =
+            std::make_shared<grid_config_hints_2d<T>>(strike, alpha_scale, beta_scale, grid_enum::Nonuniform);
+        general_heston_equation<T, std::vector, std::allocator<T>> pdesolver(
+            heat_data_ptr, discretization_ptr, vertical_upper_boundary_ptr, horizontal_boundary_pair,
+            grid_config_hints_ptr, dflt::host_expl_bwd_euler_solver_config_ptr);
+        rcontainer_2d_t solution(Sd, Vd, T{});
+        pdesolver.solve(solution);
+        lss_print::print(discretization_ptr, grid_config_hints_ptr, solution, std::cout);
+        return 0;
+    }
+
     // constructor-style cases (lss_print_t.hpp)
     double strike = 100.0, rate = 0.03, sig_sig = 0.3, sig_kappa = 2.0, sig_theta = 0.2, rho = 0.2, beta = 0.7;
     const double maturity = 1.0;
@@ -191,7 +237,7 @@ template <typename T> int run(int which)
 int main(int argc, char **argv)
 {
     if (argc < 3) {
-        std::cerr << "usage: dropin_heston <case 0..4> <double|float>\n";
+        std::cerr << "usage: dropin_heston <case 0..5> <double|float>\n";
         return 2;
     }
     const int which = std::atoi(argv[1]);

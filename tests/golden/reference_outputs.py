@@ -11,6 +11,7 @@ import numpy as np
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_outputs")
 
 DR, CS, MCS, HV = 0, 1, 2, 3
+EXPLICIT_EULER = 4  # explicit_solvers::general_heston_equation with the Euler scheme
 EULER, THETA_30, CN, THETA_80 = 0, 1, 2, 3
 DOUBLE_SWEEP, THOMAS_LU = 1, 3
 
@@ -71,6 +72,30 @@ CASES = [
          cfg=dict(x_lo=50.0, x_hi=200.0, y_lo=0.0, y_hi=1.2, nx=100, ny=50, nt=200, nonuniform=True,
                   accumulation_point=100.0, alpha_scale=3.0, beta_scale=50.0, splitting=DR, weighting=0.5,
                   implicit_scheme=CN, tridiagonal=DOUBLE_SWEEP),
+         floats=True),
+    # explicit Euler (host_expl_bwd_euler_solver_config); the reference's Device runs of the first and
+    # third case (ExplHestonEquationSolverEulerDevice, ExplSABREquationEulerOnDevice) printed the
+    # same digits as these Host runs
+    dict(stem="ExplHestonEquationSolverEulerHost",
+         where="unit_tests/common/lss_print_t.hpp:2599-2703",
+         model=("heston", dict(strike=100.0, rate=0.03, sig_sig=0.3, sig_kappa=2.0, sig_theta=0.2, rho=0.8)),
+         cfg=dict(x_lo=50.0, x_hi=200.0, y_lo=0.0, y_hi=1.0, nx=50, ny=30, nt=8000, nonuniform=True,
+                  accumulation_point=100.0, alpha_scale=3.0, beta_scale=50.0, splitting=EXPLICIT_EULER,
+                  weighting=0.5, implicit_scheme=CN, tridiagonal=THOMAS_LU),
+         floats=True),
+    dict(stem="ExplHestonEquationEulerHostFromBuilder",
+         where="unit_tests/builders/lss_2d_general_heston_equation_builder_t.hpp:171-310",
+         model=("heston", dict(strike=10.0, rate=0.03, sig_sig=0.3, sig_kappa=2.0, sig_theta=0.2, rho=0.8)),
+         cfg=dict(x_lo=0.0, x_hi=20.0, y_lo=0.0, y_hi=1.5, nx=50, ny=30, nt=8000, nonuniform=True,
+                  accumulation_point=10.0, alpha_scale=3.0, beta_scale=60.0, splitting=EXPLICIT_EULER,
+                  weighting=0.5, implicit_scheme=CN, tridiagonal=THOMAS_LU),
+         floats=True),
+    dict(stem="ExplSABREquationEulerOnHost",
+         where="unit_tests/common/lss_print_t.hpp:2835-2944",
+         model=("sabr", dict(strike=100.0, maturity=1.0, rate=0.03, sig_sig=0.081, rho=0.6, beta=0.7)),
+         cfg=dict(x_lo=50.0, x_hi=200.0, y_lo=0.0, y_hi=1.2, nx=50, ny=40, nt=8000, nonuniform=True,
+                  accumulation_point=100.0, alpha_scale=3.0, beta_scale=50.0, splitting=EXPLICIT_EULER,
+                  weighting=0.5, implicit_scheme=CN, tridiagonal=THOMAS_LU),
          floats=True),
 ]
 

@@ -22,7 +22,8 @@ CASES = {0: "ImplHestonEquationDSSolverCrankNicolsonFromBuilder",
          1: "ImplHestonEquationThomasLUSolverDouglasRachfordCrankNicolson",
          2: "ImplSABREquationDoubleSweepSolverCrankNicolson",
          3: "ImplHestonEquationCUDAQRSolverCrankNicolson",
-         4: "ImplHestonEquationThomasLUSolverHundsdorferVerwerCrankNicolson"}
+         4: "ImplHestonEquationThomasLUSolverHundsdorferVerwerCrankNicolson",
+         5: "ExplHestonEquationSolverEulerHost"}
 
 
 def _run(case, fp):
@@ -49,10 +50,12 @@ def test_dropin_double_matches_reference_surface_and_oracle(oracle, case):
     assert np.array_equal(got, printed)
 
 
-@pytest.mark.parametrize("case", [0, 1, 2, 3])
+@pytest.mark.parametrize("case", [0, 1, 2, 3, 5])
 def test_dropin_float_matches_reference_surface(case):
     if not os.path.exists(BIN):
         pytest.fail("tests/cpp/dropin_heston not built (run __graft_entry__.build())")
     got = _run(case, "float")
     _, _, ref = R.load(CASES[case], "f32")
-    assert np.abs(got - ref).max() <= 1.5e-5 * np.abs(ref).max()
+    steps = next(c for c in R.CASES if c["stem"] == CASES[case])["cfg"]["nt"] - 1
+    # float rounding accumulates over the steps (tests/test_oracle_reference_outputs.py)
+    assert np.abs(got - ref).max() <= 1.5e-5 * max(1.0, (steps / 200) ** 0.5) * np.abs(ref).max()

