@@ -10,7 +10,7 @@
 //   xsweep  stage 2: CS/MCS rhs_intermed_3 -> Y3 ; HV rhs_intermed_4 -> Y4 (stored in Y1's buffer)
 //   vsweep  stage 2: v-direction rhs of (u, Y3) or (Y2, Y4)                          -> next
 //   bnd_b   boundary stage B: S_min row and S_max Neumann extrapolation (explicit_boundary.hpp:202-238)
-// The explicit Euler solver's step is bnd_a, euler (interior and the A columns), bnd_b.
+// The explicit Euler solver's step is one launch of k_euler (interior and both boundary stages).
 // Kernels are compiled with -ffp-contract=off and keep the reference's operand order, so with
 // FIELDS coefficients every value is rounded exactly as the reference's host code rounds it.
 //
@@ -276,14 +276,21 @@ __global__ void __launch_bounds__(TILE) k_hv_y3(const Prob<T> *__restrict__ prob
 }
 
 // -------------------------------------------------------------------------------------------------
-// Explicit Euler step of the explicit solver (lss_heston_euler_solver_scheme.hpp:63-67, 78-83): the
-// interior of next from u by explicit_heston_scheme::rhs (lss_heston_euler_solver_method.hpp:26-52),
-// whose C is gamma * c (lss_heston_euler_coefficients.hpp:50-51); columns 0 and ny - 1 of next take
-// boundary stage A's colA0 / colA1, which the reference writes into next for every row
-// (explicit_boundary.hpp:181-200); rows 0 and nx - 1 are left to boundary stage B.  Two-dimensional
-// tiles of GTR x GTC points, one thread per point, lanes along the contiguous v direction.
+// Explicit Euler step of the explicit solver in one launch (lss_heston_euler_solver_scheme.hpp:63-67,
+// 78-83: the interior, boundary stage A, boundary stage B, each from the level before):
+//   interior  explicit_heston_scheme::rhs (lss_heston_euler_solver_method.hpp:26-52), whose C is
+//             gamma * c (lss_heston_euler_coefficients.hpp:50-51);
+//   column 0 and ny - 1  boundary stage A (explicit_boundary.hpp:39-84, 181-200), which writes them
+//             for every row;
+//   row 0 and nx - 1     boundary stage B (explicit_boundary.hpp:202-238); its S_max extrapolation
+//             reads rows nx - 2 and nx - 3 of the new level through idx2, and the thread recomputes
+//             those two values from u with the same arithmetic instead of waiting for them.
+// Every value is rounded as the reference rounds it (this file is compiled without contraction), so
+// with FIELDS coefficients the step is bit-identical to the three stages run one after another.
+// PAT selects compile-time factor patterns of SEPARABLE plans.  Two-dimensional tiles of GTR x GTC
+// points, one thread per point, lanes along the contiguous v direction.
 // -------------------------------------------------------------------------------------------------
-template <typename T>
+template <typename T, int PAT>
 __global__ void __launch_bounds__(GTR *GTC) k_euler(const Prob<T> *__restrict__ probs,
                                                     const Tile *__restrict__ tiles, int step)
 {
@@ -298,19 +305,36 @@ __global__ void __launch_bounds__(GTR *GTC) k_euler(const Prob<T> *__restrict__ 
         return;
     const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
     const gptr<T> nxt = G((step & 1) ? p.a : p.b);
-    const int64_t q = (int64_t)i * ny + j;
-    if (j == 0 || j == ny - 1) {
-        nxt[q] = G((const T *)(j == 0 ? p.colA0 : p.colA1))[i];
-        return;
+    const gptr<const T> cs = coeff_set(p, step);
+    const gptr<const T> bs = bnd_set(p, step);
+    const T one = (T)1.0, two = (T)2.0, three = (T)3.0, four = (T)4.0;
+    // the value stages "interior" and A give point (r, c), 1 <= r <= nx - 2
+    auto level = [&](int r, int c) -> T {
+        const gptr<const T> w = u + ((int64_t)r * ny + c);
+        if (c == ny - 1)
+            return bs[r];  // v_upper(t, X(zeta_r))
+        if (c == 0) {      // the degenerate v = 0 row
+            T D, E, F;
+            def_row0<T, PAT>(p, cs, r, D, E, F);
+            const T delta = p.delta, ni = p.ni, rho = p.rho;
+            return (-delta * D * w[-ny]) + ((one - three * ni * E + rho * F) * w[0]) + (delta * D * w[ny]) -
+                   (ni * E * w[2]) + (four * ni * E * w[1]);
+        }
+        const Stencil<T> s = stencil_at<T, PAT>(p, cs, r, c);
+        const T C = p.gamma * s.C;
+        return C * w[-ny - 1] + s.M * w[-ny] - C * w[-ny + 1] + s.Mt * w[-1] + (one - s.Z - s.W) * w[0] +
+               s.Pt * w[1] - C * w[ny - 1] + s.P * w[ny] + C * w[ny + 1];
+    };
+    T v;
+    if (i == 0) {
+        v = bs[nx + j];  // h_lower(t, Y(eta_j))
+    } else if (i == nx - 1) {
+        const int jj = (int)p.idx2[j];
+        v = (((four * level(nx - 2, jj)) - level(nx - 3, jj) - (two * p.h1 * bs[nx + ny + j])) / three);
+    } else {
+        v = level(i, j);
     }
-    if (i == 0 || i == nx - 1)
-        return;
-    const Stencil<T> s = stencil_at(p, coeff_set(p, step), i, j);
-    const T one = (T)1.0;
-    const T C = p.gamma * s.C;
-    const gptr<const T> w = u + q;
-    nxt[q] = C * w[-ny - 1] + s.M * w[-ny] - C * w[-ny + 1] + s.Mt * w[-1] + (one - s.Z - s.W) * w[0] +
-             s.Pt * w[1] - C * w[ny - 1] + s.P * w[ny] + C * w[ny + 1];
+    nxt[(int64_t)i * ny + j] = v;
 }
 
 // -------------------------------------------------------------------------------------------------
@@ -374,7 +398,9 @@ __global__ void __launch_bounds__(256) k_tridiag_lane(int64_t B, int n, int stri
     template __global__ void k_xsweep<T>(const Prob<T> *, const Tile *, int, int);                            \
     template __global__ void k_vsweep<T>(const Prob<T> *, const Tile *, int, int);                            \
     template __global__ void k_hv_y3<T>(const Prob<T> *, const Tile *, int);                                  \
-    template __global__ void k_euler<T>(const Prob<T> *, const Tile *, int);                                  \
+    template __global__ void k_euler<T, PAT_GENERIC>(const Prob<T> *, const Tile *, int);                     \
+    template __global__ void k_euler<T, PAT_HESTON>(const Prob<T> *, const Tile *, int);                      \
+    template __global__ void k_euler<T, PAT_SABR0>(const Prob<T> *, const Tile *, int);                       \
     template __global__ void k_tridiag_lane<T>(int64_t, int, int, const T *, const T *, const T *, const T *, \
                                                T *, T *, LineBC<T>);
 LSS_INST(double)
