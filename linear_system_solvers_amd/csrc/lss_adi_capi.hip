@@ -32,7 +32,7 @@ template <typename T> __global__ void k_bnd_b(const Prob<T> *, const Tile *, int
 template <typename T> __global__ void k_xsweep(const Prob<T> *, const Tile *, int, int);
 template <typename T> __global__ void k_vsweep(const Prob<T> *, const Tile *, int, int);
 template <typename T> __global__ void k_hv_y3(const Prob<T> *, const Tile *, int);
-template <typename T> __global__ void k_euler(const Prob<T> *, const Tile *, int);
+template <typename T, int PAT> __global__ void k_euler(const Prob<T> *, const Tile *, int);
 template <typename T>
 __global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *, LineBC<T>);
 template <typename T, int M, int STAGE, int W, int PAT, int NC>
@@ -390,6 +390,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             p.bnd = t + L.off_bnd;
             p.coeff_stride = (int64_t)coeff_stride(d);
             p.bnd_stride = d.nx + 2 * d.ny;
+            max_nxy = std::max<int64_t>(max_nxy, (int64_t)d.nx * d.ny);
             uint32_t *ix = (uint32_t *)(base + L.off_idx_bytes);
             p.idx1 = ix;
             p.idx2 = ix + d.nx;
@@ -708,18 +709,22 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             if (prof)
                 prof->end();
         };
-        pre(LSS_K_BND_A);
-        hipLaunchKernelGGL(k_bnd_a<T>, dim3(grid(TB_I)), dim3(TILE), 0, st, pd, td + tile_off[TB_I], s);
-        post();
-        if (euler) {  // heston_euler_time_loop::run (lss_heston_euler_solver_scheme.hpp:63-67)
+        if (euler) {  // heston_euler_time_loop::run (lss_heston_euler_solver_scheme.hpp:63-67), one launch
+            const dim3 g(grid(TB_U)), b(GTR * GTC);
+            const Tile *tu = td + tile_off[TB_U];
             pre(LSS_K_EULER);
-            hipLaunchKernelGGL(k_euler<T>, dim3(grid(TB_U)), dim3(GTR * GTC), 0, st, pd, td + tile_off[TB_U], s);
-            post();
-            pre(LSS_K_BND_B);
-            hipLaunchKernelGGL(k_bnd_b<T>, dim3(grid(TB_J)), dim3(TILE), 0, st, pd, td + tile_off[TB_J], s);
+            if (fpat == PAT_HESTON)
+                hipLaunchKernelGGL((k_euler<T, PAT_HESTON>), g, b, 0, st, pd, tu, s);
+            else if (fpat == PAT_SABR0)
+                hipLaunchKernelGGL((k_euler<T, PAT_SABR0>), g, b, 0, st, pd, tu, s);
+            else
+                hipLaunchKernelGGL((k_euler<T, PAT_GENERIC>), g, b, 0, st, pd, tu, s);
             post();
             return;
         }
+        pre(LSS_K_BND_A);
+        hipLaunchKernelGGL(k_bnd_a<T>, dim3(grid(TB_I)), dim3(TILE), 0, st, pd, td + tile_off[TB_I], s);
+        post();
         auto xs = [&](int stage) {
             if (line_solver == LINE_EXACT) {
                 hipLaunchKernelGGL(k_xsweep<T>, dim3(grid(TB_X)), dim3(TILE), 0, st, pd, td + tile_off[TB_X], s, stage);
@@ -1348,9 +1353,17 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         const hipMemcpyKind kout = host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
         if (!tuned)
             autotune(stream);
-        for (int64_t q = 0; q < count; ++q) {
-            const Prob<T> &p = probs[q];
-            HIPCHK(hipMemcpyAsync(p.a, u_in[order[q]], sizeof(T) * (size_t)p.nx * p.ny, kin, stream));
+        // device surfaces of a plan of many problems: two batched copy launches instead of one copy
+        // per problem and direction
+        const bool batched_io = !host && count >= IO_BATCH_MIN;
+        if (batched_io) {
+            io_jobs(u_in, u_out, stream);
+            launch_jobs((const CopyJob *)io_jobs_dev.p, (int)count, stream);
+        } else {
+            for (int64_t q = 0; q < count; ++q) {
+                const Prob<T> &p = probs[q];
+                HIPCHK(hipMemcpyAsync(p.a, u_in[order[q]], sizeof(T) * (size_t)p.nx * p.ny, kin, stream));
+            }
         }
         if (use_graph) {
             if (!graph_exec) {
@@ -1365,13 +1378,42 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         } else {
             launch_loop(stream);
         }
-        for (int64_t q = 0; q < count; ++q) {
-            const Prob<T> &p = probs[q];
-            const T *res = (p.steps & 1) ? p.b : p.a;
-            HIPCHK(hipMemcpyAsync(u_out[order[q]], res, sizeof(T) * (size_t)p.nx * p.ny, kout, stream));
+        if (batched_io) {
+            launch_jobs((const CopyJob *)io_jobs_dev.p + count, (int)count, stream);
+        } else {
+            for (int64_t q = 0; q < count; ++q) {
+                const Prob<T> &p = probs[q];
+                const T *res = (p.steps & 1) ? p.b : p.a;
+                HIPCHK(hipMemcpyAsync(u_out[order[q]], res, sizeof(T) * (size_t)p.nx * p.ny, kout, stream));
+            }
         }
         if (host)
             HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // copy jobs of the surfaces of a device run (count inputs, then count outputs), rebuilt when the
+    // caller's pointers change
+    static constexpr int64_t IO_BATCH_MIN = 8;
+    dev_buffer io_jobs_dev;
+    std::vector<const void *> io_last;
+    void io_jobs(const void *const *u_in, void *const *u_out, hipStream_t stream)
+    {
+        std::vector<const void *> key(u_in, u_in + count);
+        key.insert(key.end(), u_out, u_out + count);
+        if (key == io_last)
+            return;
+        std::vector<CopyJob> jobs(2 * count);
+        for (int64_t q = 0; q < count; ++q) {
+            const Prob<T> &p = probs[q];
+            const int64_t n = (int64_t)p.nx * p.ny;
+            jobs[q] = {u_in[order[q]], p.a, n};
+            jobs[count + q] = {(p.steps & 1) ? p.b : p.a, u_out[order[q]], n};
+        }
+        if (!io_jobs_dev.p)
+            io_jobs_dev.alloc(jobs.size() * sizeof(CopyJob));
+        HIPCHK(hipStreamSynchronize(stream));  // an earlier run may still read the old table
+        HIPCHK(hipMemcpy(io_jobs_dev.p, jobs.data(), jobs.size() * sizeof(CopyJob), hipMemcpyHostToDevice));
+        io_last = std::move(key);
     }
 };
 
