@@ -188,7 +188,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     // tile tables: rows [0,nx), cols [0,ny), S-lines [1,ny-2], v-lines [1,nx-2], interior rows
     // adjoint: S-line tiles of P_XTJ, v-line tiles of P_VROWS, one tile per row (all rows)
     // forward partitioned sweeps: S-line tiles of 16 (TB_X) or 8 (TB_X8), v-line tiles of 4 (TB_V) or 16 (TB_V16)
-    enum { TB_I = 0, TB_J, TB_X, TB_V, TB_R, TB_AX, TB_AV, TB_U, TB_X8, TB_V16, TB_X4, TB_N };
+    enum { TB_I = 0, TB_J, TB_X, TB_V, TB_R, TB_AX, TB_AV, TB_U, TB_X8, TB_V16, TB_X4, TB_E, TB_N };
     std::vector<Tile> tiles[TB_N];
     size_t tile_off[TB_N] = {};
     std::vector<int> prefix[TB_N];     // prefix[tb][q] = tiles of internal problems [0, q)
@@ -443,10 +443,11 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 case TB_V16: lo = 1, hi = p.nx - 1, step = 16; break;
                 default: lo = 1, hi = p.nx - 1, step = 1; break;
                 }
-                if (tb == TB_U) {  // two-dimensional gather tiles of the adjoint
-                    for (int r = 0; r < p.nx; r += GTR)
+                if (tb == TB_U || tb == TB_E) {  // two-dimensional tiles (adjoint gathers, explicit step)
+                    const int rows = tb == TB_U ? GTR : GTR * ERPT;
+                    for (int r = 0; r < p.nx; r += rows)
                         for (int c = 0; c < p.ny; c += GTC)
-                            tiles[tb].push_back({(int32_t)q, r, std::min(GTR, p.nx - r), c});
+                            tiles[tb].push_back({(int32_t)q, r, std::min(rows, p.nx - r), c});
                 } else {
                     for (int s = lo; s < hi; s += step) {
                         Tile t;
@@ -710,8 +711,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 prof->end();
         };
         if (euler) {  // heston_euler_time_loop::run (lss_heston_euler_solver_scheme.hpp:63-67), one launch
-            const dim3 g(grid(TB_U)), b(GTR * GTC);
-            const Tile *tu = td + tile_off[TB_U];
+            const dim3 g(grid(TB_E)), b(GTR * GTC);
+            const Tile *tu = td + tile_off[TB_E];
             pre(LSS_K_EULER);
             if (fpat == PAT_HESTON)
                 hipLaunchKernelGGL((k_euler<T, PAT_HESTON>), g, b, 0, st, pd, tu, s);

@@ -114,6 +114,7 @@ struct Tile {
 // -------------------------------------------------------------------------------------------------
 constexpr int NPAR = 6;    // model parameters per problem
 constexpr int GTR = 8, GTC = 32;  // gather tiles of the adjoint: rows x columns (one thread per point)
+constexpr int ERPT = 8;           // rows per thread of the explicit step (its tiles: GTR * ERPT rows)
 constexpr int NTERM = 12;  // derivative terms per problem
 
 // Adjoint state of one problem.  The forward values a reverse step needs (U_s, U_{s+1} and the

@@ -287,8 +287,9 @@ __global__ void __launch_bounds__(TILE) k_hv_y3(const Prob<T> *__restrict__ prob
 //             those two values from u with the same arithmetic instead of waiting for them.
 // Every value is rounded as the reference rounds it (this file is compiled without contraction), so
 // with FIELDS coefficients the step is bit-identical to the three stages run one after another.
-// PAT selects compile-time factor patterns of SEPARABLE plans.  Two-dimensional tiles of GTR x GTC
-// points, one thread per point, lanes along the contiguous v direction.
+// PAT selects compile-time factor patterns of SEPARABLE plans.  Two-dimensional tiles of GTR * ERPT
+// rows x GTC columns; a thread takes ERPT consecutive rows of one column (lanes along the contiguous v
+// direction) and reads the rows of its three columns once.
 // -------------------------------------------------------------------------------------------------
 template <typename T, int PAT>
 __global__ void __launch_bounds__(GTR *GTC) k_euler(const Prob<T> *__restrict__ probs,
@@ -298,17 +299,19 @@ __global__ void __launch_bounds__(GTR *GTC) k_euler(const Prob<T> *__restrict__ 
     const Prob<T> &p = probs[tl.prob];
     if (step >= p.steps)
         return;
-    const int tr = threadIdx.x / GTC, tc = threadIdx.x % GTC;
-    const int i = tl.first + tr, j = tl.col + tc;
+    const int tg = threadIdx.x / GTC, tc = threadIdx.x % GTC;
+    const int j = tl.col + tc;
     const int nx = p.nx, ny = p.ny;
-    if (tr >= tl.count || j >= ny)
+    const int r0 = tl.first + tg * ERPT;               // this thread's rows r0 .. r0 + rn - 1
+    const int rn = min(ERPT, tl.first + tl.count - r0);
+    if (j >= ny || rn <= 0)
         return;
     const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
     const gptr<T> nxt = G((step & 1) ? p.a : p.b);
     const gptr<const T> cs = coeff_set(p, step);
     const gptr<const T> bs = bnd_set(p, step);
     const T one = (T)1.0, two = (T)2.0, three = (T)3.0, four = (T)4.0;
-    // the value stages "interior" and A give point (r, c), 1 <= r <= nx - 2
+    // the value stages "interior" and A give point (r, c), 1 <= r <= nx - 2, from direct loads
     auto level = [&](int r, int c) -> T {
         const gptr<const T> w = u + ((int64_t)r * ny + c);
         if (c == ny - 1)
@@ -325,16 +328,43 @@ __global__ void __launch_bounds__(GTR *GTC) k_euler(const Prob<T> *__restrict__ 
         return C * w[-ny - 1] + s.M * w[-ny] - C * w[-ny + 1] + s.Mt * w[-1] + (one - s.Z - s.W) * w[0] +
                s.Pt * w[1] - C * w[ny - 1] + s.P * w[ny] + C * w[ny + 1];
     };
-    T v;
-    if (i == 0) {
-        v = bs[nx + j];  // h_lower(t, Y(eta_j))
-    } else if (i == nx - 1) {
-        const int jj = (int)p.idx2[j];
-        v = (((four * level(nx - 2, jj)) - level(nx - 3, jj) - (two * p.h1 * bs[nx + ny + j])) / three);
-    } else {
-        v = level(i, j);
+    // rows r0 - 1 .. r0 + ERPT of columns j - 1, j, j + 1 (clamped into the grid), all loaded before
+    // the first store, so that each value is read once and the interior points share them
+    const int jm = max(j - 1, 0), jp = min(j + 1, ny - 1);
+    T w[ERPT + 2][3];
+#pragma unroll
+    for (int t = 0; t < ERPT + 2; ++t) {
+        const gptr<const T> row = u + (int64_t)min(max(r0 - 1 + t, 0), nx - 1) * ny;
+        w[t][0] = row[jm];
+        w[t][1] = row[j];
+        w[t][2] = row[jp];
     }
-    nxt[(int64_t)i * ny + j] = v;
+    T out[ERPT];
+#pragma unroll
+    for (int t = 0; t < ERPT; ++t) {
+        const int i = r0 + t;
+        T v = (T)0;
+        if (t < rn) {
+            if (i == 0) {
+                v = bs[nx + j];  // h_lower(t, Y(eta_j))
+            } else if (i == nx - 1) {
+                const int jj = (int)p.idx2[j];
+                v = (((four * level(nx - 2, jj)) - level(nx - 3, jj) - (two * p.h1 * bs[nx + ny + j])) / three);
+            } else if (j == 0 || j == ny - 1) {
+                v = level(i, j);
+            } else {
+                const Stencil<T> s = stencil_at<T, PAT>(p, cs, i, j);
+                const T C = p.gamma * s.C;
+                v = C * w[t][0] + s.M * w[t][1] - C * w[t][2] + s.Mt * w[t + 1][0] + (one - s.Z - s.W) * w[t + 1][1] +
+                    s.Pt * w[t + 1][2] - C * w[t + 2][0] + s.P * w[t + 2][1] + C * w[t + 2][2];
+            }
+        }
+        out[t] = v;
+    }
+#pragma unroll
+    for (int t = 0; t < ERPT; ++t)
+        if (t < rn)
+            nxt[(int64_t)(r0 + t) * ny + j] = out[t];
 }
 
 // -------------------------------------------------------------------------------------------------
