@@ -43,7 +43,7 @@ MAX_PARAMS = 6
 MAX_TERMS = 12
 HESTON_PARAMS = ("kappa", "theta_v", "sigma", "rho", "r")
 SABR_PARAMS = ("nu", "rho", "beta")
-KERNEL_CONFIG_LEN = 11
+KERNEL_CONFIG_LEN = 12
 KERNEL_KINDS = ("bnd_a", "xsweep1", "vsweep1", "hv_y3", "xsweep2", "vsweep2", "bnd_b", "euler")
 
 
@@ -260,7 +260,7 @@ class Plan:
         c = (C.c_int32 * KERNEL_CONFIG_LEN)()
         _check(lib().lss_plan_kernel_config(self._h, c))
         keys = ("line_solver", "s_lines_per_group", "v_lines_per_group", "xcd_order", "factor_pattern",
-                "s_chunk_bound", "v_chunk_bound", "tuned", "adjoint_pattern", "adjoint_s_lines_per_group", "adjoint_overlap")
+                "s_chunk_bound", "v_chunk_bound", "tuned", "adjoint_pattern", "adjoint_s_lines_per_group", "adjoint_overlap", "graph_groups")
         return dict(zip(keys, list(c)))
 
     def set_mapping(self, s_lines_per_group: int, v_lines_per_group: int, xcd_order: int = 0) -> None:

@@ -192,6 +192,32 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     std::vector<Tile> tiles[TB_N];
     size_t tile_off[TB_N] = {};
     std::vector<int> prefix[TB_N];     // prefix[tb][q] = tiles of internal problems [0, q)
+    // Graph replays of plans of GRAPH_GROUP_MIN or more problems run as GROUPS concurrent groups
+    // (internal problem q belongs to group q mod GROUPS, so every group keeps the steps-descending
+    // order), one graph per group on a stream of the plan's own: the groups' launches share the CUs.
+    // Measured on the 2048-problem Heston book: 2 groups 4.4 % faster than one, 3 and 4 groups less.
+    static constexpr int GROUPS = 2;
+    static constexpr int64_t GRAPH_GROUP_MIN = 512;
+    int ngroups = 1;
+    int set_ = 0;                      // tiles of the launches: 0 every problem, g + 1 group g
+    std::vector<int> gprefix[GROUPS][TB_N];  // gprefix[g][tb][k] = tiles of group g's first k problems
+    size_t gtile_off[GROUPS][TB_N] = {};
+    hipStream_t gstream[GROUPS] = {};
+    hipGraph_t ggraph[GROUPS] = {};
+    hipGraphExec_t gexec[GROUPS] = {};
+    hipEvent_t gfork = nullptr, gjoin[GROUPS] = {};
+
+    // launch grid of tile table tb over the na problems still stepping, and the first tile, in the
+    // current tile set
+    int pre(int tb, int64_t na) const
+    {
+        if (set_ == 0)
+            return prefix[tb][na];
+        const int g = set_ - 1;
+        const int64_t k = na > g ? (na - g + ngroups - 1) / ngroups : 0;
+        return gprefix[g][tb][k];
+    }
+    size_t toff(int tb) const { return set_ == 0 ? tile_off[tb] : gtile_off[set_ - 1][tb]; }
     int max_steps = 0;
     int line_solver = LINE_EXACT;
     int pmx = 1, pmy = 1;              // points per chunk bound (template M) of S-lines and v-lines
@@ -215,6 +241,18 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             (void)hipStreamDestroy(cap_stream);
         if (aux_stream)
             (void)hipStreamDestroy(aux_stream);
+        for (int g = 0; g < GROUPS; ++g) {
+            if (gexec[g])
+                (void)hipGraphExecDestroy(gexec[g]);
+            if (ggraph[g])
+                (void)hipGraphDestroy(ggraph[g]);
+            if (gstream[g])
+                (void)hipStreamDestroy(gstream[g]);
+            if (gjoin[g])
+                (void)hipEventDestroy(gjoin[g]);
+        }
+        if (gfork)
+            (void)hipEventDestroy(gfork);
         for (hipEvent_t e : {ev_fork, ev_ready[0], ev_ready[1], ev_free[0], ev_free[1]})
             if (e)
                 (void)hipEventDestroy(e);
@@ -466,10 +504,28 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             tile_off[tb] = ntiles;
             ntiles += tiles[tb].size();
         }
+        // the groups' tile tables (copies of their problems' tiles)
+        ngroups = n >= GRAPH_GROUP_MIN ? GROUPS : 1;
+        std::vector<Tile> gtiles[GROUPS][TB_N];
+        for (int g = 0; g < ngroups && ngroups > 1; ++g)
+            for (int tb = 0; tb < TB_N; ++tb) {
+                gprefix[g][tb].assign(1, 0);
+                for (int64_t q = g; q < n; q += ngroups) {
+                    gtiles[g][tb].insert(gtiles[g][tb].end(), tiles[tb].begin() + prefix[tb][q],
+                                         tiles[tb].begin() + prefix[tb][q + 1]);
+                    gprefix[g][tb].push_back((int)gtiles[g][tb].size());
+                }
+                gtile_off[g][tb] = ntiles;
+                ntiles += gtiles[g][tb].size();
+            }
         tiles_dev.alloc(ntiles * sizeof(Tile));
         for (int tb = 0; tb < TB_N; ++tb)
             HIPCHK(hipMemcpy((Tile *)tiles_dev.p + tile_off[tb], tiles[tb].data(), tiles[tb].size() * sizeof(Tile),
                              hipMemcpyHostToDevice));
+        for (int g = 0; g < ngroups && ngroups > 1; ++g)
+            for (int tb = 0; tb < TB_N; ++tb)
+                HIPCHK(hipMemcpy((Tile *)tiles_dev.p + gtile_off[g][tb], gtiles[g][tb].data(),
+                                 gtiles[g][tb].size() * sizeof(Tile), hipMemcpyHostToDevice));
         probs_dev.alloc(n * sizeof(Prob<T>));
         HIPCHK(hipMemcpy(probs_dev.p, probs.data(), n * sizeof(Prob<T>), hipMemcpyHostToDevice));
     }
@@ -520,8 +576,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     template <int M, int W, int PAT> void launch_x_p(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
         const int tb = W == 16 ? TB_X : (W == 8 ? TB_X8 : TB_X4);
-        const dim3 g((unsigned)prefix[tb][na]);
-        const Tile *tx = (const Tile *)tiles_dev.p + tile_off[tb];
+        const dim3 g((unsigned)pre(tb, na));
+        const Tile *tx = (const Tile *)tiles_dev.p + toff(tb);
         if constexpr (M >= 16) {
             if constexpr (W <= 8) {
                 if (stage == 1)
@@ -567,8 +623,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     template <int M, int VR, int PAT> void launch_v_p(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
     {
         const int tb = VR == 4 ? TB_V : TB_V16;
-        const dim3 g((unsigned)prefix[tb][na]), b(64 * VR);
-        hipLaunchKernelGGL((k_vsweep_p<T, M, VR, PAT>), g, b, 0, st, pd, (const Tile *)tiles_dev.p + tile_off[tb],
+        const dim3 g((unsigned)pre(tb, na)), b(64 * VR);
+        hipLaunchKernelGGL((k_vsweep_p<T, M, VR, PAT>), g, b, 0, st, pd, (const Tile *)tiles_dev.p + toff(tb),
                            s, stage, swz);
     }
     template <int M, int VR> void launch_v_m(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)
@@ -605,6 +661,45 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             HIPCHK(hipGraphDestroy(graph));
             graph = nullptr;
         }
+        for (int g = 0; g < GROUPS; ++g)
+            if (gexec[g]) {
+                HIPCHK(hipGraphExecDestroy(gexec[g]));
+                gexec[g] = nullptr;
+                HIPCHK(hipGraphDestroy(ggraph[g]));
+                ggraph[g] = nullptr;
+            }
+    }
+
+    // the time loop as ngroups graphs (captured once), run concurrently after the work queued on
+    // `stream` and joined back into it
+    void run_groups(hipStream_t stream)
+    {
+        if (!gexec[0]) {
+            if (!gfork) {
+                HIPCHK(hipEventCreateWithFlags(&gfork, hipEventDisableTiming));
+                for (int g = 0; g < ngroups; ++g) {
+                    HIPCHK(hipStreamCreateWithFlags(&gstream[g], hipStreamNonBlocking));
+                    HIPCHK(hipEventCreateWithFlags(&gjoin[g], hipEventDisableTiming));
+                }
+            }
+            for (int g = 0; g < ngroups; ++g) {
+                set_ = g + 1;
+                HIPCHK(hipStreamBeginCapture(gstream[g], hipStreamCaptureModeThreadLocal));
+                for (int s = 0; s < probs[g].steps; ++s)  // group g's first problem has its most steps
+                    launch_step(s, gstream[g]);
+                set_ = 0;
+                HIPCHK(hipStreamEndCapture(gstream[g], &ggraph[g]));
+                HIPCHK(hipGraphInstantiate(&gexec[g], ggraph[g], nullptr, nullptr, 0));
+            }
+        }
+        HIPCHK(hipEventRecord(gfork, stream));
+        for (int g = 0; g < ngroups; ++g) {
+            HIPCHK(hipStreamWaitEvent(gstream[g], gfork, 0));
+            HIPCHK(hipGraphLaunch(gexec[g], gstream[g]));
+            HIPCHK(hipEventRecord(gjoin[g], gstream[g]));
+        }
+        for (int g = 0; g < ngroups; ++g)
+            HIPCHK(hipStreamWaitEvent(stream, gjoin[g], 0));
     }
 
     void tune(hipStream_t stream, bool again, int *x, int *v, int *z) override
@@ -631,6 +726,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         cfg[8] = ck > 0 ? apat : -1;
         cfg[9] = ck > 0 ? axw : 0;
         cfg[10] = ck > 0 ? (dbl ? 1 : 0) : -1;
+        cfg[11] = ngroups;
     }
 
     void set_mapping(int x, int v, int z) override
@@ -696,7 +792,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         const int64_t na = active(s);
         const Prob<T> *pd = table ? table : (const Prob<T> *)probs_dev.p;
         const Tile *td = (const Tile *)tiles_dev.p;
-        auto grid = [&](int tb) { return (unsigned)prefix[tb][na]; };
+        auto grid = [&](int tb) { return (unsigned)pre(tb, na); };
         bool any_split2 = false, any_hv = false;
         for (int64_t q = 0; q < na; ++q) {
             any_split2 |= probs[q].scheme != SCHEME_DR;
@@ -712,7 +808,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         };
         if (euler) {  // heston_euler_time_loop::run (lss_heston_euler_solver_scheme.hpp:63-67), one launch
             const dim3 g(grid(TB_E)), b(GTR * GTC);
-            const Tile *tu = td + tile_off[TB_E];
+            const Tile *tu = td + toff(TB_E);
             pre(LSS_K_EULER);
             if (fpat == PAT_HESTON)
                 hipLaunchKernelGGL((k_euler<T, PAT_HESTON>), g, b, 0, st, pd, tu, s);
@@ -724,18 +820,18 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             return;
         }
         pre(LSS_K_BND_A);
-        hipLaunchKernelGGL(k_bnd_a<T>, dim3(grid(TB_I)), dim3(TILE), 0, st, pd, td + tile_off[TB_I], s);
+        hipLaunchKernelGGL(k_bnd_a<T>, dim3(grid(TB_I)), dim3(TILE), 0, st, pd, td + toff(TB_I), s);
         post();
         auto xs = [&](int stage) {
             if (line_solver == LINE_EXACT) {
-                hipLaunchKernelGGL(k_xsweep<T>, dim3(grid(TB_X)), dim3(TILE), 0, st, pd, td + tile_off[TB_X], s, stage);
+                hipLaunchKernelGGL(k_xsweep<T>, dim3(grid(TB_X)), dim3(TILE), 0, st, pd, td + toff(TB_X), s, stage);
                 return;
             }
             launch_x(stage, s, st, pd, na);
         };
         auto vs = [&](int stage) {
             if (line_solver == LINE_EXACT) {
-                hipLaunchKernelGGL(k_vsweep<T>, dim3(grid(TB_V)), dim3(TILE), 0, st, pd, td + tile_off[TB_V], s, stage);
+                hipLaunchKernelGGL(k_vsweep<T>, dim3(grid(TB_V)), dim3(TILE), 0, st, pd, td + toff(TB_V), s, stage);
                 return;
             }
             launch_v(stage, s, st, pd, na);
@@ -748,7 +844,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         post();
         if (any_hv && line_solver == LINE_EXACT) {
             pre(LSS_K_HV_Y3);
-            hipLaunchKernelGGL(k_hv_y3<T>, dim3(grid(TB_R)), dim3(TILE), 0, st, pd, td + tile_off[TB_R], s);
+            hipLaunchKernelGGL(k_hv_y3<T>, dim3(grid(TB_R)), dim3(TILE), 0, st, pd, td + toff(TB_R), s);
             post();
         }
         if (any_split2) {
@@ -760,7 +856,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             post();
         }
         pre(LSS_K_BND_B);
-        hipLaunchKernelGGL(k_bnd_b<T>, dim3(grid(TB_J)), dim3(TILE), 0, st, pd, td + tile_off[TB_J], s);
+        hipLaunchKernelGGL(k_bnd_b<T>, dim3(grid(TB_J)), dim3(TILE), 0, st, pd, td + toff(TB_J), s);
         post();
     }
 
@@ -1366,7 +1462,9 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 HIPCHK(hipMemcpyAsync(p.a, u_in[order[q]], sizeof(T) * (size_t)p.nx * p.ny, kin, stream));
             }
         }
-        if (use_graph) {
+        if (use_graph && ngroups > 1) {
+            run_groups(stream);
+        } else if (use_graph) {
             if (!graph_exec) {
                 HIPCHK(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
                 HIPCHK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));

@@ -302,3 +302,32 @@ def test_book_problems_unstable_in_the_reference_match_it_on_the_exact_path(orac
     for q, g in zip(stable, fast):
         c = ref[q]
         assert float(np.abs(g - c).max() / np.abs(c).max()) <= 1e-10
+
+
+def test_graph_groups_of_a_large_plan_equal_direct_launches(oracle):
+    """A plan of 512 or more problems replays its time loop as two graphs on two streams (problems
+    alternate between the groups); every replay must equal the direct launches bit for bit, with
+    the caller's stream ordered around them, and the problems must match their oracle solves."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(31)
+    models = [_stable(int(rng.integers(5, 30)), int(rng.integers(5, 20)), steps=int(rng.integers(1, 14)),
+                      splitting=q % 4, strike=float(rng.uniform(60, 140)), coeff_mode=L.COEFF_SEPARABLE,
+                      line_solver=L.LINE_PARTITIONED)
+              for q in range(520)]
+    plan = L.Plan.from_models(models)
+    assert plan.kernel_config()["graph_groups"] == 2
+    ins = [torch.from_numpy(plan.initial(p)).cuda() for p in range(plan.count)]
+    st = torch.cuda.current_stream().cuda_stream
+    res = []
+    for graph in (False, True, True, True):
+        outs = [torch.full_like(u, float("nan")) for u in ins]
+        plan.run_dev([u.data_ptr() for u in ins], [o.data_ptr() for o in outs], st, graph=graph)
+        got = [(o * 1.0).cpu().numpy() for o in outs]  # stream-ordered work right after the replay
+        torch.cuda.synchronize()
+        res.append(got)
+    for k in range(1, 4):
+        assert all(np.array_equal(a, b) for a, b in zip(res[0], res[k]))
+    for q in range(0, 520, 37):
+        c = oracle_solve(oracle, models[q])
+        assert _normwise(res[1][q], c) <= 1e-12
+    plan.close()
