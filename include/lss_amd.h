@@ -130,7 +130,7 @@ int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v
  * S-lines per adjoint S-sweep workgroup (0 before first use) and whether the backward pass overlaps
  * the recomputation of the next segment with the adjoint of the current one (two segment storage
  * sets, for plans whose second set takes at most 2 GiB; -1 before first use), and the number of
- * concurrent groups a graph replay of lss_adi_run_dev runs the problems in (2 for plans of 512 or
+ * concurrent groups a graph replay of lss_adi_run_dev runs the problems in (2 for plans of two or
  * more problems, else 1). */
 enum { LSS_KERNEL_CONFIG_LEN = 12 };
 int lss_plan_kernel_config(const lss_adi_plan *plan, int32_t *config);

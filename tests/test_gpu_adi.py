@@ -305,7 +305,7 @@ def test_book_problems_unstable_in_the_reference_match_it_on_the_exact_path(orac
 
 
 def test_graph_groups_of_a_large_plan_equal_direct_launches(oracle):
-    """A plan of 512 or more problems replays its time loop as two graphs on two streams (problems
+    """A plan of two or more problems replays its time loop as two graphs on two streams (problems
     alternate between the groups); every replay must equal the direct launches bit for bit, with
     the caller's stream ordered around them, and the problems must match their oracle solves."""
     torch = pytest.importorskip("torch")
