@@ -685,9 +685,18 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             }
             for (int g = 0; g < ngroups; ++g) {
                 set_ = g + 1;
-                HIPCHK(hipStreamBeginCapture(gstream[g], hipStreamCaptureModeThreadLocal));
-                for (int s = 0; s < probs[g].steps; ++s)  // group g's first problem has its most steps
-                    launch_step(s, gstream[g]);
+                try {
+                    HIPCHK(hipStreamBeginCapture(gstream[g], hipStreamCaptureModeThreadLocal));
+                    for (int s = 0; s < probs[g].steps; ++s)  // group g's first problem has its most steps
+                        launch_step(s, gstream[g]);
+                } catch (...) {
+                    set_ = 0;
+                    hipGraph_t partial = nullptr;
+                    if (hipStreamEndCapture(gstream[g], &partial) == hipSuccess && partial)
+                        (void)hipGraphDestroy(partial);
+                    drop_graph();
+                    throw;
+                }
                 set_ = 0;
                 HIPCHK(hipStreamEndCapture(gstream[g], &ggraph[g]));
                 HIPCHK(hipGraphInstantiate(&gexec[g], ggraph[g], nullptr, nullptr, 0));
