@@ -123,13 +123,17 @@ int lss_adi_autotune(lss_adi_plan *plan, void *stream, int32_t again, int32_t *s
 /* Sets the mapping instead of timing it (16, 8 or 4 S-lines, 4 or 16 v-lines per workgroup, 0 or 1). */
 int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v_lines_per_group,
                         int32_t xcd_order);
+/* (no counterpart in the reference) How lss_adi_backward schedules its recomputation: 1 always with
+ * a second segment store on a second stream, 0 never, -1 (the default) when the second store takes
+ * at most 2 GiB.  Takes effect at the next lss_adi_forward. */
+int lss_adi_set_adjoint_overlap(lss_adi_plan *plan, int32_t mode);
 /* Kernel configuration of a plan (LSS_KERNEL_CONFIG_LEN values): line solver, S-lines and v-lines per
  * workgroup, XCD order, compile-time factor pattern of the coefficients (0 none, 1 Heston, 2 SABR with
  * r = 0), chunk bounds of S-lines and v-lines, whether the autotuner has run, the compile-time pattern
  * of the adjoint kernels (factor and derivative-term tables; -1 before the adjoint's first use), the
  * S-lines per adjoint S-sweep workgroup (0 before first use) and whether the backward pass overlaps
  * the recomputation of the next segment with the adjoint of the current one (two segment storage
- * sets, for plans whose second set takes at most 2 GiB; -1 before first use), and the number of
+ * sets, see lss_adi_set_adjoint_overlap; -1 before first use), and the number of
  * concurrent groups a graph replay of lss_adi_run_dev runs the problems in (2 for plans of two or
  * more problems, else 1). */
 enum { LSS_KERNEL_CONFIG_LEN = 12 };
@@ -227,12 +231,12 @@ int lss_adi_run_profiled(lss_adi_plan *plan, const void *const *u_in, void *cons
  *                     ckpt_every steps from its checkpoint and runs the adjoint steps backwards.  Writes
  *                     lam_in[p] = dJ/du_in[p] (device; a NULL entry or array skips it) and, on the host,
  *                     dparams[p * LSS_MAX_PARAMS + k] = dJ/dparam_k of problem p.  Synchronous.
- *                     Needs nx - 2 <= 1024 and ny - 2 <= 1024.  When a second segment store
+ *                     Needs nx - 2 <= 1024 and ny - 2 <= 1024.  With a second segment store
  *                     (ckpt_every + 1 levels and the stage outputs of ckpt_every steps, over all
- *                     problems) takes at most 2 GiB, the plan keeps two, and the recomputation of
- *                     the next segment runs on a stream of the plan's own (ordered after the work
- *                     already queued on `stream`) while the adjoint of the current one runs on
- *                     `stream`; every result is ready when the call returns.
+ *                     problems; see lss_adi_set_adjoint_overlap) the recomputation of the next
+ *                     segment runs on a stream of the plan's own (ordered after the work already
+ *                     queued on `stream`) while the adjoint of the current one runs on `stream`;
+ *                     every result is ready when the call returns.
  * Parameters enter through the transformed coefficients: a derivative term says
  *   d q(i, j) / d param_k = X_t[i] * Y_t[j]   for q one of a..f (LSS_Q_*),
  * i.e. the derivative of heat_data_transform's coefficient at grid point (i, j)

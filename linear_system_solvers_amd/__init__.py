@@ -38,7 +38,7 @@ EXPORTS = ("lss_last_error", "lss_version", "lss_adi_plan_create", "lss_adi_plan
            "lss_adi_run_host", "lss_model_plan_create", "lss_model_initial", "lss_model_grid", "lss_plan_problem_shape",
            "lss_tridiag_batched", "lss_adi_run_profiled", "lss_adi_set_param_terms", "lss_plan_param_count",
            "lss_adi_forward", "lss_adi_backward", "lss_adi_autotune",
-           "lss_adi_set_mapping", "lss_plan_kernel_config", "lss_tridiag_batched_bc")
+           "lss_adi_set_mapping", "lss_plan_kernel_config", "lss_tridiag_batched_bc", "lss_adi_set_adjoint_overlap")
 MAX_PARAMS = 6
 MAX_TERMS = 12
 HESTON_PARAMS = ("kappa", "theta_v", "sigma", "rho", "r")
@@ -165,6 +165,7 @@ def lib():
                                               C.POINTER(C.c_int32), C.c_int64, C.c_void_p]
         L.lss_plan_param_count.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]
         L.lss_adi_set_mapping.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
+        L.lss_adi_set_adjoint_overlap.argtypes = [C.c_void_p, C.c_int32]
         L.lss_plan_kernel_config.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
         L.lss_adi_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
                                        C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -265,6 +266,11 @@ class Plan:
 
     def set_mapping(self, s_lines_per_group: int, v_lines_per_group: int, xcd_order: int = 0) -> None:
         _check(lib().lss_adi_set_mapping(self._h, s_lines_per_group, v_lines_per_group, xcd_order))
+
+    def set_adjoint_overlap(self, mode: int) -> None:
+        """-1 automatic (default), 0 never, 1 always: recomputation of the next segment on a second
+        stream during the backward pass; takes effect at the next forward()."""
+        _check(lib().lss_adi_set_adjoint_overlap(self._h, mode))
 
     # ---- reverse mode -------------------------------------------------------------------------
     def param_count(self, p: int) -> int:

@@ -147,6 +147,7 @@ struct lss_adi_plan {
     virtual void invalidate_adjoint() = 0;
     virtual void tune(hipStream_t stream, bool again, int *xw, int *vr, int *xcd) = 0;
     virtual void set_mapping(int xw, int vr, int xcd) = 0;
+    virtual void set_overlap(int mode) = 0;
     virtual void kernel_config(int32_t *cfg) const = 0;
     virtual void run(const void *const *u_in, void *const *u_out, hipStream_t stream, bool use_graph, bool host) = 0;
     virtual void profile(const void *const *u_in, void *const *u_out, hipStream_t stream, double *ms, double *bytes,
@@ -739,6 +740,13 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         cfg[11] = ngroups;
     }
 
+    void set_overlap(int mode) override
+    {
+        require(mode >= -1 && mode <= 1, "adjoint overlap mode must be -1, 0 or 1");
+        overlap_mode = mode;
+        invalidate_adjoint();
+    }
+
     void set_mapping(int x, int v, int z) override
     {
         require((x == 16 || x == 8 || x == 4) && (v == 4 || v == 16) && (z == 0 || z == 1),
@@ -888,9 +896,12 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     int axw = 16;                                 // S-lines per adjoint x-sweep workgroup
     int apat = PAT_GENERIC;                       // compile-time factor and term tables of the adjoint
     // Two segment storage sets (dbl): the backward pass then recomputes segment c - 1 on aux_stream
-    // while the adjoint steps of segment c run on the caller's stream.  Used when the second set
-    // takes at most SEG2_MAX_BYTES (plans of few problems, whose kernels alone leave CUs idle).
+    // while the adjoint steps of segment c run on the caller's stream.  overlap_mode -1 (automatic)
+    // uses them when the second set takes at most SEG2_MAX_BYTES: plans of few problems, whose
+    // kernels alone leave CUs idle.  (The 256-problem SABR book, which fills the GPU, gained 1 % from
+    // a 35 GB second set.)
     static constexpr size_t SEG2_MAX_BYTES = (size_t)2 << 30;
+    int overlap_mode = -1;
     bool dbl = false;
     hipStream_t aux_stream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_ready[2] = {}, ev_free[2] = {};
@@ -1002,7 +1013,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             const size_t nys = (dr ? 1 : 3) + ((hv && line_solver == LINE_EXACT) ? 1 : 0);
             seg_bytes += ((size_t)(K + 1) + (size_t)K * nys) * nxy * sizeof(T);
         }
-        dbl = seg_bytes <= SEG2_MAX_BYTES;
+        dbl = overlap_mode >= 0 ? overlap_mode == 1 : seg_bytes <= SEG2_MAX_BYTES;
         for (int64_t q = 0; q < n; ++q) {
             const Prob<T> &p = probs[q];
             const size_t nxy = (size_t)p.nx * p.ny, nx_ = (size_t)p.nx, ny_ = (size_t)p.ny;
@@ -2075,6 +2086,14 @@ int lss_adi_set_mapping(lss_adi_plan *plan, int32_t s_lines_per_group, int32_t v
     return guarded([&] {
         require(plan != nullptr, "null plan");
         plan->set_mapping(s_lines_per_group, v_lines_per_group, xcd_order);
+    });
+}
+
+int lss_adi_set_adjoint_overlap(lss_adi_plan *plan, int32_t mode)
+{
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        plan->set_overlap(mode);
     });
 }
 

@@ -214,24 +214,28 @@ def test_specialized_adjoint_kernels_match_the_generic_ones(model):
 
 
 def test_overlapped_backward_matches_the_single_stream_one():
-    """A plan whose second segment storage set fits in 2 GiB recomputes segment c - 1 on a second
-    stream while the adjoint of segment c runs; one whose set does not runs both on the caller's
-    stream.  The checkpoint interval alone decides here (600 x 460 MCS, theta 0.5, 300 steps: 256
-    steps per segment need 2.3 GB per set, 32 steps 0.28 GB), and the gradients must not depend on
-    the schedule."""
+    """With a second segment store the backward pass recomputes segment c - 1 on a second stream
+    while the adjoint of segment c runs; without it both run on the caller's stream.  The gradients
+    must not depend on the schedule (600 x 460 MCS, theta 0.5, 300 steps, segments of 32)."""
     m = _heston(2, nx=600, ny=460, steps=300, weighting=0.5)
     plan = L.Plan.from_models([m])
     u0 = plan.initial(0)
     w = _price_weights(plan, 0, 100.0, 0.04)
     res = {}
-    for ckpt in (256, 32):
-        out, g, dp = _grad(plan, [u0], [w], ckpt=ckpt)
-        res[ckpt] = (plan.kernel_config()["adjoint_overlap"], out[0], g[0], dp[0])
-    assert res[256][0] == 0 and res[32][0] == 1
-    assert np.isfinite(res[256][2]).all() and np.isfinite(res[256][3]).all()
-    np.testing.assert_array_equal(res[256][1], res[32][1])
-    gd = np.abs(res[256][2] - res[32][2]).max() / np.abs(res[256][2]).max()
-    print(f"overlapped vs single-stream backward: surface gradient {gd:.2e}, parameters {res[32][3][:5]} vs "
-          f"{res[256][3][:5]}", flush=True)
+    for mode in (0, 1):
+        plan.set_adjoint_overlap(mode)
+        out, g, dp = _grad(plan, [u0], [w], ckpt=32)
+        res[mode] = (plan.kernel_config()["adjoint_overlap"], out[0], g[0], dp[0])
+    assert res[0][0] == 0 and res[1][0] == 1
+    assert np.isfinite(res[0][2]).all() and np.isfinite(res[0][3]).all()
+    np.testing.assert_array_equal(res[0][1], res[1][1])
+    gd = np.abs(res[0][2] - res[1][2]).max() / np.abs(res[0][2]).max()
+    print(f"overlapped vs single-stream backward: surface gradient {gd:.2e}, parameters {res[1][3][:5]} vs "
+          f"{res[0][3][:5]}", flush=True)
     assert gd <= 1e-13
-    np.testing.assert_allclose(res[32][3], res[256][3], rtol=1e-12, atol=1e-14 * np.abs(res[256][3]).max())
+    np.testing.assert_allclose(res[1][3], res[0][3], rtol=1e-12, atol=1e-14 * np.abs(res[0][3]).max())
+    plan.set_adjoint_overlap(-1)  # automatic: this plan's 0.28 GB second store is under 2 GiB
+    _grad(plan, [u0], [w], ckpt=32)
+    assert plan.kernel_config()["adjoint_overlap"] == 1
+    with pytest.raises(L.LssError):
+        plan.set_adjoint_overlap(2)
