@@ -474,13 +474,16 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 case TB_I: lo = 0, hi = p.nx; break;
                 case TB_J: lo = 0, hi = p.ny; break;
                 case TB_X: lo = 1, hi = p.ny - 1, step = line_solver == LINE_PARTITIONED ? P_XTJ : TILE; break;
-                case TB_V: lo = 1, hi = p.nx - 1, step = line_solver == LINE_PARTITIONED ? P_VROWS : TILE; break;
+                case TB_V:
+                    lo = 1, hi = p.nx - 1;
+                    step = line_solver == LINE_PARTITIONED ? P_VROWS * vlines_per_wave(p.ny - 2, pmy) : TILE;
+                    break;
                 case TB_AX: lo = 1, hi = p.ny - 1, step = P_XTJ; break;
                 case TB_AV: lo = 1, hi = p.nx - 1, step = P_VROWS; break;
                 case TB_U: lo = 0, hi = p.nx, step = 1; break;
                 case TB_X8: lo = 1, hi = p.ny - 1, step = 8; break;
                 case TB_X4: lo = 1, hi = p.ny - 1, step = 4; break;
-                case TB_V16: lo = 1, hi = p.nx - 1, step = 16; break;
+                case TB_V16: lo = 1, hi = p.nx - 1, step = 16 * vlines_per_wave(p.ny - 2, pmy); break;
                 default: lo = 1, hi = p.nx - 1, step = 1; break;
                 }
                 if (tb == TB_U || tb == TB_E) {  // two-dimensional tiles (adjoint gathers, explicit step)

@@ -228,12 +228,14 @@ template <int M, typename F> __device__ __forceinline__ void with_chunk_len(int 
     f(std::integral_constant<int, M>{});
 }
 
-// One line per wavefront; lane = chunk; the interface system is reduced with shuffles.
-// Every lane of the wavefront must call it; chunk_len(N) <= M.
-template <typename T, int M, bool PUT_ALL = false, typename EqF, typename PutF>
+// One line per NL lanes (NL = 64: one line per wavefront; 32: two), lane = chunk (its index within
+// the line's lanes); the interface system is reduced with shuffles inside each group of NL lanes.
+// Every lane of the wavefront must call it, and the lines of a wavefront must have the same N;
+// chunk_len(N, NL) <= M.
+template <typename T, int M, bool PUT_ALL = false, int NL = NCH, typename EqF, typename PutF>
 __device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put)
 {
-    with_chunk_len<M>(chunk_len(N), [&](auto cc) {
+    with_chunk_len<M>(chunk_len(N, NL), [&](auto cc) {
         constexpr int C = decltype(cc)::value;
         const int s = 1 + lane * C;
         Chunk<T, C> ch;
@@ -247,31 +249,31 @@ __device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put
         }
         T PF, QF, RF;
         ch.first_rel(PF, QF, RF);
-        T nPF = __shfl_down(PF, 1), nQF = __shfl_down(QF, 1), nRF = __shfl_down(RF, 1);
-        if (last_real || lane == NCH - 1) {
+        T nPF = __shfl_down(PF, 1, NL), nQF = __shfl_down(QF, 1, NL), nRF = __shfl_down(RF, 1, NL);
+        if (last_real || lane == NL - 1) {
             nPF = (T)0;
             nQF = (T)0;
             nRF = (T)0;
         }
         Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
 #pragma unroll
-        for (int d = 1; d < NCH; d <<= 1) {
+        for (int d = 1; d < NL; d <<= 1) {
             const NEq<T> m = normalized(e);
             NEq<T> l, h;
-            l.a = __shfl_up(m.a, d);
-            l.c = __shfl_up(m.c, d);
-            l.r = __shfl_up(m.r, d);
-            h.a = __shfl_down(m.a, d);
-            h.c = __shfl_down(m.c, d);
-            h.r = __shfl_down(m.r, d);
+            l.a = __shfl_up(m.a, d, NL);
+            l.c = __shfl_up(m.c, d, NL);
+            l.r = __shfl_up(m.r, d, NL);
+            h.a = __shfl_down(m.a, d, NL);
+            h.c = __shfl_down(m.c, d, NL);
+            h.r = __shfl_down(m.r, d, NL);
             if (lane < d)
                 l = zero_neq<T>();
-            if (lane + d >= NCH)
+            if (lane + d >= NL)
                 h = zero_neq<T>();
             e = pcr_step(e, l, h);
         }
         const T xi = e.r * fast_rcp(e.b);
-        T xl = __shfl_up(xi, 1);
+        T xl = __shfl_up(xi, 1, NL);
         if (lane == 0)
             xl = (T)0;
         if constexpr (PUT_ALL) {

@@ -29,20 +29,25 @@ namespace lssamd {
 // v-line sweep.  stage 1: rhs_v(u, Y1) -> Y2 (DR: next); stage 2: rhs_v(u, Y3) (CS/MCS) or
 // rhs_v(Y2, Y4) (HV) -> next.  Ends colA0/colA1[idx1[i]] (implicit_kernel.hpp:69-84).
 // -------------------------------------------------------------------------------------------------
-template <typename T, int M, int VR, int PAT>
-__global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict__ probs,
-                                                      const Tile *__restrict__ tiles, int step, int stage, int swz)
+// LPW v-lines per wavefront: 1 (the row is wavefront-uniform, in scalar registers), or 2 for lines of
+// at most 32 M interior points (lanes 0-31 and 32-63 each hold one line: half the reduction steps per
+// line).  A half without a line of its own repeats its neighbour's line and stores nothing, so both
+// halves take the same chunk length.
+template <typename T, int M, int VR, int PAT, int LPW>
+__device__ __forceinline__ void vsweep_lines(const Prob<T> &p, const Tile &tl, int step, int stage)
 {
-    const Tile tl = tiles[swz ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
-    const Prob<T> &p = probs[tl.prob];
-    if (step >= p.steps || (stage == 2 && p.scheme == SCHEME_DR))
-        return;  // uniform over the workgroup
     const GArr<T> cs{coeff_set(p, step)};
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (wave >= tl.count)
+    if (wave * LPW >= tl.count)
         return;
-    // the row is the same for the whole wavefront: scalar registers, scalar loads of its X factors
-    const int i = __builtin_amdgcn_readfirstlane(tl.first + wave);
+    constexpr int NL = 64 / LPW;
+    const int half = LPW == 2 ? lane >> 5 : 0, ll = lane & (NL - 1);
+    const bool own = wave * LPW + half < tl.count;  // this half has a line of its own
+    int i;
+    if constexpr (LPW == 1)
+        i = __builtin_amdgcn_readfirstlane(tl.first + wave);
+    else
+        i = tl.first + min(wave * LPW + half, tl.count - 1);
     const int ny = p.ny, N = ny - 2;
     const T *u = (step & 1) ? p.b : p.a;
     T *nxt = (step & 1) ? p.a : p.b;
@@ -86,12 +91,29 @@ __global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict_
         }
         return q;
     };
-    auto put = [&](int, int jj, T x) { stg(out, row + jj, x); };
-    wave_solve<T, M>(N, lane, eq, put);
-    if (lane == 0) {
+    auto put = [&](int, int jj, T x) {
+        if (own)
+            stg(out, row + jj, x);
+    };
+    wave_solve<T, M, false, NL>(N, ll, eq, put);
+    if (ll == 0 && own) {
         stg(out, row, lo);
         stg(out, row + ny - 1, hi);
     }
+}
+
+template <typename T, int M, int VR, int PAT>
+__global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict__ probs,
+                                                      const Tile *__restrict__ tiles, int step, int stage, int swz)
+{
+    const Tile tl = tiles[swz ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    if (step >= p.steps || (stage == 2 && p.scheme == SCHEME_DR))
+        return;  // uniform over the workgroup
+    if (vlines_per_wave(p.ny - 2, M) == 2)  // (the plan's tiles hold 2 VR lines for these problems)
+        vsweep_lines<T, M, VR, PAT, 2>(p, tl, step, stage);
+    else
+        vsweep_lines<T, M, VR, PAT, 1>(p, tl, step, stage);
 }
 
 // -------------------------------------------------------------------------------------------------
