@@ -2,8 +2,9 @@
 // solves (lss_adi_chunk.hpp) whose intermediates stay in registers and LDS.  Each input is read
 // once and each output written once.
 //
-//   k_vsweep_p<VR>: VR v-lines per workgroup, one wavefront per v-line (contiguous in HBM);
-//                   lane = chunk; the interface system is reduced across lanes with shuffles.
+//   k_vsweep_p<VR>: VR wavefronts per workgroup, one v-line (contiguous in HBM) per wavefront, or two
+//                   for short lines; lane = chunk; the interface system is reduced across lanes with
+//                   shuffles.
 //   k_xsweep_p<W>:  a workgroup covers W adjacent S-lines (W = 16: one 128-byte row segment per
 //                   access) and 64 row chunks per line; the interface system is reduced through LDS.
 //                   The HV stage-2 sweep evaluates the explicit Y3 stage pointwise inside its
