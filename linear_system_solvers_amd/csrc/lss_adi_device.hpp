@@ -117,9 +117,10 @@ constexpr int GTR = 8, GTC = 32;  // gather tiles of the adjoint: rows x columns
 constexpr int ERPT = 8;           // rows per thread of the explicit step (its tiles: GTR * ERPT rows)
 
 // v-lines per wavefront of the partitioned v-sweep for lines of N interior points and the plan's
-// chunk bound M: 2 when the line fits 32 chunks of at most M points, else 1 (the plan's v-line tiles
-// and the kernel follow the same rule)
-__host__ __device__ constexpr int vlines_per_wave(int N, int M) { return N <= 32 * M ? 2 : 1; }
+// chunk bound M: 2 when the line fits 32 chunks of at most M <= 4 points, else 1 (the plan's v-line
+// tiles and the kernel follow the same rule; larger chunk bounds keep one line per wavefront, whose
+// register budget the second body would strain)
+__host__ __device__ constexpr int vlines_per_wave(int N, int M) { return M <= 4 && N <= 32 * M ? 2 : 1; }
 constexpr int NTERM = 12;  // derivative terms per problem
 
 // Adjoint state of one problem.  The forward values a reverse step needs (U_s, U_{s+1} and the

@@ -103,18 +103,23 @@ __device__ __forceinline__ void vsweep_lines(const Prob<T> &p, const Tile &tl, i
     }
 }
 
+// (eight wavefronts per SIMD: the two-lines-per-wavefront body must not raise the register count of
+// the kernel above 64, which costs a wavefront per SIMD on every line)
 template <typename T, int M, int VR, int PAT>
-__global__ void __launch_bounds__(64 * VR) k_vsweep_p(const Prob<T> *__restrict__ probs,
-                                                      const Tile *__restrict__ tiles, int step, int stage, int swz)
+__global__ void __launch_bounds__(64 * VR) __attribute__((amdgpu_waves_per_eu(M <= 4 ? 8 : 1)))
+k_vsweep_p(const Prob<T> *__restrict__ probs, const Tile *__restrict__ tiles, int step, int stage, int swz)
 {
     const Tile tl = tiles[swz ? xcd_tile(blockIdx.x, gridDim.x) : blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
     if (step >= p.steps || (stage == 2 && p.scheme == SCHEME_DR))
         return;  // uniform over the workgroup
-    if (vlines_per_wave(p.ny - 2, M) == 2)  // (the plan's tiles hold 2 VR lines for these problems)
-        vsweep_lines<T, M, VR, PAT, 2>(p, tl, step, stage);
-    else
-        vsweep_lines<T, M, VR, PAT, 1>(p, tl, step, stage);
+    if constexpr (M <= 4) {
+        if (vlines_per_wave(p.ny - 2, M) == 2) {  // (the plan's tiles hold 2 VR lines for these problems)
+            vsweep_lines<T, M, VR, PAT, 2>(p, tl, step, stage);
+            return;
+        }
+    }
+    vsweep_lines<T, M, VR, PAT, 1>(p, tl, step, stage);
 }
 
 // -------------------------------------------------------------------------------------------------
