@@ -202,7 +202,7 @@ def parity_sample(models, outs, log, count=8):
 CONDITIONING_FACTOR = 100.0
 
 
-def check_all(models, outs, log, exact_outs=None, threads: int = 16, tol: float = 1e-10):
+def check_all(models, outs, log, exact_outs=None, threads: int = 0, tol: float = 1e-10):
     """Every book problem against the oracle (cached coefficients; same arithmetic as the faithful
     mode), on a pool of host threads (ctypes releases the GIL during each solve).
 
@@ -225,7 +225,7 @@ def check_all(models, outs, log, exact_outs=None, threads: int = 16, tol: float 
     import dataclasses
     from oracle import oracle as O
     from tests.helpers import oracle_solve
-    threads = max(1, min(threads, os.cpu_count() or 1))
+    threads = threads or host_threads()
     t0 = time.time()
 
     def normwise(g, c):
@@ -316,23 +316,71 @@ def exact_book(torch, log):
     return res
 
 
-def cpu_baseline(models, log, budget_s: float = 15.0):
-    """The oracle (faithful mode: every coefficient evaluated through the transform chain at every use,
-    as the reference does), single thread, on the smallest problems of the book until the budget."""
+def host_threads() -> int:
+    """Worker threads for host-side work: the CPU share of one GPU on the box (16), or fewer cores."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(models, log, budget_s: float = 10.0, threads: int = 0):
+    """The oracle in faithful mode (every coefficient evaluated through the transform chain at every use,
+    as the reference does) on the smallest problems of the book until the budget: on one thread (the
+    reference is single-threaded) and with the problems distributed over `threads` host threads."""
+    import concurrent.futures as cf
+    import threading
     from oracle import oracle as O
     from tests.helpers import oracle_cfg, oracle_pde
     order = sorted(range(len(models)), key=lambda q: models[q].nx * models[q].ny * models[q].steps)
-    done, ps, t0 = 0, 0, time.time()
-    for q in order:
+
+    def solve(q):
         m = models[q]
         O.solve(oracle_pde(O, m), oracle_cfg(O, m, cached=False), "f64")
+        return m.nx * m.ny * m.steps
+
+    done, ps, t0 = 0, 0, time.time()
+    for q in order:
+        ps += solve(q)
         done += 1
-        ps += m.nx * m.ny * m.steps
         if time.time() - t0 > budget_s:
             break
-    secs = time.time() - t0
-    log(f"cpu baseline: {done} problems, {ps:.3e} point-steps in {secs:.1f}s (1 core)")
-    return ps / secs, done, ps
+    secs1 = time.time() - t0
+    log(f"cpu baseline: {done} problems, {ps:.3e} point-steps in {secs1:.1f}s (1 thread)")
+    threads = threads or host_threads()
+    lock, it = threading.Lock(), iter(order)
+    tot = {"ps": 0, "n": 0}
+    t0 = time.time()
+
+    def worker():
+        while time.time() - t0 < budget_s:
+            with lock:
+                q = next(it, None)
+            if q is None:
+                return
+            v = solve(q)
+            with lock:
+                tot["ps"] += v
+                tot["n"] += 1
+
+    with cf.ThreadPoolExecutor(threads) as ex:
+        for f in [ex.submit(worker) for _ in range(threads)]:
+            f.result()
+    secsn = time.time() - t0
+    log(f"cpu baseline: {tot['n']} problems, {tot['ps']:.3e} point-steps in {secsn:.1f}s ({threads} threads)")
+    return {"one": (ps / secs1, done, ps), "all": (tot["ps"] / secsn, tot["n"], tot["ps"]), "threads": threads}
 
 
 def adjoint_model(nx=1024, ny=512, steps=1000, **over):
@@ -569,8 +617,9 @@ def main():
                          "solved --steps times with no events and no timing (hipEvents are not used under --pmc)")
     ap.add_argument("--only", choices=("adjoint", "sabr", "tridiag", "explicit"), default="",
                     help="run only this secondary workload (no book, no JSON bench line)")
-    ap.add_argument("--check-all", action="store_true",
-                    help="check every book problem against the oracle on host threads (about 12 min on 16 cores)")
+    ap.add_argument("--no-check-all", action="store_true",
+                    help="skip checking every book problem (fast and exact GPU paths) against the oracle on host "
+                         "threads (about 4 min on 16 threads, outside the timed region)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run reference kernels of known HBM traffic (a 256 MiB fill, a 256 MiB elementwise "
                          "read and write) for scaling the HBM counters")
@@ -639,9 +688,16 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "heston_book_2048_seed3", "problems": len(models),
+        "config": {"workload": "heston_book_2048_seed3", "problems": len(models), "seed": 3,
                    "grid_point_steps_per_solve": total_ps, "coeff": "separable", "line_solver": "partitioned",
-                   "graph": not args.no_graph, "kernel_config": plan.kernel_config()},
+                   "graph": not args.no_graph, "kernel_config": plan.kernel_config(),
+                   "assumed": {"grid": "grid_enum::Uniform (sinh map, accumulation point 0; configs[0]'s grid)",
+                               "S_range": "[0, 2K]", "v_range": "[0, 1.2]", "theta": 0.5,
+                               "implicit_scheme": "CrankNicolson", "zeta_weighting": 0.5,
+                               "hints": "grid_config_hints_2d defaults (point 0, alpha_scale 3, beta_scale 50)",
+                               "boundaries": "Dirichlet 0 at S=0, Neumann -1 (dU/dS=1) at S_max, "
+                                             "Dirichlet U=S at v_max",
+                               "direction": "Backward", "payoff": "max(S-K, 0)"}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                      "kernel": ("k_" + dom.rstrip("12") + ("_p" if dom.startswith(("x", "v")) else "")),
@@ -649,9 +705,22 @@ def main():
                      "algo_bytes_per_launch": round(d_bytes / d_n),
                      "whole_step_algo_GBs": round(total_algo / (total_ms / 1e3) / 1e9, 1)},
     }
-    if args.check_all:
-        result["check_all"] = check_all(models, [o.cpu().numpy() for o in outs], log,
-                                        exact_outs=exact_book(torch, log))
+    failed = 0
+    if not args.no_check_all:
+        ca = check_all(models, [o.cpu().numpy() for o in outs], log, exact_outs=exact_book(torch, log))
+        result["check_all"] = ca
+        S, U = ca["stable"], ca["unstable"]
+        within = S["within_tolerance"] + U["within_tolerance"]
+        cond = S["within_conditioning"] + U["within_conditioning"]
+        failed = ca["failed"]
+        result["parity_qualifier"] = (
+            f"{within}/{ca['problems']} book problems within 1e-10 normwise of the CPU oracle (stable: "
+            f"{S['within_tolerance']}/{S['problems']}, max per-value {S['max_rel_err_per_value']:.1e}); {cond} more "
+            f"(stable {S['within_conditioning']}, unstable in the reference scheme {U['within_conditioning']}) "
+            f"within {CONDITIONING_FACTOR:.0f}x their own sensitivity to rounding (max {max(S['max_err_over_spread'], U['max_err_over_spread']):.1f}x); "
+            f"{ca['overflow_in_oracle_and_gpu']} overflow in both; {failed} failed; exact GPU path bit-identical "
+            f"on {ca.get('exact_path_bit_identical', 0)}/{ca['problems']}")
+        log("parity: " + result["parity_qualifier"])
     if not args.no_extras:
         from tests.helpers import rel_err as _re  # noqa: F401
         outs_np = [o.cpu().numpy() for o in outs]
@@ -669,16 +738,28 @@ def main():
         plan.close()
         plan = None
         result["adjoint"] = bench_adjoint(torch, log)
+        # the metric's second part: a full gradient (checkpointed forward + backward) over one forward solve
+        result["adjoint_over_forward"] = result["adjoint"]["ratio_gradient_over_forward"]
         result["sabr_book"] = bench_sabr(torch, log)
         result["explicit_euler_book"] = bench_explicit(torch, log)
     if not args.no_cpu:
-        rate, nprob, ps = cpu_baseline(models, log)
-        result["cpu_baseline"] = {"value": rate, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
-                                  "sample": f"{nprob} smallest book problems ({ps:.3e} point-steps), oracle faithful "
-                                            f"mode (closure-per-coefficient evaluation like the reference), 1 thread"}
+        cb = cpu_baseline(models, log)
+        r1, n1, p1 = cb["one"]
+        rn, nn, pn = cb["all"]
+        result["cpu_baseline"] = {
+            "value": rn, "unit": "grid-point-steps/s", "cores": cb["threads"], "kind": "port",
+            "sample": f"the smallest book problems, oracle faithful mode (every coefficient through the closure and "
+                      f"transform chain at every use, like the reference), about 10 s each way: {nn} problems "
+                      f"({pn:.3e} point-steps) over {cb['threads']} threads (problems distributed, one per thread "
+                      f"at a time); {n1} problems ({p1:.3e} point-steps) on 1 thread",
+            "single_thread_value": r1, "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "threads_used": cb["threads"]}
     print(json.dumps(result), flush=True)
     if plan is not None:
         plan.close()
+    if failed:
+        log(f"{failed} book problems outside both the tolerance and their sensitivity bound")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -45,6 +45,9 @@ template <typename T> struct grid_spec {
     int direction;        // LSS_FORWARD / LSS_BACKWARD
 };
 
+// How closures are sampled over the loop times: TIME_AUTO samples them at every loop time and keeps
+// one set when every set equals the first bit for bit (else all of them); TIME_INDEPENDENT samples
+// once at the first loop time; TIME_DEPENDENT keeps one set per step.
 enum time_mode { TIME_AUTO = 0, TIME_INDEPENDENT = 1, TIME_DEPENDENT = 2 };
 
 // grid_transform_config_2d (lss_grid_transform_config.hpp:118-149)
@@ -256,30 +259,45 @@ void sample_fields(const user_data<T> &u, const grid_spec<T> &g, sampled<T> &s, 
             s.init[i * ny + j] = u.init(s.xs[i], s.ys[j]);
 
     const std::size_t cstride = 7 * nx * ny + 3 * nx, bstride = nx + 2 * ny, n = s.steps;
-    auto decide = [&](time_mode mode, std::size_t stride, auto fill) -> std::size_t {
-        if (mode == TIME_INDEPENDENT || n == 1)
+    // TIME_AUTO samples every loop time and compares each set with the first: one set is kept only
+    // when all of them agree bit for bit (a closure that changes at any loop time is sampled per step)
+    auto sample = [&](time_mode mode, std::size_t stride, auto fill, std::vector<T> &out) -> std::size_t {
+        if (mode == TIME_INDEPENDENT || n == 1) {
+            out.assign(stride, T(0));
+            fill(s.times[0], out.data());
             return 1;
-        if (mode == TIME_DEPENDENT)
-            return n;
-        std::vector<T> a(stride), b(stride);
-        fill(s.times[0], a.data());
-        for (std::size_t probe : {n / 2, n - 1}) {
-            fill(s.times[probe], b.data());
-            if (std::memcmp(a.data(), b.data(), stride * sizeof(T)) != 0)
-                return n;
         }
-        return 1;
+        std::size_t q = 1;
+        if (mode == TIME_AUTO) {
+            std::vector<T> first(stride), cur(stride);
+            fill(s.times[0], first.data());
+            for (; q < n; ++q) {
+                fill(s.times[q], cur.data());
+                if (std::memcmp(first.data(), cur.data(), stride * sizeof(T)) != 0)
+                    break;
+            }
+            if (q == n) {
+                out = std::move(first);
+                return 1;
+            }
+            out.assign(n * stride, T(0));
+            std::memcpy(out.data(), first.data(), stride * sizeof(T));
+            std::memcpy(out.data() + q * stride, cur.data(), stride * sizeof(T));
+            for (std::size_t r = 1; r < q; ++r)  // equal to the first set
+                std::memcpy(out.data() + r * stride, first.data(), stride * sizeof(T));
+            ++q;
+        } else {
+            out.assign(n * stride, T(0));
+            fill(s.times[0], out.data());
+        }
+        for (; q < n; ++q)
+            fill(s.times[q], out.data() + q * stride);
+        return n;
     };
     auto fill_c = [&](T t, T *o) { sample_field_set(u, g, s, t, o); };
     auto fill_b = [&](T t, T *o) { sample_bnd_set(u, g, s, t, o); };
-    s.coeff_sets = decide(coeff_time, cstride, fill_c);
-    s.coeff.assign(s.coeff_sets * cstride, T(0));
-    for (std::size_t q = 0; q < s.coeff_sets; ++q)
-        fill_c(s.times[q], s.coeff.data() + q * cstride);
-    s.bnd_sets = decide(bnd_time, bstride, fill_b);
-    s.bnd.assign(s.bnd_sets * bstride, T(0));
-    for (std::size_t q = 0; q < s.bnd_sets; ++q)
-        fill_b(s.times[q], s.bnd.data() + q * bstride);
+    s.coeff_sets = sample(coeff_time, cstride, fill_c, s.coeff);
+    s.bnd_sets = sample(bnd_time, bstride, fill_b, s.bnd);
 }
 
 }  // namespace lss_amd_detail

@@ -184,7 +184,9 @@ int lss_plan_problem_shape(const lss_adi_plan *plan, int64_t p, int64_t *nx, int
  * and a[0] = c[n-1] = 0, i.e. the plain algorithm).  B systems of length n; layout CONTIGUOUS
  * (x[k*n+i]) or STRIDED (x[i*B+k]); device pointers; enqueued on `stream`.  Systems of up to 1024
  * unknowns (contiguous fp64: 512) are solved by the register-resident partitioned elimination with
- * no scratch; longer ones one lane per system, with a grow-only device workspace kept per device.
+ * no scratch; longer ones one lane per system, with B*n values of scratch allocated on `stream`
+ * (stream-ordered, hipMallocAsync) and freed on it after the solve, so that calls on different streams
+ * never share scratch.
  */
 int lss_tridiag_batched(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b,
                         const void *c, const void *d, void *x, void *stream);
@@ -208,6 +210,7 @@ typedef struct {
 int lss_tridiag_batched_bc(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b,
                            const void *c, const void *d, const lss_line_bc *lower, const lss_line_bc *upper, void *x,
                            void *stream);
+
 
 /*
  * Same time loop as lss_adi_run_dev (device pointers, no graph) with a hipEvent pair recorded on
@@ -241,7 +244,9 @@ int lss_adi_run_profiled(lss_adi_plan *plan, const void *const *u_in, void *cons
  *   d q(i, j) / d param_k = X_t[i] * Y_t[j]   for q one of a..f (LSS_Q_*),
  * i.e. the derivative of heat_data_transform's coefficient at grid point (i, j)
  * (lss_heat_data_transform.hpp:174-214).  Model plans carry their terms (Heston: kappa, theta_v, sigma,
- * rho, r; SABR: nu, rho, beta); other plans set them with lss_adi_set_param_terms: term t has
+ * rho, r; SABR: nu, rho, beta and lam, the scale of the alpha axis about y_lo: the grid, and with it a
+ * price read at a fixed eta, stretched by lam, so that dPrice/dalpha0 = dJ/dlam / alpha0 when y_lo = 0
+ * and y_hi is a multiple of alpha0); other plans set them with lss_adi_set_param_terms: term t has
  * parameter term_param[t] and coefficient term_coeff[t]; factors holds `sets` (1 or steps) blocks of
  * nterm * (nx X-values then ny Y-values) in the plan's precision.
  */

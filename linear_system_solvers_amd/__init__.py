@@ -42,7 +42,9 @@ EXPORTS = ("lss_last_error", "lss_version", "lss_adi_plan_create", "lss_adi_plan
 MAX_PARAMS = 6
 MAX_TERMS = 12
 HESTON_PARAMS = ("kappa", "theta_v", "sigma", "rho", "r")
-SABR_PARAMS = ("nu", "rho", "beta")
+# SABR: nu, rho, beta and "alpha_axis", the scale lam of the alpha axis about y_lo (grid and price point
+# stretched together): dPrice/dalpha0 = d/dlam / alpha0 when y_lo = 0 and y_hi is a multiple of alpha0
+SABR_PARAMS = ("nu", "rho", "beta", "alpha_axis")
 KERNEL_CONFIG_LEN = 12
 KERNEL_KINDS = ("bnd_a", "xsweep1", "vsweep1", "hv_y3", "xsweep2", "vsweep2", "bnd_b", "euler")
 
@@ -236,6 +238,12 @@ class Plan:
         dt = np_dtype(self.fp)
         if inits is None:
             inits = [self.initial(p) for p in range(self.count)]
+        if len(inits) != self.count:
+            raise LssError(-1, f"run_host needs {self.count} surfaces, got {len(inits)}")
+        for p, a in enumerate(inits):
+            nx, ny, _ = self.shape(p)
+            if np.shape(a) != (nx, ny):
+                raise LssError(-1, f"surface {p} has shape {np.shape(a)}, the plan's problem is {(nx, ny)}")
         ins = [np.ascontiguousarray(a, dtype=dt) for a in inits]
         outs = [np.zeros_like(a) for a in ins]
         pin = (C.c_void_p * self.count)(*[a.ctypes.data for a in ins])
@@ -354,3 +362,4 @@ def tridiag_batched_bc(fp: int, layout: int, B: int, n: int, a: int, b: int, c: 
     _check(lib().lss_tridiag_batched_bc(fp, layout, B, n, C.c_void_p(a), C.c_void_p(b), C.c_void_p(c),
                                         C.c_void_p(d), C.byref(lo), C.byref(hi), C.c_void_p(x),
                                         C.c_void_p(stream)))
+

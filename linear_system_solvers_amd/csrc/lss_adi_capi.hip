@@ -1682,11 +1682,14 @@ void separable_set(const lss_model &m, const lss_amd_detail::grid_spec<T> &g,
 //   Heston (kappa, theta_v, sigma, rho, r):
 //     e' = (theta_v - v) / Y', kappa / Y';  b' = sigma v / Y'^2, c' = rho v S / (X' Y'),
 //     e' = -sigma v Y'' / Y'^3;  c' = sigma v S / (X' Y');  d' = S / X', f' = -1
-//   SABR (nu, rho, beta), D = exp(-r (T - t)):
+//   SABR (nu, rho, beta, lam), D = exp(-r (T - t)):
 //     b' = nu al^2 / Y'^2, c' = rho al^2 cX / (X' Y'), e' = -nu al^2 Y'' / Y'^3;  c' = nu al^2 cX / (X' Y');
 //     a' = aX_b al^2 / X'^2, c' = rho nu al^2 cX_b / (X' Y'), d' = -aX_b al^2 X'' / X'^3
 //     with aX = F^(2 beta) D^(2 (1 - beta)) / 2, cX = F^beta D^(1 - beta), aX_b = 2 aX ln(F / D),
-//     cX_b = cX ln(F / D) (0 at F = 0).
+//     cX_b = cX ln(F / D) (0 at F = 0);
+//     lam scales the alpha axis about y_lo (Yh = al - y_lo): a' = aX 2 al Yh / X'^2,
+//     b' = nu^2 al (Yh - al) / Y'^2, c' = rho nu cX al (2 Yh - al) / (X' Y'), d' = -aX X'' 2 al Yh / X'^3,
+//     e' = -nu^2 al Y'' (Yh - al) / Y'^3.
 template <typename T>
 void param_terms(const lss_model &m, const lss_amd_detail::grid_spec<T> &g, const lss_amd_detail::sampled<T> &s,
                  T t, lss_adi_plan::Terms &tm, T *out)
@@ -1746,7 +1749,20 @@ void param_terms(const lss_model &m, const lss_amd_detail::grid_spec<T> &g, cons
         term(2, 0, [&](size_t i) { return 2.0 * aX(i) * lnFD(i) / (dX[i] * dX[i]); }, a2);
         term(2, 2, [&](size_t i) { return cX(i) * lnFD(i) / dX[i]; }, [&](size_t j) { return rho * sig * a2(j) / dY[j]; });
         term(2, 3, [&](size_t i) { return -2.0 * aX(i) * lnFD(i) * ddX[i] / (dX[i] * dX[i] * dX[i]); }, a2);
-        tm.nparam = 3;
+        // alpha axis scaled by lam about y_lo (grid, and with it the price point at a fixed eta):
+        // d Y / d lam = Y - y_lo = Yh, d Y' / d lam = Y', d Y'' / d lam = Y''; with y_lo = 0 only the a, c
+        // and d terms are non-zero (b and e are invariant), and dPrice / d alpha0 = dPrice / d lam / alpha0
+        // when y_hi is a multiple of alpha0
+        const double ylo = (double)g.y_lo;
+        auto yh = [&](size_t j) { return V[j] - ylo; };
+        term(3, 0, [&](size_t i) { return aX(i) / (dX[i] * dX[i]); }, [&](size_t j) { return 2.0 * V[j] * yh(j); });
+        term(3, 1, one, [&](size_t j) { return sig * sig * V[j] * (yh(j) - V[j]) / (dY[j] * dY[j]); });
+        term(3, 2, [&](size_t i) { return cX(i) / dX[i]; },
+             [&](size_t j) { return rho * sig * (V[j] * (2.0 * yh(j) - V[j])) / dY[j]; });
+        term(3, 3, [&](size_t i) { return -aX(i) * ddX[i] / (dX[i] * dX[i] * dX[i]); },
+             [&](size_t j) { return 2.0 * V[j] * yh(j); });
+        term(3, 4, one, [&](size_t j) { return -sig * sig * V[j] * ddY[j] * (yh(j) - V[j]) / (dY[j] * dY[j] * dY[j]); });
+        tm.nparam = 4;
     }
     tm.nterm = nt;
 }
@@ -1858,33 +1874,6 @@ void tridiag_part(int M, int layout, int64_t B, int n, const void *a, const void
     }
 }
 
-// grow-only per-device workspace of lss_tridiag_batched
-std::mutex g_ws_mutex;
-std::vector<std::pair<void *, size_t>> g_ws;
-
-void *workspace(size_t bytes)
-{
-    int dev = 0;
-    HIPCHK(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(g_ws_mutex);
-    if ((int)g_ws.size() <= dev)
-        g_ws.resize(dev + 1, {nullptr, 0});
-    auto &w = g_ws[dev];
-    if (w.second < bytes) {
-        if (w.first) {
-            HIPCHK(hipDeviceSynchronize());
-            HIPCHK(hipFree(w.first));
-            w.first = nullptr;
-            w.second = 0;
-        }
-        hipError_t e = hipMalloc(&w.first, bytes);
-        if (e != hipSuccess)
-            throw lss_error(LSS_ERR_ALLOC, std::string("workspace hipMalloc failed: ") + hipGetErrorString(e));
-        w.second = bytes;
-    }
-    return w.first;
-}
-
 template <typename T> LineBC<T> line_bc(const lss_line_bc *lo, const lss_line_bc *hi, int64_t n)
 {
     LineBC<T> bc;
@@ -1899,7 +1888,9 @@ template <typename T> LineBC<T> line_bc(const lss_line_bc *lo, const lss_line_bc
     return bc;
 }
 
-// lss_tridiag_batched(_bc): lower/upper null = plain rows
+// lss_tridiag_batched(_bc): lower/upper null = plain rows.  The one-lane-per-system kernel needs
+// B * n values of scratch: a stream-ordered allocation that is freed on `st` after the kernel, so that
+// calls on different streams never share scratch memory.
 void tridiag_run(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b, const void *c,
                  const void *d, const lss_line_bc *lo, const lss_line_bc *hi, void *x, hipStream_t st)
 {
@@ -1928,7 +1919,11 @@ void tridiag_run(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a
         else
             tridiag_part<float>(M, layout, B, (int)n, a, b, c, d, x, lds_rows, line_bc<float>(lo, hi, n), st);
     } else {
-        void *scr = workspace((size_t)B * (size_t)n * es);
+        const size_t bytes = (size_t)B * (size_t)n * es;
+        void *scr = nullptr;
+        hipError_t e = hipMallocAsync(&scr, bytes, st);
+        if (e != hipSuccess)
+            throw lss_error(LSS_ERR_ALLOC, std::string("scratch hipMallocAsync failed: ") + hipGetErrorString(e));
         const unsigned blocks = (unsigned)((B + 255) / 256);
         if (fp == LSS_FP64)
             hipLaunchKernelGGL(k_tridiag_lane<double>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
@@ -1938,6 +1933,10 @@ void tridiag_run(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a
             hipLaunchKernelGGL(k_tridiag_lane<float>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
                                (const float *)a, (const float *)b, (const float *)c, (const float *)d,
                                (float *)x, (float *)scr, line_bc<float>(lo, hi, n));
+        const hipError_t le = hipGetLastError();
+        HIPCHK(hipFreeAsync(scr, st));
+        HIPCHK(le);
+        return;
     }
     HIPCHK(hipGetLastError());
 }

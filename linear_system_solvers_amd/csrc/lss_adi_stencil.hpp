@@ -28,7 +28,7 @@ template <> struct FPat<PAT_SABR0> {
 
 // Derivative-term tables of the reference models' parameters in the packed form the plan stores
 // (Adj::tp, tq, fx, fy; fx/fy: -2 one, k >= 0 packed factor vector k): Heston (kappa, theta_v,
-// sigma, rho, r) and SABR (nu, rho, beta), in the order the plan generates them.  The adjoint
+// sigma, rho, r) and SABR (nu, rho, beta, scale of the alpha axis), in the order the plan generates them.  The adjoint
 // kernels use them at compile time when every problem of the plan has exactly these tables and the
 // same factor pattern.
 template <int PAT> struct TPat;
@@ -39,12 +39,12 @@ template <> struct TPat<PAT_HESTON> {
     static constexpr int fx[8] = {-2, -2, -2, 0, -2, 0, 0, -2};
     static constexpr int fy[8] = {0, 1, 2, 3, 4, 5, -2, 6};
 };
-template <> struct TPat<PAT_SABR0> {
-    static constexpr int n = 7;
-    static constexpr int tp[7] = {0, 0, 0, 1, 2, 2, 2};
-    static constexpr int tq[7] = {1, 2, 4, 2, 0, 2, 3};
-    static constexpr int fx[7] = {-2, 0, -2, 0, 1, 2, 3};
-    static constexpr int fy[7] = {0, 1, 2, 3, 4, 5, 4};
+template <> struct TPat<PAT_SABR0> {  // (with y_lo = 0 the alpha-axis terms of b and e vanish)
+    static constexpr int n = 10;
+    static constexpr int tp[10] = {0, 0, 0, 1, 2, 2, 2, 3, 3, 3};
+    static constexpr int tq[10] = {1, 2, 4, 2, 0, 2, 3, 0, 2, 3};
+    static constexpr int fx[10] = {-2, 0, -2, 0, 1, 2, 3, 4, 0, 5};
+    static constexpr int fy[10] = {0, 1, 2, 3, 4, 5, 4, 6, 5, 6};
 };
 
 // factor slot SLOT of a packed SEPARABLE block B (length-n vectors) at index i

@@ -9,6 +9,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* grids of at least this many points run the lines of each sweep on OpenMP threads */
+static int64_t lss_or_parallel_min = 262144;
+void lss_or_set_parallel_min(int64_t points) { lss_or_parallel_min = points; }
+
 /* ---- double instantiation ---- */
 #define FP double
 #define SFX _f64
@@ -16,6 +20,7 @@
 #define COSH cosh
 #define ASINH asinh
 #include "lss_oracle_impl.inc"
+#include "lss_oracle_heat1d.inc"
 #undef FP
 #undef SFX
 #undef SINH
@@ -29,6 +34,7 @@
 #define COSH coshf
 #define ASINH asinhf
 #include "lss_oracle_impl.inc"
+#include "lss_oracle_heat1d.inc"
 #undef FP
 #undef SFX
 #undef SINH
@@ -162,5 +168,56 @@ void lss_or_sabr_pde(const lss_or_sabr *p, lss_or_pde *out)
     out->h_lower = sabr_low;
     out->h_upper = sabr_high;
     out->v_upper = bnd_identity;
+    out->ctx = (void *)p;
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Black-Scholes data of unit_tests/pde_solvers/one_dimensional/lss_black_scholes_equation_t.hpp:66-80
+ * (and unit_tests/common/lss_print_t.hpp:151-166): the captured constants are double, so each
+ * expression is evaluated in double.
+ * ---------------------------------------------------------------------------------------------- */
+static double bs_a(void *c, double t, double x)
+{
+    const lss_or_bs *p = (const lss_or_bs *)c;
+    (void)t;
+    return 0.5 * p->sig * p->sig * x * x;
+}
+static double bs_b(void *c, double t, double x)
+{
+    const lss_or_bs *p = (const lss_or_bs *)c;
+    (void)t;
+    return p->rate * x;
+}
+static double bs_c(void *c, double t, double x)
+{
+    const lss_or_bs *p = (const lss_or_bs *)c;
+    (void)t; (void)x;
+    return -p->rate;
+}
+static double bs_init(void *c, double x)
+{
+    const lss_or_bs *p = (const lss_or_bs *)c;
+    const double v = x - p->strike;
+    return v > 0.0 ? v : 0.0;
+}
+static double bs_lo(void *c, double t) { (void)c; (void)t; return 0.0; }
+static double bs_hi(void *c, double t)
+{
+    const lss_or_bs *p = (const lss_or_bs *)c;
+    return (p->x_hi - p->strike * exp(-p->rate * (p->maturity - t)));
+}
+
+void lss_or_bs_heat1d(const lss_or_bs *p, lss_or_heat1d_pde *out)
+{
+    memset(out, 0, sizeof(*out));
+    out->a = bs_a;
+    out->b = bs_b;
+    out->c = bs_c;
+    out->src = NULL;
+    out->init = bs_init;
+    out->lo_type = LSS_OR_BC_DIRICHLET;
+    out->hi_type = LSS_OR_BC_DIRICHLET;
+    out->lo_val = bs_lo;
+    out->hi_val = bs_hi;
     out->ctx = (void *)p;
 }

@@ -101,6 +101,10 @@ typedef struct lss_or_sabr {
 void lss_or_heston_pde(const lss_or_heston *p, lss_or_pde *out);
 void lss_or_sabr_pde(const lss_or_sabr *p, lss_or_pde *out);
 
+/* Grids of at least `points` points (default 262144) run the lines of each sweep on OpenMP threads
+ * (OMP_NUM_THREADS); results do not depend on the thread count. */
+void lss_or_set_parallel_min(int64_t points);
+
 /* Full solve: out is nx*ny row-major (row = x/S index, column = y/v index).  Returns 0 or <0. */
 int lss_or_solve_f64(const lss_or_pde *pde, const lss_or_cfg *cfg, double *out);
 int lss_or_solve_f32(const lss_or_pde *pde, const lss_or_cfg *cfg, float *out);
@@ -139,6 +143,49 @@ int lss_or_thomas_batched_f64(int64_t B, int64_t n, int32_t strided, const doubl
                               const double *c, const double *d, double *x);
 int lss_or_thomas_batched_f32(int64_t B, int64_t n, int32_t strided, const float *a, const float *b,
                               const float *c, const float *d, float *x);
+
+/* ---------------------------------------------------------------------------------------------
+ * One-dimensional implicit heat equation u_t = a u_xx + b u_x + c u (+ src) (general_heat_equation,
+ * pde_solvers/one_dimensional/heat_type/lss_1d_general_heat_equation.hpp:134-528) with the Host
+ * ThomasLU / DoubleSweep kernel, theta = 1 (Euler) or 1/2 (CrankNicolson).  Boundaries: Dirichlet
+ * value(t), Neumann value(t) (u' + value = 0), Robin lin(t), value(t) (u' + lin u + value = 0), in the
+ * reference's 1-D boundary conventions.  Callbacks compute in double; the oracle rounds to fp.
+ * ------------------------------------------------------------------------------------------- */
+typedef double (*lss_or_h1_coef_fn)(void *ctx, double t, double x);
+typedef double (*lss_or_h1_init_fn)(void *ctx, double x);
+typedef double (*lss_or_h1_bnd_fn)(void *ctx, double t);
+
+typedef struct lss_or_heat1d_pde {
+    lss_or_h1_coef_fn a, b, c;
+    lss_or_h1_coef_fn src;          /* heat source F(t, x), or NULL */
+    lss_or_h1_init_fn init;
+    int32_t lo_type, hi_type;       /* LSS_OR_BC_* */
+    lss_or_h1_bnd_fn lo_val, lo_lin, hi_val, hi_lin; /* lin: Robin only */
+    void *ctx;
+} lss_or_heat1d_pde;
+
+typedef struct lss_or_heat1d_cfg {
+    double x_lo, x_hi, t_lo, t_hi;
+    int64_t n, nt;                  /* number of space points, number of time points */
+    double accumulation_point, alpha_scale;
+    int32_t nonuniform, implicit_scheme, tridiagonal, direction;
+} lss_or_heat1d_cfg;
+
+/* out: final level (n); surfaces: NULL or nt * n, row time_idx = level time_idx (run_with_stepping) */
+int lss_or_heat1d_solve_f64(const lss_or_heat1d_pde *pde, const lss_or_heat1d_cfg *cfg, double *out, double *surfaces);
+int lss_or_heat1d_solve_f32(const lss_or_heat1d_pde *pde, const lss_or_heat1d_cfg *cfg, float *out, float *surfaces);
+/* physical grid X(zeta_i) (n) and loop times (nt - 1) */
+int lss_or_heat1d_grid_f64(const lss_or_heat1d_cfg *cfg, double *xs, double *times);
+int lss_or_heat1d_grid_f32(const lss_or_heat1d_cfg *cfg, float *xs, float *times);
+
+/* Black-Scholes call data as the reference's tests write it
+ * (unit_tests/pde_solvers/one_dimensional/lss_black_scholes_equation_t.hpp:19-109):
+ * a = 0.5 sig^2 x^2, b = r x, c = -r, terminal max(0, x - K), Dirichlet 0 at x_lo and
+ * x_hi - K exp(-r (T - t)) at x_hi. */
+typedef struct lss_or_bs {
+    double strike, rate, sig, maturity, x_hi;
+} lss_or_bs;
+void lss_or_bs_heat1d(const lss_or_bs *p, lss_or_heat1d_pde *out);
 
 #ifdef __cplusplus
 }

@@ -134,9 +134,15 @@ def lib():
                                                              T, T, P]
             getattr(L, f"lss_or_thomas_batched_{sfx}").argtypes = [C.c_int64, C.c_int64, C.c_int32, P, P, P, P, P]
         L.lss_or_heston_pde.argtypes = [C.POINTER(_Heston), C.POINTER(_Pde)]
+        L.lss_or_set_parallel_min.argtypes = [C.c_int64]
         L.lss_or_sabr_pde.argtypes = [C.POINTER(_Sabr), C.POINTER(_Pde)]
         _lib = L
     return _lib
+
+
+def set_parallel_min(points: int) -> None:
+    """Grids of at least `points` points run the lines of each sweep on OpenMP threads."""
+    lib().lss_or_set_parallel_min(int(points))
 
 
 def _np_dtype(fp: str):
@@ -244,3 +250,111 @@ def thomas_batched(a, b, c, d, B: int, n: int, strided: bool, fp: str = "f64") -
     if rc != 0:
         raise ValueError(f"oracle batched failed rc={rc}")
     return x
+
+
+# ---------------------------------------------------------------------------------------------
+# One-dimensional implicit heat equation (general_heat_equation, Host ThomasLU / DoubleSweep)
+# ---------------------------------------------------------------------------------------------
+H1_COEF_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double, C.c_double)
+H1_INIT_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double)
+H1_BND_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double)
+
+
+class _Heat1dPde(C.Structure):
+    _fields_ = [("a", H1_COEF_FN), ("b", H1_COEF_FN), ("c", H1_COEF_FN), ("src", H1_COEF_FN),
+                ("init", H1_INIT_FN), ("lo_type", C.c_int32), ("hi_type", C.c_int32),
+                ("lo_val", H1_BND_FN), ("lo_lin", H1_BND_FN), ("hi_val", H1_BND_FN), ("hi_lin", H1_BND_FN),
+                ("ctx", C.c_void_p)]
+
+
+class _Heat1dCfg(C.Structure):
+    _fields_ = [("x_lo", C.c_double), ("x_hi", C.c_double), ("t_lo", C.c_double), ("t_hi", C.c_double),
+                ("n", C.c_int64), ("nt", C.c_int64), ("accumulation_point", C.c_double),
+                ("alpha_scale", C.c_double), ("nonuniform", C.c_int32), ("implicit_scheme", C.c_int32),
+                ("tridiagonal", C.c_int32), ("direction", C.c_int32)]
+
+
+class _Bs(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("strike", "rate", "sig", "maturity", "x_hi")]
+
+
+@dataclasses.dataclass
+class Heat1dCfg:
+    """pde_discretization_config_1d + grid_config_hints_1d + heat_implicit_solver_config of one 1-D solve."""
+    x_lo: float
+    x_hi: float
+    n: int
+    nt: int
+    t_lo: float = 0.0
+    t_hi: float = 1.0
+    accumulation_point: float = 0.0
+    alpha_scale: float = 3.0
+    nonuniform: bool = False
+    implicit_scheme: int = CRANK_NICOLSON
+    tridiagonal: int = THOMAS_LU
+    direction: int = BACKWARD
+
+    def c(self) -> _Heat1dCfg:
+        return _Heat1dCfg(self.x_lo, self.x_hi, self.t_lo, self.t_hi, self.n, self.nt, self.accumulation_point,
+                          self.alpha_scale, int(self.nonuniform), self.implicit_scheme, self.tridiagonal,
+                          self.direction)
+
+
+def _h1lib():
+    L = lib()
+    if not getattr(L, "_h1_ready", False):
+        for sfx, T in (("f64", C.c_double), ("f32", C.c_float)):
+            P = C.POINTER(T)
+            getattr(L, f"lss_or_heat1d_solve_{sfx}").argtypes = [C.POINTER(_Heat1dPde), C.POINTER(_Heat1dCfg), P, P]
+            getattr(L, f"lss_or_heat1d_grid_{sfx}").argtypes = [C.POINTER(_Heat1dCfg), P, P]
+        L.lss_or_bs_heat1d.argtypes = [C.POINTER(_Bs), C.POINTER(_Heat1dPde)]
+        L._h1_ready = True
+    return L
+
+
+def bs_heat1d_pde(strike: float, rate: float, sig: float, maturity: float, x_hi: float) -> Pde:
+    """Black-Scholes call data of the reference's 1-D tests (lss_black_scholes_equation_t.hpp:66-80)."""
+    p = _Bs(strike, rate, sig, maturity, x_hi)
+    pde = _Heat1dPde()
+    _h1lib().lss_or_bs_heat1d(C.byref(p), C.byref(pde))
+    return Pde(pde, p)
+
+
+def custom_heat1d_pde(a: Callable, b: Callable, c: Callable, init: Callable, lower: tuple, upper: tuple,
+                      src: Optional[Callable] = None) -> Pde:
+    """1-D data from Python callables: a, b, c, src take (t, x), init (x); lower/upper are
+    (BC_DIRICHLET | BC_NEUMANN, value(t)) or (BC_ROBIN, lin(t), value(t))."""
+    fa, fb, fc = (H1_COEF_FN(lambda _c, t, x, g=g: float(g(t, x))) for g in (a, b, c))
+    fs = H1_COEF_FN(lambda _c, t, x: float(src(t, x))) if src is not None else H1_COEF_FN()
+    fi = H1_INIT_FN(lambda _c, x: float(init(x)))
+
+    def bnd(e):
+        val = H1_BND_FN(lambda _c, t, g=e[-1]: float(g(t)))
+        lin = H1_BND_FN(lambda _c, t, g=e[1]: float(g(t))) if len(e) == 3 else H1_BND_FN()
+        return int(e[0]), val, lin
+
+    lt, lv, ll = bnd(lower)
+    ht, hv, hl = bnd(upper)
+    pde = _Heat1dPde(fa, fb, fc, fs, fi, lt, ht, lv, ll, hv, hl, None)
+    return Pde(pde, (fa, fb, fc, fs, fi, lv, ll, hv, hl))
+
+
+def heat1d_solve(pde: Pde, cfg: Heat1dCfg, fp: str = "f64", surfaces: bool = False):
+    """Final level (n,), or (final level, every level (nt, n)) with surfaces=True."""
+    dt = _np_dtype(fp)
+    out = np.zeros(cfg.n, dtype=dt)
+    surf = np.zeros((cfg.nt, cfg.n), dtype=dt) if surfaces else None
+    rc = getattr(_h1lib(), f"lss_or_heat1d_solve_{fp}")(C.byref(pde.pde), C.byref(cfg.c()), _ptr(out, fp),
+                                                       _ptr(surf, fp) if surfaces else None)
+    if rc != 0:
+        raise ValueError(f"oracle 1-D solve failed rc={rc}")
+    return (out, surf) if surfaces else out
+
+
+def heat1d_grid(cfg: Heat1dCfg, fp: str = "f64"):
+    """(xs, times): the physical grid X(zeta_i) and the loop times."""
+    dt = _np_dtype(fp)
+    xs = np.zeros(cfg.n, dtype=dt)
+    times = np.zeros(cfg.nt - 1, dtype=dt)
+    getattr(_h1lib(), f"lss_or_heat1d_grid_{fp}")(C.byref(cfg.c()), _ptr(xs, fp), _ptr(times, fp))
+    return xs, times

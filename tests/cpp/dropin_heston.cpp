@@ -12,6 +12,7 @@
 // usage: dropin_heston <case> <double|float>  (surface to stdout)
 #include <cmath>
 #include <cstdlib>
+#include <iomanip>
 #include <iostream>
 #include <string>
 
@@ -237,10 +238,14 @@ template <typename T> int run(int which)
 int main(int argc, char **argv)
 {
     if (argc < 3) {
-        std::cerr << "usage: dropin_heston <case 0..5> <double|float>\n";
+        std::cerr << "usage: dropin_heston <case 0..5> <double|float> [full]\n";
         return 2;
     }
     const int which = std::atoi(argv[1]);
+    // "full": enough significant digits to round-trip every value (17 double, 9 float) instead of
+    // the reference's 6
+    if (argc > 3 && std::string(argv[3]) == "full")
+        std::cout << std::setprecision(std::string(argv[2]) == "float" ? 9 : 17);
     try {
         return std::string(argv[2]) == "float" ? run<float>(which) : run<double>(which);
     } catch (const std::exception &e) {

@@ -110,3 +110,62 @@ def load(stem: str, fp: str = "f64"):
     ys = np.array([float(v) for v in lines[3].split(",")])
     vals = np.array([[float(v) for v in row.split(",")] for row in lines[5:] if row.strip()])
     return xs, ys, vals
+
+
+# ---------------------------------------------------------------------------------------------
+# One-dimensional implicit heat equation (general_heat_equation): the reference's 1-D print tests.
+# Each writes <stem>_approx_<type>.txt (the solver's final level, or every level for "Surf") and
+# <stem>_bench_<type>.txt (the closed form: black_scholes_exact or the heat series).
+# model "bs": a = 0.5 sig^2 x^2, b = r x, c = -r, terminal max(0, x - K), Dirichlet 0 and
+#   x_hi - K exp(-r (T - t)); "heat": a = 1, b = 0, c = 0, initial x, Dirichlet 0, 0;
+#   "advdiff": a = 1, b = -1, c = 0, initial 1, Dirichlet 0, 0.
+# ---------------------------------------------------------------------------------------------
+FORWARD, BACKWARD = 0, 1
+BS = dict(strike=10.0, rate=0.2, sig=0.25, maturity=1.0)
+CASES1D = [
+    dict(stem="ImplBlackScholesEquationDirichletBCThomasLUSolverEuler", where="unit_tests/common/lss_print_t.hpp:105-207",
+         model="bs", surf=False,
+         cfg=dict(x_lo=0.0, x_hi=20.0, n=100, nt=100, t_hi=1.0, accumulation_point=0.0, alpha_scale=3.0,
+                  nonuniform=False, implicit_scheme=EULER, tridiagonal=THOMAS_LU, direction=BACKWARD)),
+    dict(stem="ImplBlackScholesEquationDirichletBCThomasLUSolverCrankNicolson",
+         where="unit_tests/common/lss_print_t.hpp:209-311", model="bs", surf=False,
+         cfg=dict(x_lo=0.0, x_hi=20.0, n=100, nt=100, t_hi=1.0, accumulation_point=0.0, alpha_scale=3.0,
+                  nonuniform=False, implicit_scheme=CN, tridiagonal=THOMAS_LU, direction=BACKWARD)),
+    dict(stem="ImplBlackScholesEquationDirichletBCThomasLUSolverEulerSurf",
+         where="unit_tests/common/lss_print_t.hpp:327-437", model="bs", surf=True,
+         cfg=dict(x_lo=0.0, x_hi=20.0, n=100, nt=100, t_hi=1.0, accumulation_point=10.0, alpha_scale=3.0,
+                  nonuniform=False, implicit_scheme=EULER, tridiagonal=THOMAS_LU, direction=BACKWARD)),
+    dict(stem="ImplBlackScholesEquationDirichletBCThomasLUSolverSurf",
+         where="unit_tests/common/lss_print_t.hpp:439-552", model="bs", surf=True,
+         cfg=dict(x_lo=0.0, x_hi=20.0, n=100, nt=100, t_hi=1.0, accumulation_point=10.0, alpha_scale=3.0,
+                  nonuniform=True, implicit_scheme=CN, tridiagonal=THOMAS_LU, direction=BACKWARD)),
+    # the reference ran these two through cuSOLVER QR on the device; the oracle runs Thomas LU
+    dict(stem="ImplPureHeatEquationDirichletBCCUDASolverDeviceQREulerSurf",
+         where="unit_tests/common/lss_print_t.hpp:568-683", model="heat", surf=True,
+         cfg=dict(x_lo=0.0, x_hi=1.0, n=100, nt=100, t_hi=0.1, accumulation_point=0.0, alpha_scale=3.0,
+                  nonuniform=False, implicit_scheme=EULER, tridiagonal=THOMAS_LU, direction=FORWARD)),
+    dict(stem="ImplPureHeatEquationDirichletBCCUDASolverDeviceQRCrankNicolsonSurf",
+         where="unit_tests/common/lss_print_t.hpp:685-802", model="heat", surf=True,
+         cfg=dict(x_lo=0.0, x_hi=1.0, n=100, nt=100, t_hi=0.1, accumulation_point=0.5, alpha_scale=3.0,
+                  nonuniform=True, implicit_scheme=CN, tridiagonal=THOMAS_LU, direction=FORWARD)),
+    dict(stem="ImplAdvDiffEquationDirichletBCThomasLUSolverCN", where="unit_tests/common/lss_print_t.hpp:949-1075",
+         model="advdiff", surf=True,
+         cfg=dict(x_lo=0.0, x_hi=1.0, n=100, nt=100, t_hi=0.1, accumulation_point=0.5, alpha_scale=3.0,
+                  nonuniform=True, implicit_scheme=CN, tridiagonal=THOMAS_LU, direction=FORWARD)),
+]
+
+
+def load1d(stem: str, kind: str = "approx", fp: str = "f64"):
+    """(xs, times or None, values) of a 1-D print file: values (n,) or, with TIME_POINTS, (nt, n)."""
+    path = os.path.join(HERE, f"{stem}_{kind}_{'double' if fp == 'f64' else 'float'}.txt")
+    with open(path) as fh:
+        lines = [ln for ln in fh.read().split("\n")]
+    assert lines[0] == "SPACE_POINTS"
+    xs = np.array([float(v) for v in lines[1].split(",")])
+    if lines[2] == "TIME_POINTS":
+        ts = np.array([float(v) for v in lines[3].split(",")])
+        assert lines[4] == "VALUES"
+        vals = np.array([[float(v) for v in row.split(",")] for row in lines[5:] if row.strip()])
+        return xs, ts, vals
+    assert lines[2] == "VALUES"
+    return xs, None, np.array([float(v) for v in lines[3].split(",")])

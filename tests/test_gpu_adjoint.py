@@ -140,7 +140,7 @@ def test_heston_gradients_match_the_oracle_finite_differences(oracle):
 def test_sabr_parameter_gradients_match_finite_differences():
     m = _sabr()
     plan = L.Plan.from_models([m])
-    assert plan.param_count(0) == 3
+    assert plan.param_count(0) == 4
     x0, y0 = 0.03, 0.05
     w = _price_weights(plan, 0, x0, y0)
     _, _, dp = _grad(plan, [plan.initial(0)], [w], ckpt=4)
@@ -148,6 +148,36 @@ def test_sabr_parameter_gradients_match_finite_differences():
         fd = _fd(lambda **o: _sabr(**o), attr, getattr(m, attr), x0, y0)
         print(f"sabr {name}: adjoint {dp[0, k]:.10e} fd {fd:.10e}", flush=True)
         assert abs(dp[0, k] - fd) <= 1e-6 * abs(fd) + 1e-9
+    # the alpha axis: y_hi and the price point scaled together by lam = 1 +- h
+    h, vals = 1e-6, []
+    for sgn in (1.0, -1.0):
+        lam = 1.0 + sgn * h
+        pq = L.Plan.from_models([_sabr(y_hi=m.y_hi * lam)])
+        vals.append(float(np.sum(_price_weights(pq, 0, x0, y0 * lam) * _run(pq, [pq.initial(0)])[0])))
+    fd = (vals[0] - vals[1]) / (2 * h)
+    print(f"sabr alpha axis: adjoint {dp[0, 3]:.10e} fd {fd:.10e}", flush=True)
+    assert abs(dp[0, 3] - fd) <= 1e-6 * abs(fd) + 1e-9
+
+
+def test_sabr_alpha_axis_gradient_with_a_raised_lower_end():
+    """With y_lo > 0 the alpha-axis terms of b and e do not vanish (generic adjoint tables); the price is
+    read at a fixed eta, i.e. at y_lo + lam (y0 - y_lo)."""
+    mk = lambda **o: _sabr(y_lo=0.01, **o)  # noqa: E731
+    m = mk()
+    plan = L.Plan.from_models([m])
+    x0, y0 = 0.03, 0.05
+    w = _price_weights(plan, 0, x0, y0)
+    _, _, dp = _grad(plan, [plan.initial(0)], [w], ckpt=4)
+    assert plan.kernel_config()["adjoint_pattern"] == 0
+    h, vals = 1e-6, []
+    for sgn in (1.0, -1.0):
+        lam = 1.0 + sgn * h
+        pq = L.Plan.from_models([mk(y_hi=m.y_lo + (m.y_hi - m.y_lo) * lam)])
+        vals.append(float(np.sum(_price_weights(pq, 0, x0, m.y_lo + (y0 - m.y_lo) * lam) *
+                                 _run(pq, [pq.initial(0)])[0])))
+    fd = (vals[0] - vals[1]) / (2 * h)
+    print(f"sabr alpha axis (y_lo 0.01): adjoint {dp[0, 3]:.10e} fd {fd:.10e}", flush=True)
+    assert abs(dp[0, 3] - fd) <= 1e-6 * abs(fd) + 1e-9
 
 
 def test_batched_adjoint_equals_single_problem_adjoints():

@@ -59,3 +59,31 @@ def test_dropin_float_matches_reference_surface(case):
     steps = next(c for c in R.CASES if c["stem"] == CASES[case])["cfg"]["nt"] - 1
     # float rounding accumulates over the steps (tests/test_oracle_reference_outputs.py)
     assert np.abs(got - ref).max() <= 1.5e-5 * max(1.0, (steps / 200) ** 0.5) * np.abs(ref).max()
+
+
+def _run_full(case, fp):
+    out = subprocess.run([BIN, str(case), fp, "full"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.split("\n")
+    assert lines[4] == "VALUES"
+    return np.array([[float(v) for v in r.split(",")] for r in lines[5:] if r.strip()])
+
+
+@pytest.mark.parametrize("fp", ["double", "float"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_dropin_full_precision_equals_oracle(oracle, case, fp):
+    """The surface printed with round-trip precision (17 significant digits for double, 9 for float)
+    equals the oracle's surface bit for bit: the drop-in's C++ sampling of the user's closures
+    (FIELDS coefficients, boundary sets, index maps, loop times) and the exact line order reproduce
+    the reference's host arithmetic."""
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/dropin_heston not built (run __graft_entry__.build())")
+    got = _run_full(case, fp)
+    c = next(c for c in R.CASES if c["stem"] == CASES[case])
+    from tests.helpers import model_from_case, oracle_solve
+    o = oracle_solve(oracle, model_from_case(c), "f64" if fp == "double" else "f32")
+    got = got.astype(np.float64 if fp == "double" else np.float32)
+    diff = np.abs(got.astype(np.float64) - o.astype(np.float64)).max()
+    print(f"case {case} {fp}: max |drop-in - oracle| = {diff:.3e}", flush=True)
+    assert got.shape == o.shape
+    assert np.array_equal(got, o.astype(got.dtype))

@@ -101,3 +101,19 @@ def test_loop_times_accumulate_like_the_reference(oracle):
         t -= k
     assert np.array_equal(times, np.array(expect))
     assert times[-1] != 0.0 and abs(times[-1]) < 1e-14
+
+
+@pytest.mark.parametrize("splitting", [0, 1, 2, 3])
+def test_threaded_oracle_equals_the_serial_one(oracle, splitting):
+    """The oracle's OpenMP line loops (used for large grids) give the serial result bit for bit."""
+    import dataclasses
+    from tests.helpers import config0, oracle_solve
+    m = dataclasses.replace(config0(nx=130, ny=70, nt=9), splitting=splitting, weighting=0.6)
+    try:
+        oracle.set_parallel_min(1 << 40)
+        serial = oracle_solve(oracle, m)
+        oracle.set_parallel_min(0)
+        threaded = oracle_solve(oracle, m)
+    finally:
+        oracle.set_parallel_min(262144)
+    assert np.array_equal(serial, threaded)
