@@ -211,6 +211,21 @@ int lss_tridiag_batched_bc(int32_t fp, int32_t layout, int64_t B, int64_t n, con
                            const void *c, const void *d, const lss_line_bc *lower, const lss_line_bc *upper, void *x,
                            void *stream);
 
+/*
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the solve of
+ * lss_tridiag_batched (lower == upper == NULL) or lss_tridiag_batched_bc (both non-NULL) that calls
+ * no hipMalloc/hipFree or allocating routine.  The one-lane-per-system kernel (systems of more than
+ * 1024 unknowns, or contiguous fp64 systems of more than 512) takes its B*n values of scratch from a
+ * static per-device array of LSS_TRIDIAG_SCRATCH_BYTES, which each device holds until the process
+ * ends; larger B*n*sizeof(fp) is rejected with LSS_ERR_INVALID.  Every call on a device shares that
+ * array: calls that overlap on one device (different streams or threads) race and give wrong results.
+ * The caller passes inputs and outputs as for lss_tridiag_batched.
+ */
+#define LSS_TRIDIAG_SCRATCH_BYTES ((int64_t)64 << 20)
+int lss_tridiag_batched_noalloc(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b,
+                                const void *c, const void *d, const lss_line_bc *lower, const lss_line_bc *upper,
+                                void *x, void *stream);
 
 /*
  * Same time loop as lss_adi_run_dev (device pointers, no graph) with a hipEvent pair recorded on

@@ -38,7 +38,9 @@ EXPORTS = ("lss_last_error", "lss_version", "lss_adi_plan_create", "lss_adi_plan
            "lss_adi_run_host", "lss_model_plan_create", "lss_model_initial", "lss_model_grid", "lss_plan_problem_shape",
            "lss_tridiag_batched", "lss_adi_run_profiled", "lss_adi_set_param_terms", "lss_plan_param_count",
            "lss_adi_forward", "lss_adi_backward", "lss_adi_autotune",
-           "lss_adi_set_mapping", "lss_plan_kernel_config", "lss_tridiag_batched_bc", "lss_adi_set_adjoint_overlap")
+           "lss_adi_set_mapping", "lss_plan_kernel_config", "lss_tridiag_batched_bc", "lss_adi_set_adjoint_overlap",
+           "lss_tridiag_batched_noalloc")
+TRIDIAG_SCRATCH_BYTES = 64 << 20  # static scratch of lss_tridiag_batched_noalloc (LSS_TRIDIAG_SCRATCH_BYTES)
 MAX_PARAMS = 6
 MAX_TERMS = 12
 HESTON_PARAMS = ("kappa", "theta_v", "sigma", "rho", "r")
@@ -163,6 +165,9 @@ def lib():
         L.lss_tridiag_batched_bc.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p,
                                              C.c_void_p, C.c_void_p, C.POINTER(_LineBC), C.POINTER(_LineBC),
                                              C.c_void_p, C.c_void_p]
+        L.lss_tridiag_batched_noalloc.argtypes = [C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_void_p,
+                                                  C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(_LineBC),
+                                                  C.POINTER(_LineBC), C.c_void_p, C.c_void_p]
         L.lss_adi_set_param_terms.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_int32),
                                               C.POINTER(C.c_int32), C.c_int64, C.c_void_p]
         L.lss_plan_param_count.argtypes = [C.c_void_p, C.c_int64, C.POINTER(C.c_int32)]
@@ -363,3 +368,15 @@ def tridiag_batched_bc(fp: int, layout: int, B: int, n: int, a: int, b: int, c: 
                                         C.c_void_p(d), C.byref(lo), C.byref(hi), C.c_void_p(x),
                                         C.c_void_p(stream)))
 
+
+def tridiag_batched_noalloc(fp: int, layout: int, B: int, n: int, a: int, b: int, c: int, d: int, x: int,
+                            lower: Optional[tuple] = None, upper: Optional[tuple] = None, stream: int = 0) -> None:
+    """lss_tridiag_batched (lower = upper = None) or lss_tridiag_batched_bc without any device allocation:
+    the one-lane-per-system kernel takes its scratch from the library's static per-device array
+    (TRIDIAG_SCRATCH_BYTES).  Calls that overlap on one device race on that array."""
+    def mk(e):
+        return _LineBC(int(e[0]), float(e[1]) if len(e) == 3 else 0.0, float(e[-1]))
+    lo = C.byref(mk(lower)) if lower is not None else None
+    hi = C.byref(mk(upper)) if upper is not None else None
+    _check(lib().lss_tridiag_batched_noalloc(fp, layout, B, n, C.c_void_p(a), C.c_void_p(b), C.c_void_p(c),
+                                             C.c_void_p(d), lo, hi, C.c_void_p(x), C.c_void_p(stream)))

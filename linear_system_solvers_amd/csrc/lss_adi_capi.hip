@@ -1874,6 +1874,35 @@ void tridiag_part(int M, int layout, int64_t B, int n, const void *a, const void
     }
 }
 
+}  // namespace
+
+// Scratch of lss_tridiag_batched_noalloc: one array per device, held until the process ends.  Host
+// code never passes `tridiag_scratch` itself to a kernel (on the host it is only a stand-in for the
+// symbol); it uses the device address that hipGetSymbolAddress returns for the current device.
+static constexpr size_t TRIDIAG_SCRATCH_BYTES = (size_t)LSS_TRIDIAG_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char tridiag_scratch[TRIDIAG_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mutex;
+std::vector<void *> g_scratch_ptr;  // per device: address of tridiag_scratch (null until first use)
+
+void *static_scratch()
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if ((int)g_scratch_ptr.size() <= dev)
+        g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(tridiag_scratch)));
+        require(ptr != nullptr, "hipGetSymbolAddress returned a null scratch address");
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
 template <typename T> LineBC<T> line_bc(const lss_line_bc *lo, const lss_line_bc *hi, int64_t n)
 {
     LineBC<T> bc;
@@ -1888,11 +1917,13 @@ template <typename T> LineBC<T> line_bc(const lss_line_bc *lo, const lss_line_bc
     return bc;
 }
 
-// lss_tridiag_batched(_bc): lower/upper null = plain rows.  The one-lane-per-system kernel needs
-// B * n values of scratch: a stream-ordered allocation that is freed on `st` after the kernel, so that
-// calls on different streams never share scratch memory.
+// lss_tridiag_batched(_bc, _noalloc): lower/upper null = plain rows.  The one-lane-per-system kernel
+// needs B * n values of scratch: with noalloc it uses tridiag_scratch (sizes beyond it are rejected),
+// otherwise a stream-ordered allocation that is freed on `st` after the kernel, so that calls on
+// different streams never share scratch memory.
 void tridiag_run(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b, const void *c,
-                 const void *d, const lss_line_bc *lo, const lss_line_bc *hi, void *x, hipStream_t st)
+                 const void *d, const lss_line_bc *lo, const lss_line_bc *hi, void *x, hipStream_t st,
+                 bool noalloc = false)
 {
     require(fp == LSS_FP64 || fp == LSS_FP32, "fp must be LSS_FP64 or LSS_FP32");
     require(layout == LSS_LAYOUT_CONTIGUOUS || layout == LSS_LAYOUT_STRIDED, "unknown layout");
@@ -1921,9 +1952,18 @@ void tridiag_run(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a
     } else {
         const size_t bytes = (size_t)B * (size_t)n * es;
         void *scr = nullptr;
-        hipError_t e = hipMallocAsync(&scr, bytes, st);
-        if (e != hipSuccess)
-            throw lss_error(LSS_ERR_ALLOC, std::string("scratch hipMallocAsync failed: ") + hipGetErrorString(e));
+        if (noalloc) {
+            require(bytes <= TRIDIAG_SCRATCH_BYTES,
+                    "lss_tridiag_batched_noalloc: B * n * sizeof(fp) = " + std::to_string(bytes) +
+                        " bytes of scratch exceeds its static maximum of " + std::to_string(TRIDIAG_SCRATCH_BYTES) +
+                        " bytes (systems of more than 1024 unknowns, or contiguous fp64 systems of more than 512, "
+                        "need B * n values of scratch)");
+            scr = static_scratch();
+        } else {
+            hipError_t e = hipMallocAsync(&scr, bytes, st);
+            if (e != hipSuccess)
+                throw lss_error(LSS_ERR_ALLOC, std::string("scratch hipMallocAsync failed: ") + hipGetErrorString(e));
+        }
         const unsigned blocks = (unsigned)((B + 255) / 256);
         if (fp == LSS_FP64)
             hipLaunchKernelGGL(k_tridiag_lane<double>, dim3(blocks), dim3(256), 0, st, B, (int)n, (int)layout,
@@ -1934,7 +1974,8 @@ void tridiag_run(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a
                                (const float *)a, (const float *)b, (const float *)c, (const float *)d,
                                (float *)x, (float *)scr, line_bc<float>(lo, hi, n));
         const hipError_t le = hipGetLastError();
-        HIPCHK(hipFreeAsync(scr, st));
+        if (!noalloc)
+            HIPCHK(hipFreeAsync(scr, st));
         HIPCHK(le);
         return;
     }
@@ -2146,6 +2187,16 @@ int lss_tridiag_batched_bc(int32_t fp, int32_t layout, int64_t B, int64_t n, con
     return guarded([&] {
         require(lower && upper, "null end condition");
         tridiag_run(fp, layout, B, n, a, b, c, d, lower, upper, x, (hipStream_t)stream);
+    });
+}
+
+int lss_tridiag_batched_noalloc(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a, const void *b,
+                                const void *c, const void *d, const lss_line_bc *lower, const lss_line_bc *upper,
+                                void *x, void *stream)
+{
+    return guarded([&] {
+        require((lower == nullptr) == (upper == nullptr), "lower and upper end conditions: both or neither");
+        tridiag_run(fp, layout, B, n, a, b, c, d, lower, upper, x, (hipStream_t)stream, true);
     });
 }
 
