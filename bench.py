@@ -691,7 +691,7 @@ def main():
         "config": {"workload": "heston_book_2048_seed3", "problems": len(models), "seed": 3,
                    "grid_point_steps_per_solve": total_ps, "coeff": "separable", "line_solver": "partitioned",
                    "graph": not args.no_graph, "kernel_config": plan.kernel_config(),
-                   "assumed": {"grid": "grid_enum::Uniform (sinh map, accumulation point 0; configs[0]'s grid)",
+                   "assumed": {"grid": "grid_enum::Uniform (sinh map, accumulation point 0, as in the 100x50 Heston DR call)",
                                "S_range": "[0, 2K]", "v_range": "[0, 1.2]", "theta": 0.5,
                                "implicit_scheme": "CrankNicolson", "zeta_weighting": 0.5,
                                "hints": "grid_config_hints_2d defaults (point 0, alpha_scale 3, beta_scale 50)",

@@ -162,4 +162,65 @@ template <typename fp_type, template <typename, typename> typename container = s
           typename allocator = std::allocator<fp_type>>
 using ccontainer_2d = container_2d<by_enum::Column, fp_type, container, allocator>;
 
+// container_3d: layers of container_2d (containers/lss_container_3d.hpp:19-200 of the reference): element,
+// whole-layer access and replacement.
+template <by_enum by, typename fp_type, template <typename, typename> typename container, typename allocator>
+class container_3d {
+    std::size_t rows_ = 0, columns_ = 0, layers_ = 0;
+    std::vector<container_2d<by, fp_type, container, allocator>> rdata_;
+
+  public:
+    typedef fp_type value_type;
+    typedef container_2d<by, fp_type, container, allocator> element_type;
+
+    explicit container_3d(std::size_t rows, std::size_t columns, std::size_t layers)
+        : rows_{rows}, columns_{columns}, layers_{layers}
+    {
+        for (std::size_t l = 0; l < layers; ++l)
+            rdata_.emplace_back(element_type(rows, columns));
+    }
+    explicit container_3d(std::size_t rows, std::size_t columns, std::size_t layers, fp_type value)
+        : rows_{rows}, columns_{columns}, layers_{layers}
+    {
+        for (std::size_t l = 0; l < layers; ++l)
+            rdata_.emplace_back(element_type(rows, columns, value));
+    }
+    std::size_t rows() const { return rows_; }
+    std::size_t columns() const { return columns_; }
+    std::size_t layers() const { return layers_; }
+    std::size_t total_size() const { return rows_ * columns_ * layers_; }
+
+    fp_type operator()(std::size_t row_idx, std::size_t col_idx, std::size_t lay_idx) const
+    {
+        LSS_ASSERT(row_idx < rows_, "Outside of row range");
+        LSS_ASSERT(col_idx < columns_, "Outside of column range");
+        LSS_ASSERT(lay_idx < layers_, "Outside of layer range");
+        return rdata_[lay_idx](row_idx, col_idx);
+    }
+    element_type operator()(std::size_t lay_idx) const
+    {
+        LSS_ASSERT(lay_idx < layers_, "Outside of layer range");
+        return rdata_[lay_idx];
+    }
+    fp_type at(std::size_t row_idx, std::size_t col_idx, std::size_t lay_idx) const
+    {
+        return operator()(row_idx, col_idx, lay_idx);
+    }
+    element_type at(std::size_t lay_idx) const { return operator()(lay_idx); }
+    void operator()(std::size_t lay_idx, element_type const &container2d)
+    {
+        LSS_ASSERT(lay_idx < layers_, "Outside of layer range");
+        LSS_ASSERT(container2d.rows() == rows_, "Outside of row range");
+        LSS_ASSERT(container2d.columns() == columns_, "Outside of column range");
+        rdata_[lay_idx] = container2d;
+    }
+    void operator()(std::size_t row_idx, std::size_t col_idx, std::size_t lay_idx, fp_type value)
+    {
+        LSS_ASSERT(lay_idx < layers_, "Outside of layer range");
+        rdata_[lay_idx](row_idx, col_idx, value);
+    }
+    // layer lay_idx itself (drop-in extension used by the GPU path)
+    element_type &layer(std::size_t lay_idx) { return rdata_[lay_idx]; }
+};
+
 }  // namespace lss_containers

@@ -10,8 +10,12 @@
 //     reference's host solver returns;
 //   * boundaries are read through value() whatever their type, as heston_boundary_transform does
 //     (lss_heston_boundary_transform.hpp:36-66): Dirichlet at S_min, Neumann at S_max, Dirichlet at v_max;
-//   * a heat source makes the reference skip the time loop (implicit_kernel.hpp:1032-1044): the
-//     result is then the zero surface (Host) or the sampled initial surface (Device), as there.
+//   * a heat source is stepped: the first stage of every scheme adds theta rho F(t_next) +
+//     (1 - theta) rho F(t) (rhs_intermed_1_source, lss_heat_douglas_rachford_method.hpp:70-104, and
+//     the same function of CS/MCS/HV).  This is the one place where the drop-in does not return the
+//     reference's result: the reference defines those functions but skips the time loop when a
+//     source is set (implicit_kernel.hpp:1032-1044) and returns the zero surface (Host) or the
+//     sampled initial surface (Device).
 // Errors of the GPU library are thrown as std::runtime_error; size mismatches terminate through
 // LSS_ASSERT as in the reference.
 #pragma once
@@ -33,6 +37,7 @@ namespace implicit_solvers {
 using lss_boundary::boundary_2d_pair;
 using lss_boundary::boundary_2d_ptr;
 using lss_containers::container_2d;
+using lss_containers::container_3d;
 using lss_enumerations::by_enum;
 using lss_enumerations::grid_enum;
 using lss_grids::grid_config_hints_2d_ptr;
@@ -139,6 +144,8 @@ class general_heston_equation {
         u.h_lower = [lo](fp_type t, fp_type y) { return lo->value(t, y); };
         u.h_upper = [hi](fp_type t, fp_type y) { return hi->value(t, y); };
         u.v_upper = [vu](fp_type t, fp_type x) { return vu->value(t, x); };
+        if (heat_data_cfg_->source_data_config())
+            u.src = heat_data_cfg_->source_data_config()->heat_source();
         return u;
     }
 
@@ -157,15 +164,6 @@ class general_heston_equation {
         const auto g = grid();
         const auto u = data();
         lss_amd_detail::sampled<fp_type> s;
-        if (heat_data_cfg_->source_data_config()) {
-            lss_amd_detail::sample_grid(g, s);
-            for (std::size_t i = 0; i < g.nx; ++i)
-                for (std::size_t j = 0; j < g.ny; ++j)
-                    solution(i, j,
-                             solver_cfg_->memory_space() == memory_space_enum::Device ? u.init(s.xs[i], s.ys[j])
-                                                                                       : fp_type{});
-            return;
-        }
         lss_amd_detail::sample_fields(u, g, s, lss_amd_detail::TIME_AUTO, lss_amd_detail::TIME_AUTO);
         const lss_adi_desc d = s.desc(LSS_COEFF_FIELDS, LSS_LINE_EXACT);
         lss_adi_plan *plan = nullptr;
@@ -175,6 +173,43 @@ class general_heston_equation {
         const int rc = lss_adi_run_host(plan, in, out);
         lss_adi_plan_destroy(plan);
         detail::check(rc, "lss_adi_run_host");
+    }
+
+    /**
+     * Get every time level of the PDE (heston_equation.hpp:180, 309-313; the reference declares this
+     * solve and leaves its body empty).  Layers are time indices as the reference's 1-D
+     * run_with_stepping stores them (lss_1d_general_heat_equation_implicit_kernel.hpp:163-257): a
+     * Forward loop puts the initial surface in layer 0 and the surface after s steps in layer s; a
+     * Backward loop puts the terminal surface in the last layer and the surface after s steps in layer
+     * number_of_time_points - 1 - s.  Layer time_idx thus holds the solution at time t_lo + time_idx k.
+     *
+     * \param solutions - 3D container, nx rows (S) by ny columns (v) by number_of_time_points layers
+     */
+    void solve(container_3d<by_enum::Row, fp_type, container, allocator> &solutions)
+    {
+        const auto np = discretization_cfg_->number_of_space_points();
+        const std::size_t nt = discretization_cfg_->number_of_time_points();
+        LSS_ASSERT((solutions.rows() == np.first) && (solutions.columns() == np.second) && (solutions.layers() == nt),
+                   "The input solution container must have the correct size");
+        const auto g = grid();
+        const auto u = data();
+        lss_amd_detail::sampled<fp_type> s;
+        lss_amd_detail::sample_fields(u, g, s, lss_amd_detail::TIME_AUTO, lss_amd_detail::TIME_AUTO);
+        const lss_adi_desc d = s.desc(LSS_COEFF_FIELDS, LSS_LINE_EXACT);
+        lss_adi_plan *plan = nullptr;
+        detail::check(lss_adi_plan_create(&d, 1, &plan), "lss_adi_plan_create");
+        const std::size_t nxy = np.first * np.second;
+        std::vector<fp_type> levels(nt * nxy);
+        const void *in[1] = {s.init.data()};
+        void *lv[1] = {levels.data()};
+        const int rc = lss_adi_run_levels(plan, in, lv, nullptr, 1);
+        lss_adi_plan_destroy(plan);
+        detail::check(rc, "lss_adi_run_levels");
+        const bool fwd = g.direction == LSS_FORWARD;
+        for (std::size_t st = 0; st < nt; ++st) {
+            auto &layer = solutions.layer(fwd ? st : nt - 1 - st);
+            std::copy(levels.begin() + st * nxy, levels.begin() + (st + 1) * nxy, layer.raw());
+        }
     }
 };
 

@@ -11,8 +11,8 @@
 //   index maps                     discretization/lss_grid_config.hpp:56-59, 110-118
 //   loop times                     lss_2d_general_heston_equation_implicit_kernel.hpp:171-206
 // Time dependence: coefficient and boundary sets are sampled at every loop time unless the caller
-// states they are time independent, or (auto mode) the samples at the first, middle and last loop
-// time agree bit for bit, in which case one set is kept.
+// states they are time independent, or (auto mode) the samples at every loop time agree bit for bit
+// with the first, in which case one set is kept.  A heat source is sampled in auto mode.
 #pragma once
 
 #include <cmath>
@@ -31,6 +31,7 @@ template <typename T> struct user_data {
     std::function<T(T, T)> h_lower;              // Dirichlet at x_lo: value(t, y)
     std::function<T(T, T)> h_upper;              // Neumann at x_hi: value(t, y)
     std::function<T(T, T)> v_upper;              // Dirichlet at y_hi: value(t, x)
+    std::function<T(T, T, T)> src;               // heat source F(t, x, y), or empty
 };
 
 template <typename T> struct grid_spec {
@@ -89,6 +90,8 @@ template <typename T> struct sampled {
     std::size_t coeff_sets = 0;
     std::vector<T> bnd;
     std::size_t bnd_sets = 0;
+    std::vector<T> src;     // heat source sets [F(t) | F(t_next)] (empty without a source)
+    std::size_t src_sets = 0;
     std::vector<T> init;    // nx*ny initial surface
 
     lss_adi_desc desc(int coeff_mode, int line_solver = LSS_LINE_EXACT) const
@@ -118,6 +121,8 @@ template <typename T> struct sampled {
         d.coeff = coeff.data();
         d.bnd_sets = (int64_t)bnd_sets;
         d.bnd = bnd.data();
+        d.src_sets = (int64_t)src_sets;
+        d.src = src_sets ? src.data() : nullptr;
         return d;
     }
 };
@@ -298,6 +303,19 @@ void sample_fields(const user_data<T> &u, const grid_spec<T> &g, sampled<T> &s, 
     auto fill_b = [&](T t, T *o) { sample_bnd_set(u, g, s, t, o); };
     s.coeff_sets = sample(coeff_time, cstride, fill_c, s.coeff);
     s.bnd_sets = sample(bnd_time, bstride, fill_b, s.bnd);
+    if (u.src) {
+        // the source at the loop time t and at the loop's next time (t + k Forward, t - k Backward),
+        // at the physical points (X(zeta_i), Y(eta_j)) (lss_heat_data_transform.hpp:164-173)
+        auto fill_s = [&](T t, T *o) {
+            const T tn = g.direction == LSS_FORWARD ? t + s.k : t - s.k;
+            for (std::size_t i = 0; i < nx; ++i)
+                for (std::size_t j = 0; j < ny; ++j) {
+                    o[i * ny + j] = u.src(t, s.xs[i], s.ys[j]);
+                    o[nx * ny + i * ny + j] = u.src(tn, s.xs[i], s.ys[j]);
+                }
+        };
+        s.src_sets = sample(TIME_AUTO, 2 * nx * ny, fill_s, s.src);
+    }
 }
 
 }  // namespace lss_amd_detail

@@ -68,6 +68,13 @@ enum { LSS_LAYOUT_CONTIGUOUS = 0, LSS_LAYOUT_STRIDED = 1 };
  *              h_upper(t, Y(eta_j)) * X'(1) (lss_heston_boundary_transform.hpp:48-62).
  *   idx1/idx2  index_of maps (lss_grid_config.hpp:110-118).
  *   theta..h1  scheme scalars, already rounded to fp (implicit_kernel.hpp:978-995, coefficients.hpp:62-73).
+ *   src        heat source: src_sets == 0 (none, src NULL), 1 (time independent) or steps sets of
+ *              2 nx*ny values, F(t, X(zeta_i), Y(eta_j)) at loop step s's time t and at t_next
+ *              (t - k on a Backward loop, t + k Forward), row-major; stage 1 adds
+ *              theta rho F(t_next) + (1 - theta) rho F(t) (rhs_intermed_1_source,
+ *              lss_heat_douglas_rachford_method.hpp:70-104 and the same function of CS/MCS/HV, which the
+ *              reference defines but does not call: its loop returns without stepping when a source is
+ *              set, implicit_kernel.hpp:1032-1044).  ADI plans only.
  * FIELDS with LSS_LINE_EXACT reproduces the reference's host result bit for bit.  The partitioned
  * solver needs (nx - 2) and (ny - 2) <= 64 * 16.
  * All pointers are host pointers; the plan copies them to device memory.
@@ -87,6 +94,8 @@ typedef struct lss_adi_desc {
     const void *coeff;
     int64_t bnd_sets;
     const void *bnd;
+    int64_t src_sets;
+    const void *src;
 } lss_adi_desc;
 
 typedef struct lss_adi_plan lss_adi_plan;
@@ -139,6 +148,15 @@ int lss_adi_set_adjoint_overlap(lss_adi_plan *plan, int32_t mode);
 enum { LSS_KERNEL_CONFIG_LEN = 12 };
 int lss_plan_kernel_config(const lss_adi_plan *plan, int32_t *config);
 int lss_adi_run_host(lss_adi_plan *plan, const void *const *u_in, void *const *u_out);
+/*
+ * Every time level of the loop (solve(container_3d&), lss_2d_general_heston_equation.hpp:180, 309-313,
+ * whose body the reference leaves empty; the layering follows the reference's 1-D run_with_stepping,
+ * lss_1d_general_heat_equation_implicit_kernel.hpp:163-257): levels[p] receives steps + 1 surfaces of
+ * nx*ny values, row s being the surface after s steps of the loop (row 0 = u_in[p]).  host = 0: device
+ * pointers, enqueued on `stream`; host != 0: host pointers, synchronous.  No graph replay.
+ */
+int lss_adi_run_levels(lss_adi_plan *plan, const void *const *u_in, void *const *levels, void *stream,
+                       int32_t host);
 
 /*
  * Reference-model convenience: sample the Heston or SABR data of the reference's tests
@@ -273,6 +291,9 @@ int lss_plan_param_count(const lss_adi_plan *plan, int64_t p, int32_t *nparam);
 int lss_adi_forward(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, int32_t ckpt_every, void *stream);
 int lss_adi_backward(lss_adi_plan *plan, const void *const *lam_out, void *const *lam_in, double *dparams,
                      void *stream);
+/* The same pair on host pointers (staged through device memory; synchronous). */
+int lss_adi_forward_host(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, int32_t ckpt_every);
+int lss_adi_backward_host(lss_adi_plan *plan, const void *const *lam_out, void *const *lam_in, double *dparams);
 
 #ifdef __cplusplus
 }

@@ -39,7 +39,7 @@ EXPORTS = ("lss_last_error", "lss_version", "lss_adi_plan_create", "lss_adi_plan
            "lss_tridiag_batched", "lss_adi_run_profiled", "lss_adi_set_param_terms", "lss_plan_param_count",
            "lss_adi_forward", "lss_adi_backward", "lss_adi_autotune",
            "lss_adi_set_mapping", "lss_plan_kernel_config", "lss_tridiag_batched_bc", "lss_adi_set_adjoint_overlap",
-           "lss_tridiag_batched_noalloc")
+           "lss_tridiag_batched_noalloc", "lss_adi_run_levels", "lss_adi_forward_host", "lss_adi_backward_host")
 TRIDIAG_SCRATCH_BYTES = 64 << 20  # static scratch of lss_tridiag_batched_noalloc (LSS_TRIDIAG_SCRATCH_BYTES)
 MAX_PARAMS = 6
 MAX_TERMS = 12
@@ -79,7 +79,7 @@ class _Desc(C.Structure):
                 *[(n, C.c_double) for n in ("theta", "weighting", "alpha", "beta", "gamma", "delta", "ni", "rho",
                                             "h1")],
                 ("idx1", C.c_void_p), ("idx2", C.c_void_p), ("coeff_sets", C.c_int64), ("coeff", C.c_void_p),
-                ("bnd_sets", C.c_int64), ("bnd", C.c_void_p)]
+                ("bnd_sets", C.c_int64), ("bnd", C.c_void_p), ("src_sets", C.c_int64), ("src", C.c_void_p)]
 
 
 @dataclasses.dataclass
@@ -180,6 +180,11 @@ def lib():
                                       C.c_void_p]
         L.lss_adi_backward.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
                                        C.POINTER(C.c_double), C.c_void_p]
+        L.lss_adi_run_levels.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_void_p,
+                                         C.c_int32]
+        L.lss_adi_forward_host.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p), C.c_int32]
+        L.lss_adi_backward_host.argtypes = [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p),
+                                            C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -255,6 +260,28 @@ class Plan:
         pout = (C.c_void_p * self.count)(*[a.ctypes.data for a in outs])
         _check(lib().lss_adi_run_host(self._h, pin, pout))
         return outs
+
+    def run_levels_host(self, inits: Optional[Sequence[np.ndarray]] = None) -> list:
+        """Every time level from host arrays: per problem an array (steps + 1, nx, ny), row s = the surface
+        after s steps (row 0 = the initial surface)."""
+        dt = np_dtype(self.fp)
+        if inits is None:
+            inits = [self.initial(p) for p in range(self.count)]
+        shapes = [self.shape(p) for p in range(self.count)]
+        if len(inits) != self.count or any(np.shape(a) != sh[:2] for a, sh in zip(inits, shapes)):
+            raise LssError(-1, "run_levels_host needs one surface of the problem's shape per problem")
+        ins = [np.ascontiguousarray(a, dtype=dt) for a in inits]
+        lv = [np.zeros((st + 1, nx, ny), dtype=dt) for nx, ny, st in shapes]
+        pin = (C.c_void_p * self.count)(*[a.ctypes.data for a in ins])
+        plv = (C.c_void_p * self.count)(*[a.ctypes.data for a in lv])
+        _check(lib().lss_adi_run_levels(self._h, pin, plv, None, 1))
+        return lv
+
+    def run_levels_dev(self, in_ptrs: Sequence[int], level_ptrs: Sequence[int], stream: int = 0) -> None:
+        """Enqueue a solve that writes every level ((steps + 1) * nx * ny values per problem, device)."""
+        pin = (C.c_void_p * self.count)(*in_ptrs)
+        plv = (C.c_void_p * self.count)(*level_ptrs)
+        _check(lib().lss_adi_run_levels(self._h, pin, plv, C.c_void_p(stream), 0))
 
     def run_dev(self, in_ptrs: Sequence[int], out_ptrs: Sequence[int], stream: int = 0, graph: bool = True) -> None:
         """Enqueue a solve on `stream` from/to device pointers (e.g. torch tensors' data_ptr())."""

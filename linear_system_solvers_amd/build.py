@@ -17,7 +17,7 @@ LIB = os.path.join(PKG, "liblss_amd.so")
 # the heaviest translation units first, so that a parallel build ends soonest
 SOURCES = ([f"lss_adi_partitioned_{t}_{p}.hip" for t in ("f64", "f32") for p in ("generic", "heston", "sabr")] +
            [f"lss_adi_adjoint_{t}_{p}.hip" for t in ("f64", "f32") for p in ("generic", "heston", "sabr")] +
-           ["lss_adi_kernels.hip", "lss_tridiag.hip", "lss_adi_capi.hip"])
+           ["lss_adi_kernels.hip", "lss_tridiag.hip", "lss_adi_capi.hip", "lss_heat1d_wave.hip", "lss_heat1d.hip"])
 ARCH = os.environ.get("LSS_OFFLOAD_ARCH", "gfx950")
 
 

@@ -23,6 +23,7 @@
 #include "lss_adi_device.hpp"
 #include "lss_adi_stencil.hpp"
 #include "lss_amd.h"
+#include "lss_capi_common.hpp"
 #include "lss/lss_adi_sampler.hpp"
 
 namespace lssamd {
@@ -55,70 +56,21 @@ constexpr int P_NCH = 64, P_XTJ = 16, P_VROWS = 4;
 
 using namespace lssamd;
 
+using namespace lssamd_host;
+
+std::string &lssamd_host::last_error()
+{
+    thread_local std::string message;
+    return message;
+}
+
+#define HIPCHK(expr) LSS_HIPCHK(expr)
+
 namespace {
-
-thread_local std::string g_last_error;
-
-struct lss_error : std::runtime_error {
-    int code;
-    lss_error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHK(expr)                                                                                           \
-    do {                                                                                                       \
-        hipError_t e_ = (expr);                                                                                \
-        if (e_ != hipSuccess)                                                                                  \
-            throw lss_error(LSS_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));                \
-    } while (0)
-
-template <typename F> int guarded(F &&f)
-{
-    try {
-        f();
-        return LSS_OK;
-    } catch (const lss_error &e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        g_last_error = "host allocation failed";
-        return LSS_ERR_ALLOC;
-    } catch (const std::exception &e) {
-        g_last_error = e.what();
-        return LSS_ERR_INVALID;
-    }
-}
-
-void require(bool ok, const std::string &msg)
-{
-    if (!ok)
-        throw lss_error(LSS_ERR_INVALID, msg);
-}
-
-struct dev_buffer {
-    void *p = nullptr;
-    dev_buffer() = default;
-    dev_buffer(const dev_buffer &) = delete;
-    dev_buffer &operator=(const dev_buffer &) = delete;
-    ~dev_buffer()
-    {
-        if (p)
-            (void)hipFree(p);
-    }
-    void alloc(size_t bytes)
-    {
-        if (bytes == 0)
-            bytes = 256;
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) {
-            p = nullptr;
-            throw lss_error(LSS_ERR_ALLOC, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-        }
-    }
-};
 
 // Layout of one problem's device allocation, in elements of T (index maps appended as uint32).
 struct ProbLayout {
-    size_t off_a, off_b, off_y1, off_y2, off_y3, off_scr, off_colA0, off_colA1, off_coeff, off_bnd;
+    size_t off_a, off_b, off_y1, off_y2, off_y3, off_scr, off_colA0, off_colA1, off_coeff, off_bnd, off_src;
     size_t n_T;          // number of T elements
     size_t off_idx_bytes, bytes;
 };
@@ -150,6 +102,7 @@ struct lss_adi_plan {
     virtual void set_overlap(int mode) = 0;
     virtual void kernel_config(int32_t *cfg) const = 0;
     virtual void run(const void *const *u_in, void *const *u_out, hipStream_t stream, bool use_graph, bool host) = 0;
+    virtual void run_levels(const void *const *u_in, void *const *levels, hipStream_t stream, bool host) = 0;
     virtual void profile(const void *const *u_in, void *const *u_out, hipStream_t stream, double *ms, double *bytes,
                          int64_t *launches) = 0;
 };
@@ -285,6 +238,9 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         require(d.coeff_sets == 1 || d.coeff_sets == d.steps, "coeff_sets must be 1 or steps");
         require(d.bnd_sets == 1 || d.bnd_sets == d.steps, "bnd_sets must be 1 or steps");
         require(d.idx1 && d.idx2 && d.coeff && d.bnd, "null array in descriptor");
+        require(d.src_sets == 0 || d.src_sets == 1 || d.src_sets == d.steps, "src_sets must be 0, 1 or steps");
+        require((d.src_sets == 0) == (d.src == nullptr), "src must be null exactly when src_sets is 0");
+        require(d.src_sets == 0 || d.splitting != LSS_EXPLICIT_EULER, "explicit Euler plans take no heat source");
         for (int64_t i = 0; i < d.nx; ++i)
             require(d.idx1[i] < (uint32_t)d.nx, "idx1 out of range");
         for (int64_t j = 0; j < d.ny; ++j)
@@ -384,6 +340,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             L.off_colA1 = take((size_t)d.nx);
             L.off_coeff = take(coeff_stride(d) * (size_t)d.coeff_sets);
             L.off_bnd = take((size_t)(d.nx + 2 * d.ny) * (size_t)d.bnd_sets);
+            L.off_src = d.src_sets ? take(2 * nxy * (size_t)d.src_sets) : 0;
             L.n_T = o;
             L.off_idx_bytes = o * sizeof(T);
             L.bytes = L.off_idx_bytes + ((size_t)(d.nx + d.ny) * 4 + 255) / 256 * 256;
@@ -444,6 +401,12 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             }
             HIPCHK(hipMemcpy((void *)p.bnd, d.bnd, sizeof(T) * (size_t)(d.nx + 2 * d.ny) * (size_t)d.bnd_sets,
                              hipMemcpyHostToDevice));
+            if (d.src_sets) {
+                p.src = t + L.off_src;
+                p.src_sets = (int32_t)d.src_sets;
+                HIPCHK(hipMemcpy((void *)p.src, d.src, sizeof(T) * 2 * (size_t)(d.nx * d.ny) * (size_t)d.src_sets,
+                                 hipMemcpyHostToDevice));
+            }
             HIPCHK(hipMemcpy(ix, d.idx1, sizeof(uint32_t) * (size_t)d.nx, hipMemcpyHostToDevice));
             HIPCHK(hipMemcpy(ix + d.nx, d.idx2, sizeof(uint32_t) * (size_t)d.ny, hipMemcpyHostToDevice));
             nx[order[q]] = d.nx;
@@ -1514,6 +1477,91 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             HIPCHK(hipStreamSynchronize(stream));
     }
 
+    // Every time level: levels[p] receives steps + 1 surfaces of nx * ny values, row s = the surface after s
+    // steps (row 0 = u_in[p]).  One batched copy launch per step moves the new level of every problem still
+    // stepping; the job table (sum over problems of steps + 1 jobs) is rebuilt when the pointers change.
+    dev_buffer lv_jobs_dev;
+    std::vector<const void *> lv_last;
+    std::vector<size_t> lv_first;  // first job of each step s = 0..max_steps (s = 0: the inputs)
+    void level_jobs(const void *const *u_in, void *const *levels, hipStream_t stream)
+    {
+        std::vector<const void *> key(u_in, u_in + count);
+        key.insert(key.end(), levels, levels + count);
+        if (key == lv_last)
+            return;
+        std::vector<CopyJob> jobs;
+        lv_first.assign(max_steps + 2, 0);
+        for (int s = 0; s <= max_steps; ++s) {
+            lv_first[s] = jobs.size();
+            const int64_t na = s == 0 ? count : active(s - 1);
+            for (int64_t q = 0; q < na; ++q) {
+                const Prob<T> &p = probs[q];
+                const size_t nxy = (size_t)p.nx * p.ny;
+                T *dst = (T *)levels[order[q]] + (size_t)s * nxy;
+                // step s - 1 wrote ((s - 1) & 1 ? a : b)
+                const T *src = s == 0 ? (const T *)u_in[order[q]] : (((s - 1) & 1) ? p.a : p.b);
+                jobs.push_back({src, dst, (int64_t)nxy});
+            }
+        }
+        lv_first[max_steps + 1] = jobs.size();
+        HIPCHK(hipStreamSynchronize(stream));  // an earlier run may still read the old table
+        lv_jobs_dev.~dev_buffer();
+        new (&lv_jobs_dev) dev_buffer();
+        lv_jobs_dev.alloc(jobs.size() * sizeof(CopyJob));
+        HIPCHK(hipMemcpy(lv_jobs_dev.p, jobs.data(), jobs.size() * sizeof(CopyJob), hipMemcpyHostToDevice));
+        lv_last = std::move(key);
+    }
+
+    void run_levels(const void *const *u_in, void *const *levels, hipStream_t stream, bool host) override
+    {
+        require(u_in && levels, "null pointer array");
+        for (int64_t c = 0; c < count; ++c)
+            require(u_in[c] && levels[c], "null surface pointer");
+        if (!tuned)
+            autotune(stream);
+        if (host) {  // stage inputs and levels through one device allocation
+            std::vector<size_t> off(count);
+            size_t tot = 0;
+            for (int64_t c = 0; c < count; ++c) {
+                off[c] = tot;
+                tot += (size_t)nx[c] * ny[c] * (size_t)(steps[c] + 2);
+            }
+            dev_buffer stage;
+            stage.alloc(tot * sizeof(T));
+            std::vector<const void *> di(count);
+            std::vector<void *> dl(count);
+            for (int64_t c = 0; c < count; ++c) {
+                const size_t nxy = (size_t)nx[c] * ny[c];
+                T *base = (T *)stage.p + off[c];
+                HIPCHK(hipMemcpy(base, u_in[c], nxy * sizeof(T), hipMemcpyHostToDevice));
+                di[c] = base;
+                dl[c] = base + nxy;
+            }
+            run_levels(di.data(), dl.data(), nullptr, false);
+            HIPCHK(hipDeviceSynchronize());
+            lv_last.clear();  // the staging buffer goes away
+            for (int64_t c = 0; c < count; ++c) {
+                const size_t nxy = (size_t)nx[c] * ny[c];
+                HIPCHK(hipMemcpy(levels[c], (T *)stage.p + off[c] + nxy, nxy * (size_t)(steps[c] + 1) * sizeof(T),
+                                 hipMemcpyDeviceToHost));
+            }
+            return;
+        }
+        level_jobs(u_in, levels, stream);
+        const CopyJob *jd = (const CopyJob *)lv_jobs_dev.p;
+        for (int64_t q = 0; q < count; ++q) {
+            const Prob<T> &p = probs[q];
+            HIPCHK(hipMemcpyAsync(p.a, u_in[order[q]], sizeof(T) * (size_t)p.nx * p.ny, hipMemcpyDeviceToDevice,
+                                  stream));
+        }
+        launch_jobs(jd, (int)(lv_first[1] - lv_first[0]), stream);
+        for (int s = 0; s < max_steps; ++s) {
+            launch_step(s, stream);
+            launch_jobs(jd + lv_first[s + 1], (int)(lv_first[s + 2] - lv_first[s + 1]), stream);
+        }
+        HIPCHK(hipGetLastError());
+    }
+
     // copy jobs of the surfaces of a device run (count inputs, then count outputs), rebuilt when the
     // caller's pointers change
     static constexpr int64_t IO_BATCH_MIN = 8;
@@ -1986,7 +2034,7 @@ void tridiag_run(int32_t fp, int32_t layout, int64_t B, int64_t n, const void *a
 
 extern "C" {
 
-const char *lss_last_error(void) { return g_last_error.c_str(); }
+const char *lss_last_error(void) { return last_error().c_str(); }
 
 int32_t lss_version(void) { return 1; }
 
@@ -2026,6 +2074,14 @@ int lss_adi_run_host(lss_adi_plan *plan, const void *const *u_in, void *const *u
     return guarded([&] {
         require(plan != nullptr, "null plan");
         plan->run(u_in, u_out, nullptr, false, true);
+    });
+}
+
+int lss_adi_run_levels(lss_adi_plan *plan, const void *const *u_in, void *const *levels, void *stream, int32_t host)
+{
+    return guarded([&] {
+        require(plan != nullptr, "null plan");
+        plan->run_levels(u_in, levels, (hipStream_t)stream, host != 0);
     });
 }
 
@@ -2171,6 +2227,70 @@ int lss_adi_backward(lss_adi_plan *plan, const void *const *lam_out, void *const
     return guarded([&] {
         require(plan != nullptr, "null plan");
         plan->backward(lam_out, lam_in, dparams, (hipStream_t)stream);
+    });
+}
+
+// host-pointer forms of the adjoint pair: the surfaces are staged through device memory; the plan keeps
+// its checkpoints between the two calls as with the device forms
+int lss_adi_forward_host(lss_adi_plan *plan, const void *const *u_in, void *const *u_out, int32_t ckpt_every)
+{
+    return guarded([&] {
+        require(plan && u_in && u_out, "null pointer");
+        const size_t es = plan->fp == LSS_FP64 ? 8 : 4;
+        std::vector<size_t> off(plan->count);
+        size_t tot = 0;
+        for (int64_t c = 0; c < plan->count; ++c) {
+            require(u_in[c] && u_out[c], "null surface pointer");
+            off[c] = tot;
+            tot += 2 * (size_t)(plan->nx[c] * plan->ny[c]) * es;
+        }
+        dev_buffer stage;
+        stage.alloc(tot);
+        std::vector<const void *> di(plan->count);
+        std::vector<void *> dout(plan->count);
+        for (int64_t c = 0; c < plan->count; ++c) {
+            const size_t bytes = (size_t)(plan->nx[c] * plan->ny[c]) * es;
+            char *base = (char *)stage.p + off[c];
+            HIPCHK(hipMemcpy(base, u_in[c], bytes, hipMemcpyHostToDevice));
+            di[c] = base;
+            dout[c] = base + bytes;
+        }
+        plan->forward_ckpt(di.data(), dout.data(), ckpt_every, nullptr);
+        HIPCHK(hipDeviceSynchronize());
+        for (int64_t c = 0; c < plan->count; ++c)
+            HIPCHK(hipMemcpy(u_out[c], dout[c], (size_t)(plan->nx[c] * plan->ny[c]) * es, hipMemcpyDeviceToHost));
+    });
+}
+
+int lss_adi_backward_host(lss_adi_plan *plan, const void *const *lam_out, void *const *lam_in, double *dparams)
+{
+    return guarded([&] {
+        require(plan && lam_out && dparams, "null pointer");
+        const size_t es = plan->fp == LSS_FP64 ? 8 : 4;
+        std::vector<size_t> off(plan->count);
+        size_t tot = 0;
+        for (int64_t c = 0; c < plan->count; ++c) {
+            require(lam_out[c] != nullptr, "null adjoint seed");
+            off[c] = tot;
+            tot += 2 * (size_t)(plan->nx[c] * plan->ny[c]) * es;
+        }
+        dev_buffer stage;
+        stage.alloc(tot);
+        std::vector<const void *> dl(plan->count);
+        std::vector<void *> dg(plan->count, nullptr);
+        for (int64_t c = 0; c < plan->count; ++c) {
+            const size_t bytes = (size_t)(plan->nx[c] * plan->ny[c]) * es;
+            char *base = (char *)stage.p + off[c];
+            HIPCHK(hipMemcpy(base, lam_out[c], bytes, hipMemcpyHostToDevice));
+            dl[c] = base;
+            if (lam_in && lam_in[c])
+                dg[c] = base + bytes;
+        }
+        plan->backward(dl.data(), dg.data(), dparams, nullptr);
+        HIPCHK(hipDeviceSynchronize());
+        for (int64_t c = 0; c < plan->count; ++c)
+            if (dg[c])
+                HIPCHK(hipMemcpy(lam_in[c], dg[c], (size_t)(plan->nx[c] * plan->ny[c]) * es, hipMemcpyDeviceToHost));
     });
 }
 

@@ -216,6 +216,16 @@ __device__ __forceinline__ int chunk_len(int N, int nch = NCH)
     return c;
 }
 
+// the equation of the t-th point (unknown k) of the calling thread's chunk: eq(t, k) when eq takes the
+// position too (t is a compile-time index after unrolling, so eq may read per-point registers), else eq(k)
+template <typename EqF> __device__ __forceinline__ auto eq_at(EqF &eq, int t, int k)
+{
+    if constexpr (std::is_invocable_v<EqF &, int, int>)
+        return eq(t, k);
+    else
+        return eq(k);
+}
+
 // f(std::integral_constant<int, C>) for the chunk length c (a power of two <= M; uniform)
 template <int M, typename F> __device__ __forceinline__ void with_chunk_len(int c, F &&f)
 {
@@ -231,64 +241,69 @@ template <int M, typename F> __device__ __forceinline__ void with_chunk_len(int 
 // One line per NL lanes (NL = 64: one line per wavefront; 32: two), lane = chunk (its index within
 // the line's lanes); the interface system is reduced with shuffles inside each group of NL lanes.
 // Every lane of the wavefront must call it, and the lines of a wavefront must have the same N;
-// chunk_len(N, NL) <= M.
-template <typename T, int M, bool PUT_ALL = false, int NL = NCH, typename EqF, typename PutF>
-__device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put)
+// wave_solve_c takes the chunk length C = chunk_len(N, NL) as a template argument.
+template <typename T, int C, bool PUT_ALL = false, int NL = NCH, typename EqF, typename PutF>
+__device__ __forceinline__ void wave_solve_c(int N, int lane, EqF &&eq, PutF &&put)
 {
-    with_chunk_len<M>(chunk_len(N, NL), [&](auto cc) {
-        constexpr int C = decltype(cc)::value;
-        const int s = 1 + lane * C;
-        Chunk<T, C> ch;
-        ch.cnt = max(0, min(C, N - s + 1));
-        const bool last_real = ch.cnt > 0 && s + ch.cnt - 1 == N;
+    const int s = 1 + lane * C;
+    Chunk<T, C> ch;
+    ch.cnt = max(0, min(C, N - s + 1));
+    const bool last_real = ch.cnt > 0 && s + ch.cnt - 1 == N;
+    if (ch.cnt > 0) {
+#pragma unroll
+        for (int t = 0; t < C; ++t)
+            ch.put(t, eq_at(eq, t, s + t));
+        ch.bwd();
+    }
+    T PF, QF, RF;
+    ch.first_rel(PF, QF, RF);
+    T nPF = __shfl_down(PF, 1, NL), nQF = __shfl_down(QF, 1, NL), nRF = __shfl_down(RF, 1, NL);
+    if (last_real || lane == NL - 1) {
+        nPF = (T)0;
+        nQF = (T)0;
+        nRF = (T)0;
+    }
+    Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
+#pragma unroll
+    for (int d = 1; d < NL; d <<= 1) {
+        const NEq<T> m = normalized(e);
+        NEq<T> l, h;
+        l.a = __shfl_up(m.a, d, NL);
+        l.c = __shfl_up(m.c, d, NL);
+        l.r = __shfl_up(m.r, d, NL);
+        h.a = __shfl_down(m.a, d, NL);
+        h.c = __shfl_down(m.c, d, NL);
+        h.r = __shfl_down(m.r, d, NL);
+        if (lane < d)
+            l = zero_neq<T>();
+        if (lane + d >= NL)
+            h = zero_neq<T>();
+        e = pcr_step(e, l, h);
+    }
+    const T xi = e.r * fast_rcp(e.b);
+    T xl = __shfl_up(xi, 1, NL);
+    if (lane == 0)
+        xl = (T)0;
+    if constexpr (PUT_ALL) {
         if (ch.cnt > 0) {
 #pragma unroll
             for (int t = 0; t < C; ++t)
-                ch.put(t, eq(s + t));
-            ch.bwd();
+                put(t, s + t, ch.value(t, xl, xi));
         }
-        T PF, QF, RF;
-        ch.first_rel(PF, QF, RF);
-        T nPF = __shfl_down(PF, 1, NL), nQF = __shfl_down(QF, 1, NL), nRF = __shfl_down(RF, 1, NL);
-        if (last_real || lane == NL - 1) {
-            nPF = (T)0;
-            nQF = (T)0;
-            nRF = (T)0;
-        }
-        Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
+    } else {
 #pragma unroll
-        for (int d = 1; d < NL; d <<= 1) {
-            const NEq<T> m = normalized(e);
-            NEq<T> l, h;
-            l.a = __shfl_up(m.a, d, NL);
-            l.c = __shfl_up(m.c, d, NL);
-            l.r = __shfl_up(m.r, d, NL);
-            h.a = __shfl_down(m.a, d, NL);
-            h.c = __shfl_down(m.c, d, NL);
-            h.r = __shfl_down(m.r, d, NL);
-            if (lane < d)
-                l = zero_neq<T>();
-            if (lane + d >= NL)
-                h = zero_neq<T>();
-            e = pcr_step(e, l, h);
-        }
-        const T xi = e.r * fast_rcp(e.b);
-        T xl = __shfl_up(xi, 1, NL);
-        if (lane == 0)
-            xl = (T)0;
-        if constexpr (PUT_ALL) {
-            if (ch.cnt > 0) {
-#pragma unroll
-                for (int t = 0; t < C; ++t)
-                    put(t, s + t, ch.value(t, xl, xi));
-            }
-        } else {
-#pragma unroll
-            for (int t = 0; t < C; ++t)
-                if (t < ch.cnt)
-                    put(t, s + t, ch.value(t, xl, xi));
-        }
-    });
+        for (int t = 0; t < C; ++t)
+            if (t < ch.cnt)
+                put(t, s + t, ch.value(t, xl, xi));
+    }
+}
+
+// the same for the chunk length of N (chunk_len(N, NL) <= M)
+template <typename T, int M, bool PUT_ALL = false, int NL = NCH, typename EqF, typename PutF>
+__device__ __forceinline__ void wave_solve(int N, int lane, EqF &&eq, PutF &&put)
+{
+    with_chunk_len<M>(chunk_len(N, NL),
+                      [&](auto cc) { wave_solve_c<T, decltype(cc)::value, PUT_ALL, NL>(N, lane, eq, put); });
 }
 
 // W lines per workgroup of W * NC threads (thread = (chunk chn, line col)), NC chunks per line; the
@@ -308,7 +323,7 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NC][W], int N, int col, i
         if (ch.cnt > 0) {
 #pragma unroll
             for (int t = 0; t < C; ++t)
-                ch.put(t, eq(s + t));
+                ch.put(t, eq_at(eq, t, s + t));
             ch.bwd();
         }
         {

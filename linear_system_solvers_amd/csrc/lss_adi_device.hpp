@@ -89,7 +89,17 @@ template <typename T> struct Prob {
     int32_t fcode[24];
     const T *bnd;              // bnd_sets sets of bnd_stride values: [v_upper(nx) | h_lower(ny) | h_upper*X'(1) (ny)]
     int64_t coeff_stride, bnd_stride;
+    // heat source: null, or src_sets (1 or steps) sets of [F(t) | F(t_next)] (nx*ny each, row-major)
+    const T *src;
+    int32_t src_sets, pad_;
 };
+
+// the heat-source set of loop step `step` (null without a source)
+template <typename T> __device__ __forceinline__ const T *src_set(const Prob<T> &p, int step)
+{
+    return p.src_sets == 0 ? nullptr
+                           : p.src + (size_t)(p.src_sets == 1 ? 0 : step) * 2 * (size_t)p.nx * (size_t)p.ny;
+}
 
 // Workgroups are dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8), and every XCD has
 // its own L2.  xcd_tile maps block b so that each XCD takes a contiguous run of the g tiles:

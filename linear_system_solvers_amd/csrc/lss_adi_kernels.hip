@@ -152,15 +152,20 @@ __global__ void __launch_bounds__(TILE) k_xsweep(const Prob<T> *__restrict__ pro
     const uint32_t jj = p.idx2[j];
     const T lo = u[jj], hi = u[(int64_t)(nx - 1) * ny + jj];
     auto at = [&](gptr<const T> b, int64_t q) { return PAt<gptr<const T>>{b + q, ny}; };
+    const T *sp = kind == 0 ? src_set(p, step) : nullptr;
+    const int64_t nxy = (int64_t)nx * ny;
     T gprev = (T)0, sprev = (T)0;
     for (int t = 1; t <= end; ++t) {
         const int64_t q = (int64_t)t * ny + j;
         const Stencil<T> s = stencil_at(p, cs, t, j);
         const T low = (-theta * s.M), diag = (one + theta * s.Z), high = (-theta * s.P);
         T r;
-        if (kind == 0)
+        if (kind == 0) {
             r = rhs_intermed_1(p, s, at(in, q));
-        else if (kind == 1)
+            // rhs_intermed_1_source (lss_heat_douglas_rachford_method.hpp:70-104)
+            if (sp)
+                r = r + (theta * p.rho * sp[nxy + q]) + ((one - theta) * p.rho * sp[q]);
+        } else if (kind == 1)
             r = rhs_intermed_3<T, false>(p, s, at(in, q), at(inh, q), at(inhn, q));
         else if (kind == 2)
             r = rhs_intermed_3<T, true>(p, s, at(in, q), at(inh, q), at(inhn, q));

@@ -135,12 +135,13 @@ template <typename T, int KIND, int PAT> struct XLine {
     GArr<T> cs;
     LArr<T> xl;  // the X-block factor vectors in LDS (compile-time patterns)
     gptr<T> out;
-    int N, ny;
+    int N, ny, step_;
     uint32_t j;
     T lo, hi;
 
     __device__ __forceinline__ XLine(const Prob<T> &p_, int step, int j_, lptr<const T> xsh) : p(p_)
     {
+        step_ = step;
         xl = LArr<T>{xsh};
         ny = p.ny;
         N = p.nx - 2;
@@ -172,6 +173,13 @@ template <typename T, int KIND, int PAT> struct XLine {
             y2q{y2, k, j, (uint32_t)ny, kmax};
         if constexpr (KIND == 0) {
             e.r = rhs_intermed_1(p, st, uq);
+            // rhs_intermed_1_source (lss_heat_douglas_rachford_method.hpp:70-104), a uniform branch
+            if (p.src_sets) {
+                const gptr<const T> sc = G(src_set(p, step_));
+                const uint32_t q = min((uint32_t)k, kmax) * (uint32_t)ny + j,
+                               nxy = (uint32_t)(N + 2) * (uint32_t)ny;
+                e.r = e.r + (theta * p.rho * ldg(sc, nxy + q)) + ((one - theta) * p.rho * ldg(sc, q));
+            }
         } else if constexpr (KIND == 1) {
             e.r = rhs_intermed_3<T, false>(p, st, uq, y1q, y2q);
         } else if constexpr (KIND == 2) {

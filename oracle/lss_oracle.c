@@ -111,6 +111,7 @@ void lss_or_heston_pde(const lss_or_heston *p, lss_or_pde *out)
     out->h_lower = bnd_low;
     out->h_upper = bnd_high;
     out->v_upper = bnd_identity;
+    out->src = NULL;
     out->ctx = (void *)p;
 }
 
@@ -168,6 +169,7 @@ void lss_or_sabr_pde(const lss_or_sabr *p, lss_or_pde *out)
     out->h_lower = sabr_low;
     out->h_upper = sabr_high;
     out->v_upper = bnd_identity;
+    out->src = NULL;
     out->ctx = (void *)p;
 }
 

@@ -65,6 +65,13 @@ typedef struct lss_or_pde {
     lss_or_bnd_fn h_upper;   /* Neumann at x = x_hi, value(t, y)     */
     lss_or_bnd_fn v_upper;   /* Dirichlet at y = y_hi, value(t, x)   */
     void *ctx;
+    /* heat source F(t, x, y), or NULL.  With a source, the first stage of every scheme adds
+     * theta rho F(t_next) + (1 - theta) rho F(t) (rhs_intermed_1_source,
+     * lss_heat_douglas_rachford_method.hpp:70-104; the same in the CS and MCS files), with t_next the
+     * loop's next time (t + k Forward, t - k Backward).  The reference declares this variant but leaves
+     * it unwired (its solve(..., heat_source, ...) bodies are empty and implicit_kernel.hpp:1032-1044
+     * skips the loop); "parity unpinned": it is pinned by a manufactured solution instead. */
+    lss_or_coef_fn src;
 } lss_or_pde;
 
 typedef struct lss_or_cfg {

@@ -33,7 +33,7 @@ BND_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double, C.c_double)
 class _Pde(C.Structure):
     _fields_ = [("a", COEF_FN), ("b", COEF_FN), ("c", COEF_FN), ("d", COEF_FN), ("e", COEF_FN),
                 ("f", COEF_FN), ("init", INIT_FN), ("h_lower", BND_FN), ("h_upper", BND_FN),
-                ("v_upper", BND_FN), ("ctx", C.c_void_p)]
+                ("v_upper", BND_FN), ("ctx", C.c_void_p), ("src", COEF_FN)]
 
 
 class _Cfg(C.Structure):
@@ -176,13 +176,16 @@ def sabr_pde(p: Sabr) -> Pde:
 
 
 def custom_pde(a: Callable, b: Callable, c: Callable, d: Callable, e: Callable, f: Callable,
-               init: Callable, h_lower: Callable, h_upper: Callable, v_upper: Callable) -> Pde:
-    """PDE data from Python callables (coefficients take (t, x, y), init (x, y), boundaries (t, s))."""
+               init: Callable, h_lower: Callable, h_upper: Callable, v_upper: Callable,
+               src: Optional[Callable] = None) -> Pde:
+    """PDE data from Python callables (coefficients and the optional heat source take (t, x, y), init
+    (x, y), boundaries (t, s))."""
     fns = [COEF_FN(lambda _c, t, x, y, g=g: float(g(t, x, y))) for g in (a, b, c, d, e, f)]
     fi = INIT_FN(lambda _c, x, y: float(init(x, y)))
     fb = [BND_FN(lambda _c, t, s, g=g: float(g(t, s))) for g in (h_lower, h_upper, v_upper)]
-    pde = _Pde(*fns, fi, *fb, None)
-    return Pde(pde, (fns, fi, fb))
+    fs = COEF_FN(lambda _c, t, x, y: float(src(t, x, y))) if src is not None else COEF_FN()
+    pde = _Pde(*fns, fi, *fb, None, fs)
+    return Pde(pde, (fns, fi, fb, fs))
 
 
 def solve(pde: Pde, cfg: Cfg, fp: str = "f64") -> np.ndarray:
