@@ -11,6 +11,14 @@
 // 64-unknown interface system is solved by parallel cyclic reduction (wave shuffles or LDS; each step
 // exchanges the neighbours' rows divided by their diagonals), after which every chunk finishes its
 // points.
+//
+// Every chunk is full: a line whose N is not a multiple of the chunk length is padded with points
+// past its end, whose equations the caller supplies (finite, with c = 0 and b != 0; see wave_solve_c).
+// The last real point has c = 0 (its end is folded), so nothing the padding holds reaches a real
+// point: its spike coefficient g = c / b' is exactly zero, which zeroes every product with padded
+// values in the back substitution and every coupling of a real interface row to a padded one.  With
+// full chunks the point count of a chunk is a compile-time constant, and the elimination needs no
+// per-point selects.
 #pragma once
 
 #include "lss_adi_device.hpp"
@@ -45,20 +53,29 @@ __device__ __forceinline__ float fast_rcp(float x)
     return __builtin_fmaf(r, e, r);
 }
 
-// Register-resident chunk elimination.  After fwd()/bwd(): g holds r_k, zp holds p_k, zq holds q_k
-// for the cnt - 1 interior points, and (ae, be, ce, re) is the equation of the chunk's last point.
+// Register-resident elimination of one chunk of M points (M - 1 interior points and the interface
+// point M - 1).  After put() of every point and bwd(): g, zp, zq hold the relation
+// x_t = zp[t] + zq[t] xi_{l-1} + g[t] xi_l of each interior point t, and (ae, be, ce, re) is the
+// equation of the interface point.
 template <typename T, int M> struct Chunk {
     T g[M], zp[M], zq[M];
     T ae, be, ce, re;
-    int cnt;
+    T pl, ql, rl;  // relation of the last interior point (xi_{l-1} itself when M = 1)
 
-    // Forward elimination of point t (a compile-time index after unrolling).  The array entries are
-    // written unconditionally, so no two stores share an address computed at run time (which would
-    // move the arrays to scratch memory); the entries of the chunk's last point are never read.
+    // Forward elimination of point t (a compile-time index after unrolling; every array entry is
+    // addressed with a compile-time index, since a run-time one would move the arrays to scratch
+    // memory).  x_t = zp[t] + zq[t] xi_{l-1} - g[t] x_{t+1} afterwards.
     __device__ __forceinline__ void put(int t, const Eq<T> &q)
     {
+        if (t == M - 1) {
+            ae = q.a;
+            be = q.b;
+            ce = q.c;
+            re = q.r;
+            return;
+        }
 #pragma unroll
-        for (int u = 0; u < M; ++u) {
+        for (int u = 0; u < M - 1; ++u) {
             if (u == t) {
                 const T rb = fast_rcp(u == 0 ? q.b : q.b - q.a * g[u > 0 ? u - 1 : 0]);
                 g[u] = q.c * rb;
@@ -66,50 +83,46 @@ template <typename T, int M> struct Chunk {
                 zq[u] = (u == 0 ? -q.a : -q.a * zq[u > 0 ? u - 1 : 0]) * rb;
             }
         }
-        if (t == cnt - 1) {
-            ae = q.a;
-            be = q.b;
-            ce = q.c;
-            re = q.r;
-        }
     }
 
-    // Backward substitution of the spikes.  Every array element is addressed with a compile-time
-    // index (a runtime index would move the arrays to scratch memory); (pl, ql, rl) keep the
-    // relation of the last interior point, whose position depends on cnt.
-    T pl, ql, rl;
-
+    // Backward substitution of the spikes (x_{M-1} = xi_l).
     __device__ __forceinline__ void bwd()
     {
-        const int ni = cnt - 1;
-        pl = (T)0;
-        ql = (T)1;
-        rl = (T)0;
+        if constexpr (M >= 2) {
+            g[M - 2] = -g[M - 2];
 #pragma unroll
-        for (int u = M - 1; u >= 0; --u) {
-            const bool valid = u < ni, lastp = u == ni - 1;
-            const T gu = g[u];
-            T gn = -gu, pn = zp[u], qn = zq[u];
-            if (u + 1 < M) {
-                const T g1 = g[u + 1], p1 = zp[u + 1], q1 = zq[u + 1];
-                gn = lastp ? -gu : -gu * g1;
-                pn = lastp ? zp[u] : zp[u] - gu * p1;
-                qn = lastp ? zq[u] : zq[u] - gu * q1;
+            for (int u = M - 3; u >= 0; --u) {
+                const T gu = g[u];
+                g[u] = -gu * g[u + 1];
+                zp[u] = zp[u] - gu * zp[u + 1];
+                zq[u] = zq[u] - gu * zq[u + 1];
             }
-            g[u] = valid ? gn : gu;
-            zp[u] = valid ? pn : zp[u];
-            zq[u] = valid ? qn : zq[u];
-            pl = lastp ? pn : pl;
-            ql = lastp ? qn : ql;
-            rl = lastp ? gn : rl;
+            pl = zp[M - 2];
+            ql = zq[M - 2];
+            rl = g[M - 2];
+        } else {
+            pl = (T)0;
+            ql = (T)1;
+            rl = (T)0;
         }
     }
 
-    // (p, q, r) of the first and of the last interior point; a chunk of one point has none:
-    // then x_{e-1} = xi_{l-1} (0, 1, 0) and, seen from the left neighbour, x_s = xi_l (0, 0, 1).
+    // a chunk that holds no line: its interface row is the identity and its relations are zero
+    __device__ __forceinline__ void set_identity()
+    {
+#pragma unroll
+        for (int u = 0; u < M; ++u)
+            g[u] = zp[u] = zq[u] = (T)0;
+        ae = ce = re = (T)0;
+        be = (T)1;
+        pl = ql = rl = (T)0;
+    }
+
+    // (p, q, r) of the first interior point; a chunk of one point has none: seen from the left
+    // neighbour, its first point is xi_l itself (0, 0, 1).
     __device__ __forceinline__ void first_rel(T &P, T &Q, T &R) const
     {
-        if (cnt >= 2) {
+        if constexpr (M >= 2) {
             P = zp[0];
             Q = zq[0];
             R = g[0];
@@ -119,23 +132,15 @@ template <typename T, int M> struct Chunk {
             R = (T)1;
         }
     }
-    __device__ __forceinline__ void last_rel(T &P, T &Q, T &R) const
-    {
-        P = pl;
-        Q = ql;
-        R = rl;
-    }
 
     // interface equation of chunk l, given the next chunk's first relation (PF, QF, RF)
     __device__ __forceinline__ Eq<T> interface_eq(T PF, T QF, T RF) const
     {
-        T PL, QL, RL;
-        last_rel(PL, QL, RL);
         Eq<T> q;
-        q.a = ae * QL;
-        q.b = ae * RL + be + ce * QF;
+        q.a = ae * ql;
+        q.b = ae * rl + be + ce * QF;
         q.c = ce * RF;
-        q.r = re - ae * PL - ce * PF;
+        q.r = re - ae * pl - ce * PF;
         return q;
     }
 
@@ -143,8 +148,8 @@ template <typename T, int M> struct Chunk {
     {
         T v = xr;
 #pragma unroll
-        for (int u = 0; u < M; ++u)
-            if (u == t && u < cnt - 1)
+        for (int u = 0; u < M - 1; ++u)
+            if (u == t)
                 v = zp[u] + zq[u] * xl + g[u] * xr;
         return v;
     }
@@ -195,18 +200,18 @@ template <typename T> __device__ __forceinline__ Eq<T> identity_eq()
 
 // -------------------------------------------------------------------------------------------------
 // Generic line solves.  A line of N interior unknowns is cut into chunks of C = chunk_len(N) points
-// (the smallest power of two with NCH * C >= N, so the partition depends on N alone); the last
-// non-empty chunk may be partial and the chunks after it are empty.  Each thread with a non-empty
-// chunk evaluates eq for all C of its points without a per-point branch, so the loads of all its
-// points are issued together.
+// (the smallest power of two with NCH * C >= N, so the partition depends on N alone); the chunks are
+// padded past the line's end to C points each.  Each thread evaluates eq for all C points of its
+// chunk without a per-point branch, so the loads of all its points are issued together.
 //   eq(k) returns the equation of interior unknown k (1..N) with the ends already folded (a = 0 at
-//   k = 1, c = 0 at k = N).  It is also called with N < k <= NCH * C (points past the end of the
-//   last chunk, whose equations are discarded): it must then read only in-bounds data.
+//   k = 1, c = 0 at k = N).  It is also called with N < k <= NCH * C (the padding): it must then read
+//   only in-bounds data and return a finite equation with c = 0 and b != 0 (folding the upper end
+//   at every k >= N gives this, as does repeating the folded row N).
 //   put(t, k, x) receives the solution of unknown k (k <= N), the t-th point of the calling
 //   thread's chunk (t is a compile-time index after unrolling, so a caller may keep the values in
-//   registers and store them later).  With PUT_ALL, put is called for all C points of a non-empty
-//   chunk without per-point branches (so that the caller's loads can be issued together); then
-//   k > N marks a discarded point whose x is arbitrary (possibly not finite).
+//   registers and store them later).  With PUT_ALL, put is called for all C points without
+//   per-point branches (so that the caller's loads can be issued together); then k > N marks a
+//   padded point whose x is finite but meaningless.
 // -------------------------------------------------------------------------------------------------
 __device__ __forceinline__ int chunk_len(int N, int nch = NCH)
 {
@@ -247,23 +252,20 @@ __device__ __forceinline__ void wave_solve_c(int N, int lane, EqF &&eq, PutF &&p
 {
     const int s = 1 + lane * C;
     Chunk<T, C> ch;
-    ch.cnt = max(0, min(C, N - s + 1));
-    const bool last_real = ch.cnt > 0 && s + ch.cnt - 1 == N;
-    if (ch.cnt > 0) {
 #pragma unroll
-        for (int t = 0; t < C; ++t)
-            ch.put(t, eq_at(eq, t, s + t));
-        ch.bwd();
-    }
+    for (int t = 0; t < C; ++t)
+        ch.put(t, eq_at(eq, t, s + t));
+    ch.bwd();
     T PF, QF, RF;
     ch.first_rel(PF, QF, RF);
-    T nPF = __shfl_down(PF, 1, NL), nQF = __shfl_down(QF, 1, NL), nRF = __shfl_down(RF, 1, NL);
-    if (last_real || lane == NL - 1) {
-        nPF = (T)0;
-        nQF = (T)0;
-        nRF = (T)0;
-    }
-    Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
+    // (the last lane takes its own relation: its interface row has c = 0, as every row at k >= N)
+    const T nPF = __shfl_down(PF, 1, NL), nQF = __shfl_down(QF, 1, NL), nRF = __shfl_down(RF, 1, NL);
+    Eq<T> e = ch.interface_eq(nPF, nQF, nRF);
+    // A lane without a neighbour at distance d takes its own row from the shuffle instead of zeros:
+    // its coupling to that side is already exactly zero (a = 0 on the first d lanes, c = 0 on the
+    // last d: the line's ends are folded, the padding past the end holds rows with c = 0, and a step
+    // keeps the zeros, n.a = -a * lo.a, n.c = -c * hi.c), so the finite row it meets is multiplied
+    // by zero and every result is the one a zero row gives.
 #pragma unroll
     for (int d = 1; d < NL; d <<= 1) {
         const NEq<T> m = normalized(e);
@@ -274,26 +276,19 @@ __device__ __forceinline__ void wave_solve_c(int N, int lane, EqF &&eq, PutF &&p
         h.a = __shfl_down(m.a, d, NL);
         h.c = __shfl_down(m.c, d, NL);
         h.r = __shfl_down(m.r, d, NL);
-        if (lane < d)
-            l = zero_neq<T>();
-        if (lane + d >= NL)
-            h = zero_neq<T>();
         e = pcr_step(e, l, h);
     }
     const T xi = e.r * fast_rcp(e.b);
-    T xl = __shfl_up(xi, 1, NL);
-    if (lane == 0)
-        xl = (T)0;
+    // (lane 0 takes its own xi: its relations' xi_{l-1} coefficients are zero, a = 0 at k = 1)
+    const T xl = __shfl_up(xi, 1, NL);
     if constexpr (PUT_ALL) {
-        if (ch.cnt > 0) {
 #pragma unroll
-            for (int t = 0; t < C; ++t)
-                put(t, s + t, ch.value(t, xl, xi));
-        }
+        for (int t = 0; t < C; ++t)
+            put(t, s + t, ch.value(t, xl, xi));
     } else {
 #pragma unroll
         for (int t = 0; t < C; ++t)
-            if (t < ch.cnt)
+            if (s + t <= N)
                 put(t, s + t, ch.value(t, xl, xi));
     }
 }
@@ -318,13 +313,13 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NC][W], int N, int col, i
         constexpr int C = decltype(cc)::value;
         const int s = 1 + chn * C;
         Chunk<T, C> ch;
-        ch.cnt = active ? max(0, min(C, N - s + 1)) : 0;
-        const bool last_real = ch.cnt > 0 && s + ch.cnt - 1 == N;
-        if (ch.cnt > 0) {
+        if (active) {
 #pragma unroll
             for (int t = 0; t < C; ++t)
                 ch.put(t, eq_at(eq, t, s + t));
             ch.bwd();
+        } else {
+            ch.set_identity();
         }
         {
             T PF, QF, RF;
@@ -334,13 +329,10 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NC][W], int N, int col, i
             sh[1][2][chn][col] = RF;
         }
         __syncthreads();
-        T nPF = (T)0, nQF = (T)0, nRF = (T)0;
-        if (!last_real && chn + 1 < NC) {
-            nPF = sh[1][0][chn + 1][col];
-            nQF = sh[1][1][chn + 1][col];
-            nRF = sh[1][2][chn + 1][col];
-        }
-        Eq<T> e = ch.cnt > 0 ? ch.interface_eq(nPF, nQF, nRF) : identity_eq<T>();
+        // (the last chunk takes its own relation: its interface row has c = 0)
+        const int cn = chn + 1 < NC ? chn + 1 : chn;
+        const T nPF = sh[1][0][cn][col], nQF = sh[1][1][cn][col], nRF = sh[1][2][cn][col];
+        Eq<T> e = ch.interface_eq(nPF, nQF, nRF);
         int buf = 0;
         {
             const NEq<T> m = normalized(e);
@@ -349,19 +341,18 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NC][W], int N, int col, i
             sh[0][2][chn][col] = m.r;
         }
         __syncthreads();
+        // (a chunk without a neighbour at distance d reads its own row: its coupling to that side
+        // is exactly zero, as in wave_solve_c)
 #pragma unroll
         for (int d = 1; d < NC; d <<= 1) {
-            NEq<T> l = zero_neq<T>(), h = zero_neq<T>();
-            if (chn >= d) {
-                l.a = sh[buf][0][chn - d][col];
-                l.c = sh[buf][1][chn - d][col];
-                l.r = sh[buf][2][chn - d][col];
-            }
-            if (chn + d < NC) {
-                h.a = sh[buf][0][chn + d][col];
-                h.c = sh[buf][1][chn + d][col];
-                h.r = sh[buf][2][chn + d][col];
-            }
+            NEq<T> l, h;
+            const int cl = chn >= d ? chn - d : chn, ch_ = chn + d < NC ? chn + d : chn;
+            l.a = sh[buf][0][cl][col];
+            l.c = sh[buf][1][cl][col];
+            l.r = sh[buf][2][cl][col];
+            h.a = sh[buf][0][ch_][col];
+            h.c = sh[buf][1][ch_][col];
+            h.r = sh[buf][2][ch_][col];
             e = pcr_step(e, l, h);
             buf ^= 1;
             const NEq<T> m = normalized(e);
@@ -373,19 +364,17 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NC][W], int N, int col, i
         const T xi = e.r * fast_rcp(e.b);
         sh[buf ^ 1][0][chn][col] = xi;
         __syncthreads();
-        const T xl = chn > 0 ? sh[buf ^ 1][0][chn - 1][col] : (T)0;
+        const T xl = sh[buf ^ 1][0][chn > 0 ? chn - 1 : 0][col];
         if (!active)
             return;
         if constexpr (PUT_ALL) {
-            if (ch.cnt > 0) {
 #pragma unroll
-                for (int t = 0; t < C; ++t)
-                    put(t, s + t, ch.value(t, xl, xi));
-            }
+            for (int t = 0; t < C; ++t)
+                put(t, s + t, ch.value(t, xl, xi));
         } else {
 #pragma unroll
             for (int t = 0; t < C; ++t)
-                if (t < ch.cnt)
+                if (s + t <= N)
                     put(t, s + t, ch.value(t, xl, xi));
         }
     });

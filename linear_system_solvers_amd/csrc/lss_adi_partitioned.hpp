@@ -86,7 +86,7 @@ __device__ __forceinline__ void vsweep_lines(const Prob<T> &p, const Tile &tl, i
             q.r = q.r - q.a * lo;
             q.a = (T)0;
         }
-        if (jj == N) {
+        if (jj >= N) {  // (and the padding past the end: c = 0 there too)
             q.r = q.r - q.c * hi;
             q.c = (T)0;
         }
@@ -192,7 +192,7 @@ template <typename T, int KIND, int PAT> struct XLine {
             e.r = e.r - e.a * lo;
             e.a = (T)0;
         }
-        if (k == N) {
+        if (k >= N) {  // (and the padding past the end: c = 0 there too)
             e.r = e.r - e.c * hi;
             e.c = (T)0;
         }
@@ -220,9 +220,16 @@ __device__ __forceinline__ void xsweep_lines(T (*sh)[4][NC][W], const Prob<T> &p
         ln.store_ends();
 }
 
+#ifndef LSS_XS1_WAVES
+#define LSS_XS1_WAVES 1
+#endif
+#ifndef LSS_XS2_WAVES
+#define LSS_XS2_WAVES 1
+#endif
 // W S-lines per workgroup of NC chunks each, the chunks spread over the workgroup (block_solve)
 template <typename T, int M, int STAGE, int W, int PAT, int NC>
-__global__ void __launch_bounds__(W *NC) k_xsweep_p(const Prob<T> *__restrict__ probs,
+__global__ void __launch_bounds__(W *NC) __attribute__((amdgpu_waves_per_eu(STAGE == 1 ? LSS_XS1_WAVES : LSS_XS2_WAVES)))
+k_xsweep_p(const Prob<T> *__restrict__ probs,
                                                     const Tile *__restrict__ tiles, int step, int swz)
 {
     __shared__ T sh[2][4][NC][W];

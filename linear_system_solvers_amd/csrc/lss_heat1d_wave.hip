@@ -133,11 +133,12 @@ __global__ void __launch_bounds__(64 * H1_WAVES) k_h1_wave(const H1Prob<T> *__re
                 e.b = (one + two * theta * B);
                 e.c = (-theta * D);
                 e.r = r;
-                bc.fold(kk == 1, kk == Nu, e.a, e.b, e.c, e.r);
+                bc.fold(kk == 1, kk >= Nu, e.a, e.b, e.c, e.r);  // (c = 0 on the padding past the end too)
                 return e;
             };
             T nu[C];
-            wave_solve_c<T, C, true>(Nu, lane, eq, [&](int t, int, T x) { nu[t] = x; });
+            // (the padding past the end restarts from zero every step, so that it stays finite)
+            wave_solve_c<T, C, true>(Nu, lane, eq, [&](int t, int kk, T x) { nu[t] = kk <= Nu ? x : (T)0; });
 #pragma unroll
             for (int t = 0; t < C; ++t)
                 u[t] = nu[t];
