@@ -558,11 +558,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         } else if (stage == 1) {
             hipLaunchKernelGGL((k_xsweep_p<T, M, 1, W, PAT, 64>), g, dim3(W * 64), 0, st, pd, tx, s, swz);
         } else {
-#if defined(LSS_XS2_NC64) && LSS_XS2_NC64
-            hipLaunchKernelGGL((k_xsweep_p<T, M, 2, W, PAT, 64>), g, dim3(W * 64), 0, st, pd, tx, s, swz);
-#else
             hipLaunchKernelGGL((k_xsweep_p<T, 2 * M, 2, W, PAT, 32>), g, dim3(W * 32), 0, st, pd, tx, s, swz);
-#endif
         }
     }
     template <int M, int W> void launch_x_m(int stage, int s, hipStream_t st, const Prob<T> *pd, int64_t na)

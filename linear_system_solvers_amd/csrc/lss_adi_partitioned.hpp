@@ -220,16 +220,9 @@ __device__ __forceinline__ void xsweep_lines(T (*sh)[4][NC][W], const Prob<T> &p
         ln.store_ends();
 }
 
-#ifndef LSS_XS1_WAVES
-#define LSS_XS1_WAVES 1
-#endif
-#ifndef LSS_XS2_WAVES
-#define LSS_XS2_WAVES 1
-#endif
 // W S-lines per workgroup of NC chunks each, the chunks spread over the workgroup (block_solve)
 template <typename T, int M, int STAGE, int W, int PAT, int NC>
-__global__ void __launch_bounds__(W *NC) __attribute__((amdgpu_waves_per_eu(STAGE == 1 ? LSS_XS1_WAVES : LSS_XS2_WAVES)))
-k_xsweep_p(const Prob<T> *__restrict__ probs,
+__global__ void __launch_bounds__(W *NC) k_xsweep_p(const Prob<T> *__restrict__ probs,
                                                     const Tile *__restrict__ tiles, int step, int swz)
 {
     __shared__ T sh[2][4][NC][W];
