@@ -635,14 +635,20 @@ def main():
     import torch
     torch.cuda.init()
     if args.calibrate_copy:
-        # reference kernels of known traffic: a fill writing 256 MiB, then an elementwise kernel
-        # reading 256 MiB and writing 256 MiB
+        # reference kernels of known traffic: a fill writing 256 MiB, an elementwise kernel reading
+        # 256 MiB and writing 256 MiB, and (an independent second scale) one reading 2 x 256 MiB and
+        # writing 256 MiB; each reads what an earlier kernel wrote, as the ADI stages do
         x = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda").uniform_()
         y = torch.empty_like(x)
         torch.mul(x, 2.0, out=y)
+        a, b = torch.full_like(x, 1.5), torch.full_like(x, 2.5)
         torch.cuda.synchronize()
-        log("calibration: fill of 256 MiB, then 256 MiB read + 256 MiB written by one elementwise kernel")
-        del x, y
+        c = torch.empty_like(x)
+        torch.add(a, b, out=c)
+        torch.cuda.synchronize()
+        log("calibration: fill of 256 MiB; 256 MiB read + 256 MiB written (mul); two fills; 512 MiB read + "
+            "256 MiB written (add)")
+        del x, y, a, b, c
     if args.only:  # one secondary workload alone (for kernel traces of it)
         fn = {"adjoint": bench_adjoint, "sabr": bench_sabr, "tridiag": bench_tridiag,
               "explicit": bench_explicit}[args.only]

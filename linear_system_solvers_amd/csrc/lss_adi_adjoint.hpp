@@ -289,43 +289,6 @@ template <typename T> __device__ __forceinline__ bool inside(const Prob<T> &p, i
     return i >= 1 && i <= p.nx - 2 && j >= 1 && j <= p.ny - 2;
 }
 
-// lambda(Y2) at (i, j) (interior row i, any column): gathered from the X2 equations (CS/MCS: Y2 in
-// the Delta terms; HV: rhs_intermed_4 and the explicit stage H, whose multiplier is also lx2) and,
-// for HV, from the V2 equations (in = Y2).  S(di, dj) gives the stencil at (i + di, j + dj).  SCH is
-// the scheme (compile time).  Every neighbour is read (at clamped indices) and weighted by whether
-// it is an equation point, so that all loads are issued together.
-template <int SCH, typename T, typename SF>
-__device__ __forceinline__ T lam_y2(const Prob<T> &p, const Adj<T> &A, SF &&S, int i, int j)
-{
-    constexpr bool hv = SCH == SCHEME_HV, mcs = SCH == SCHEME_MCS;
-    const int ny = p.ny, P = p.nx - 1;
-    const gptr<const T> lx2 = G((const T *)A.lx2), lv2 = G((const T *)A.lv2);
-    T v = (T)0;
-#pragma unroll
-    for (int di = -1; di <= 1; ++di) {
-#pragma unroll
-        for (int dj = -1; dj <= 1; ++dj) {
-            const int ei = i + di, ej = j + dj;
-            const bool in = inside(p, ei, ej);
-            const int q = min(max(ei, 0), P) * ny + min(max(ej, 0), ny - 1);
-            const Stencil<T> s = S(di, dj);  // not defined outside the equation points: selected away
-            T t;
-            if constexpr (hv) {
-                T w = w_h(p, s, -di, -dj, 2);
-                if (dj == 0)
-                    w += w_r4(p, s, -di);
-                t = w * lx2[q];
-                if (di == 0)
-                    t += w_rv(p, s, -dj) * lv2[q];
-            } else {
-                t = w_r3(p, s, -di, -dj, mcs, false) * lx2[q];
-            }
-            v += in ? t : (T)0;
-        }
-    }
-    return v;
-}
-
 // -------------------------------------------------------------------------------------------------
 // Adjoint of a v-sweep (one wavefront per row i).  last: the step's last v-stage, whose output went
 // through boundary stage B into U_{s+1}; otherwise V1 of CS/MCS/HV.
@@ -604,8 +567,11 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
 }
 
 // -------------------------------------------------------------------------------------------------
-// Gathers over two-dimensional tiles of GTR x GTC points (one thread per point).  The stencils of
-// the (GTR + 2) x (GTC + 2) equation points around the tile are evaluated once into LDS.
+// Gathers over two-dimensional tiles of GTR x GTC points (one thread per point).  Every equation point
+// e of the (GTR + 2) x (GTC + 2) points around the tile first forms, once, the nine contributions it
+// makes to the lambdas of the values it reads, c_e(oi, oj) for the value at e + (oi, oj) (the weight
+// of that value in each of e's equations times the equation's lambda), and leaves them in LDS; each
+// point of the tile then sums the nine contributions addressed to it.
 //   WHAT 0: lambda(U_s) from every equation that reads U_s (the stencils of the right-hand sides,
 //           the Dirichlet ends of the S-lines through idx2, boundary stage A), and the parameter
 //           terms of boundary stage A (its D, E, F on the v = 0 row)
@@ -615,7 +581,7 @@ template <typename T, int WHAT, int PAT>
 __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
                                                     const Tile *__restrict__ tiles, int step, double *__restrict__ slots)
 {
-    __shared__ T sst[7][GTR + 2][GTC + 2];
+    __shared__ T cst[9][GTR + 2][GTC + 2];  // c_e(oi, oj) at [(oi + 1) * 3 + oj + 1]
     __shared__ double red[GTR * GTC / 64][NPAR];
     const Tile tl = tiles[blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
@@ -628,38 +594,6 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
     const int r0 = tl.first, c0 = tl.col;
     const int nx = p.nx, ny = p.ny, P = nx - 1;
     const GArr<T> cs{coeff_set(p, step)};
-    if (run) {
-        for (int idx = threadIdx.x; idx < (GTR + 2) * (GTC + 2); idx += GTR * GTC) {
-            const int lr = idx / (GTC + 2), lc = idx % (GTC + 2);
-            const int ei = r0 - 1 + lr, ej = c0 - 1 + lc;
-            if (inside(p, ei, ej)) {
-                const Stencil<T> st = stencil_at<T, PAT>(p, cs, ei, ej);
-                sst[0][lr][lc] = st.M;
-                sst[1][lr][lc] = st.Mt;
-                sst[2][lr][lc] = st.P;
-                sst[3][lr][lc] = st.Pt;
-                sst[4][lr][lc] = st.Z;
-                sst[5][lr][lc] = st.W;
-                sst[6][lr][lc] = st.C;
-            }
-        }
-    }
-    __syncthreads();
-    const int tr = threadIdx.x / GTC, tc = threadIdx.x % GTC;
-    const int r = r0 + tr, c = c0 + tc;
-    // stencil at (r + di, c + dj); the point must be an equation point (inside)
-    auto S = [&](int di, int dj) -> Stencil<T> {
-        Stencil<T> st;
-        const int lr = tr + 1 + di, lc = tc + 1 + dj;
-        st.M = sst[0][lr][lc];
-        st.Mt = sst[1][lr][lc];
-        st.P = sst[2][lr][lc];
-        st.Pt = sst[3][lr][lc];
-        st.Z = sst[4][lr][lc];
-        st.W = sst[5][lr][lc];
-        st.C = sst[6][lr][lc];
-        return st;
-    };
     // the scheme as a compile-time constant (uniform over the workgroup)
     auto with_scheme = [&](auto &&f) {
         switch (p.scheme) {
@@ -669,16 +603,85 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
         default: f(std::integral_constant<int, SCHEME_HV>{}); break;
         }
     };
+    if (run) {
+        const gptr<const T> lv2 = G((const T *)A.lv2), lv1 = G((const T *)A.lv1), lx2 = G((const T *)A.lx2),
+                            lx1 = G((const T *)A.lx1);
+        for (int idx = threadIdx.x; idx < (GTR + 2) * (GTC + 2); idx += GTR * GTC) {
+            const int lr = idx / (GTC + 2), lc = idx % (GTC + 2);
+            const int ei = r0 - 1 + lr, ej = c0 - 1 + lc;
+            if (!inside(p, ei, ej)) {  // not an equation point: no contributions
+#pragma unroll
+                for (int k = 0; k < 9; ++k)
+                    cst[k][lr][lc] = (T)0;
+                continue;
+            }
+            const Stencil<T> st = stencil_at<T, PAT>(p, cs, ei, ej);
+            const uint32_t q = (uint32_t)ei * (uint32_t)ny + (uint32_t)ej;
+            with_scheme([&](auto sc) {
+                constexpr int SCH = decltype(sc)::value;
+                constexpr bool DR = SCH == SCHEME_DR, HV = SCH == SCHEME_HV, MCS = SCH == SCHEME_MCS;
+                if constexpr (WHAT == 0) {
+                    const T l1 = ldg(lx1, q), l2 = DR ? (T)0 : ldg(lx2, q);
+                    const T lv = (HV ? (T)0 : ldg(lv2, q)) + (DR ? (T)0 : ldg(lv1, q));
+#pragma unroll
+                    for (int oi = -1; oi <= 1; ++oi) {
+#pragma unroll
+                        for (int oj = -1; oj <= 1; ++oj) {
+                            T t = w_r1(p, st, oi, oj) * l1;
+                            if (oi == 0)
+                                t += w_rv(p, st, oj) * lv;
+                            if constexpr (!DR && !HV)
+                                t += w_r3(p, st, oi, oj, MCS, true) * l2;
+                            if constexpr (HV)
+                                t += w_h(p, st, oi, oj, 1) * l2;
+                            cst[(oi + 1) * 3 + oj + 1][lr][lc] = t;
+                        }
+                    }
+                } else {  // lambda(Y2): the X2 equations (and, for HV, V2, whose input is Y2)
+                    const T l2 = ldg(lx2, q), lv = HV ? ldg(lv2, q) : (T)0;
+#pragma unroll
+                    for (int oi = -1; oi <= 1; ++oi) {
+#pragma unroll
+                        for (int oj = -1; oj <= 1; ++oj) {
+                            T t;
+                            if constexpr (HV) {
+                                T w = w_h(p, st, oi, oj, 2);
+                                if (oj == 0)
+                                    w += w_r4(p, st, oi);
+                                t = w * l2;
+                                if (oi == 0)
+                                    t += w_rv(p, st, oj) * lv;
+                            } else {
+                                t = w_r3(p, st, oi, oj, MCS, false) * l2;
+                            }
+                            cst[(oi + 1) * 3 + oj + 1][lr][lc] = t;
+                        }
+                    }
+                }
+            });
+        }
+    }
+    __syncthreads();
+    const int tr = threadIdx.x / GTC, tc = threadIdx.x % GTC;
+    const int r = r0 + tr, c = c0 + tc;
     if (run && tr < tl.count && c < ny) {
+        // the contributions addressed to (r, c): from e = (r + di, c + dj) at offset (-di, -dj)
+        T v = (T)0;
+#pragma unroll
+        for (int di = -1; di <= 1; ++di) {
+#pragma unroll
+            for (int dj = -1; dj <= 1; ++dj)
+                v += cst[(1 - di) * 3 + 1 - dj][tr + 1 + di][tc + 1 + dj];
+        }
         if (WHAT == 1) {
             if (r >= 1 && r <= P - 1)
-                with_scheme([&](auto sc) { G(A.ly2)[(int)r * ny + c] = lam_y2<decltype(sc)::value>(p, A, S, r, c); });
+                G(A.ly2)[(int)r * ny + c] = v;
         } else {
             const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
             const gptr<T> lam = G((step & 1) ? A.lb : A.la);
             const gptr<const T> bs = bnd_set(p, step);
-            const gptr<const T> lv2 = G((const T *)A.lv2), lv1 = G((const T *)A.lv1), lx2 = G((const T *)A.lx2),
-                                lx1 = G((const T *)A.lx1), elo2 = G((const T *)A.elo2), elo1 = G((const T *)A.elo1);
+            const gptr<const T> lx2 = G((const T *)A.lx2), lx1 = G((const T *)A.lx1), elo2 = G((const T *)A.elo2),
+                                elo1 = G((const T *)A.elo1);
             const gptr<const uint32_t> inv1o = G(A.inv1o), inv1i = G(A.inv1i), inv2o = G(A.inv2o),
                                        inv2i = G(A.inv2i);
             const T theta = p.theta, delta = p.delta, ni = p.ni, rho = p.rho;
@@ -693,33 +696,6 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
                 }
                 return v;
             };
-            // the stencils of the right-hand sides that read U_s at (r, c); every neighbour is read
-            // (at clamped indices) and weighted by whether it is an equation point
-            T v = (T)0;
-            with_scheme([&](auto sc) {
-                constexpr int SCH = decltype(sc)::value;
-                constexpr bool DR = SCH == SCHEME_DR, HV = SCH == SCHEME_HV, MCS = SCH == SCHEME_MCS;
-#pragma unroll
-                for (int di = -1; di <= 1; ++di) {
-#pragma unroll
-                    for (int dj = -1; dj <= 1; ++dj) {
-                        const int ei = r + di, ej = c + dj;
-                        const bool in = inside(p, ei, ej);
-                        const int q = min(max(ei, 0), P) * ny + min(max(ej, 0), ny - 1);
-                        const Stencil<T> st = S(di, dj);  // not defined outside: selected away
-                        T t = w_r1(p, st, -di, -dj) * lx1[q];
-                        if (di == 0 && !HV)
-                            t += w_rv(p, st, -dj) * lv2[q];
-                        if (di == 0 && !DR)
-                            t += w_rv(p, st, -dj) * lv1[q];
-                        if constexpr (!DR && !HV)
-                            t += w_r3(p, st, -di, -dj, MCS, true) * lx2[q];
-                        if constexpr (HV)
-                            t += w_h(p, st, -di, -dj, 1) * lx2[q];
-                        v += in ? t : (T)0;
-                    }
-                }
-            });
             // Dirichlet ends of the S-lines j with idx2[j] == c (implicit_kernel.hpp:86-101)
             if (r == 0 || r == P) {
                 const int ei = (r == 0) ? 1 : P - 1;
@@ -727,7 +703,7 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
                     const int j = (int)inv2i[k];
                     if (j < 1 || j > ny - 2)
                         continue;
-                    const Stencil<T> st = (j >= c - 1 && j <= c + 1) ? S(ei - r, j - c) : stencil_at<T, PAT>(p, cs, ei, j);
+                    const Stencil<T> st = stencil_at<T, PAT>(p, cs, ei, j);
                     const T w = (r == 0) ? theta * st.M : theta * st.P;  // -(folded coefficient)
                     const int q = (int)ei * ny + j;
                     T l = lx1[q];
@@ -775,7 +751,8 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
             lam[(int)r * ny + c] = v;
         }
     }
-    if (WHAT == 0)
+    // (only the tiles of column 0 hold parameter terms: boundary stage A's D, E, F on the v = 0 row)
+    if (WHAT == 0 && c0 == 0)
         block_acc_store<GTR * GTC / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
 
