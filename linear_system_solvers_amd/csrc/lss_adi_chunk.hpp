@@ -107,7 +107,8 @@ template <typename T, int M> struct Chunk {
         }
     }
 
-    // a chunk that holds no line: its interface row is the identity and its relations are zero
+    // a chunk that holds no points of a line: its interface row is the identity and its relations
+    // are zero
     __device__ __forceinline__ void set_identity()
     {
 #pragma unroll
@@ -252,10 +253,14 @@ __device__ __forceinline__ void wave_solve_c(int N, int lane, EqF &&eq, PutF &&p
 {
     const int s = 1 + lane * C;
     Chunk<T, C> ch;
+    if (s <= N) {
 #pragma unroll
-    for (int t = 0; t < C; ++t)
-        ch.put(t, eq_at(eq, t, s + t));
-    ch.bwd();
+        for (int t = 0; t < C; ++t)
+            ch.put(t, eq_at(eq, t, s + t));
+        ch.bwd();
+    } else {  // a chunk wholly past the end: identity rows, and no loads
+        ch.set_identity();
+    }
     T PF, QF, RF;
     ch.first_rel(PF, QF, RF);
     // (the last lane takes its own relation: its interface row has c = 0, as every row at k >= N)
@@ -313,12 +318,12 @@ __device__ __forceinline__ void block_solve(T (*sh)[4][NC][W], int N, int col, i
         constexpr int C = decltype(cc)::value;
         const int s = 1 + chn * C;
         Chunk<T, C> ch;
-        if (active) {
+        if (active && s <= N) {
 #pragma unroll
             for (int t = 0; t < C; ++t)
                 ch.put(t, eq_at(eq, t, s + t));
             ch.bwd();
-        } else {
+        } else {  // no line, or a chunk wholly past its end: identity rows, and no loads
             ch.set_identity();
         }
         {
