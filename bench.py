@@ -541,7 +541,8 @@ def bench_sabr(torch, log, count=256, fd_check=3, oracle_check=1):
         pin, pout = [u.data_ptr() for u in ins], [u.data_ptr() for u in o]
         plan.run_dev(pin, pout, stream, graph=True)
         secs = time_stream(torch, lambda: plan.run_dev(pin, pout, stream, graph=True), 2) / 2
-        res[name] = {"ms": round(secs * 1e3, 2), "grid_point_steps_per_s": ps / secs}
+        res[name] = {"ms": round(secs * 1e3, 2), "grid_point_steps_per_s": ps / secs,
+                     "kernel_config": plan.kernel_config()}
         outs[name] = [t.cpu().numpy().astype(np.float64) for t in o]
         log(f"sabr book {name}: {secs * 1e3:.1f} ms, {ps / secs:.3e} grid-point-steps/s")
         if fp == L.FP64:
@@ -557,30 +558,32 @@ def bench_sabr(torch, log, count=256, fd_check=3, oracle_check=1):
             t_bwd = time.perf_counter() - t0
             res["adjoint_backward_ms"] = round(t_bwd * 1e3, 2)
             res["adjoint_ratio_backward_over_forward"] = round(t_bwd / secs, 3)
-            # d price / d alpha0: slope of the bilinear interpolant in alpha (grid held fixed)
-            da0 = []
-            for p in range(count):
-                xs, ys = plan.grid(p)
-                U = outs["f64"][p]
-                i = int(np.clip(np.searchsorted(xs, models[p].strike, side="right") - 1, 0, len(xs) - 2))
-                j = int(np.clip(np.searchsorted(ys, a0s[p], side="right") - 1, 0, len(ys) - 2))
-                wx = (models[p].strike - xs[i]) / (xs[i + 1] - xs[i])
-                da0.append(((1 - wx) * (U[i, j + 1] - U[i, j]) + wx * (U[i + 1, j + 1] - U[i + 1, j])) /
-                           (ys[j + 1] - ys[j]))
+            # d price / d alpha0 from the adjoint: alpha_max = 5 alpha0 and the price point alpha0 move
+            # together, so the price is read at a fixed eta and d/d alpha0 = (d/d of the alpha-axis
+            # scale) / alpha0 (the plan's fourth SABR parameter term)
+            da0 = [float(dp[p, 3]) / a0s[p] for p in range(count)]
             worst = 0.0
             for p in range(fd_check):
-                for k, attr in enumerate(("sig_sig", "rho", "beta")):
-                    v = getattr(models[p], attr)
+                for k, attr in enumerate(("sig_sig", "rho", "beta", "alpha0")):
+                    v = a0s[p] if attr == "alpha0" else getattr(models[p], attr)
                     h = 1e-6 * abs(v)
                     vals = []
                     for sgn in (1.0, -1.0):
                         mq = sabr_models(count, nx=models[p].nx)[0][p]
-                        setattr(mq, attr, v + sgn * h)
+                        if attr == "alpha0":
+                            mq.y_hi = 5.0 * (v + sgn * h)
+                        else:
+                            setattr(mq, attr, v + sgn * h)
                         pq = L.Plan.from_models([mq], L.FP64)
-                        vals.append(float(np.sum(ws[p] * pq.run_host()[0])))
+                        wq = ws[p]
+                        if attr == "alpha0":
+                            gx, gy = pq.grid(0)
+                            wq = L.bilinear_weights(gx, gy, mq.strike, v + sgn * h)
+                        vals.append(float(np.sum(wq * pq.run_host()[0])))
                         pq.close()
                     fdv = (vals[0] - vals[1]) / (2 * h)
-                    worst = max(worst, abs(dp[p, k] - fdv) / max(abs(fdv), 1e-12))
+                    got = da0[p] if attr == "alpha0" else dp[p, k]
+                    worst = max(worst, abs(got - fdv) / max(abs(fdv), 1e-12))
             res["adjoint_params"] = ["alpha0", "nu", "rho", "beta"]
             res["adjoint_sample"] = [[float(da0[p]), *map(float, dp[p, :3])] for p in range(min(count, 4))]
             res["adjoint_fd_max_rel_diff"] = worst

@@ -201,8 +201,9 @@ template <typename T> __device__ __forceinline__ Eq<T> identity_eq()
 
 // -------------------------------------------------------------------------------------------------
 // Generic line solves.  A line of N interior unknowns is cut into chunks of C = chunk_len(N) points
-// (the smallest power of two with NCH * C >= N, so the partition depends on N alone); the chunks are
-// padded past the line's end to C points each.  Each thread evaluates eq for all C points of its
+// (the smallest of 1, 2, 3, 4, 6, 8, 12, 16, ... (powers of two and three quarters of them) with
+// NCH * C >= N, so the partition depends on N alone); the chunks are padded past the line's end to C
+// points each.  Each thread evaluates eq for all C points of its
 // chunk without a per-point branch, so the loads of all its points are issued together.
 //   eq(k) returns the equation of interior unknown k (1..N) with the ends already folded (a = 0 at
 //   k = 1, c = 0 at k = N).  It is also called with N < k <= NCH * C (the padding): it must then read
@@ -219,6 +220,9 @@ __device__ __forceinline__ int chunk_len(int N, int nch = NCH)
     int c = 1;
     while (c * nch < N)
         c <<= 1;
+    // three quarters of a power of two >= 4 when they suffice: fewer points past the end
+    if (c >= 4 && (3 * c / 4) * nch >= N)
+        c = 3 * c / 4;
     return c;
 }
 
@@ -232,12 +236,19 @@ template <typename EqF> __device__ __forceinline__ auto eq_at(EqF &eq, int t, in
         return eq(k);
 }
 
-// f(std::integral_constant<int, C>) for the chunk length c (a power of two <= M; uniform)
+// f(std::integral_constant<int, C>) for the chunk length c (a value of chunk_len <= M, M a power of
+// two; uniform)
 template <int M, typename F> __device__ __forceinline__ void with_chunk_len(int c, F &&f)
 {
     if constexpr (M > 1) {
-        if (c < M) {
+        if (c <= M / 2) {
             with_chunk_len<M / 2>(c, f);
+            return;
+        }
+    }
+    if constexpr (M >= 4) {
+        if (c <= 3 * M / 4) {
+            f(std::integral_constant<int, 3 * M / 4>{});
             return;
         }
     }

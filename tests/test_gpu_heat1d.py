@@ -59,13 +59,13 @@ def test_exact_kernel_is_bit_identical_to_the_oracle(oracle, fp, case):
 
 
 @pytest.mark.parametrize("fp,tol", [(L.FP64, 1e-12), (L.FP32, 1e-5)])
-@pytest.mark.parametrize("n", [10, 66, 100, 130, 300, 514, 1026])
+@pytest.mark.parametrize("n", [10, 66, 100, 130, 152, 300, 514, 702, 1026])
 def test_partitioned_kernel_within_tolerance(oracle, fp, tol, n):
     """f64: within 1e-12 of the oracle.  f32: the partitioned elimination rounds differently from the
     sequential Thomas sweep, and Crank-Nicolson carries that difference through the loop undamped; the
     bar is 1e-5 or the f32 oracle's own distance from the f64 oracle, whichever is larger, and the GPU's
     distance from the f64 oracle within 1.5 times the f32 oracle's (as accurate as the reference's f32)."""
-    # unknowns n - 2 take chunk lengths 1, 1, 2, 4, 8, 8, 16
+    # unknowns n - 2 take chunk lengths 1, 1, 2, 2, 3, 6, 8, 12, 16
     for extra in (dict(), dict(implicit_scheme=L.EULER, grid=L.NONUNIFORM, accumulation_point=10.0,
                                direction=L.FORWARD)):
         m = H.BsModel(n=n, nt=60, line_solver=L.LINE_PARTITIONED, **extra)
