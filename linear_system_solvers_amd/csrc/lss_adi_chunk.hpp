@@ -201,8 +201,7 @@ template <typename T> __device__ __forceinline__ Eq<T> identity_eq()
 
 // -------------------------------------------------------------------------------------------------
 // Generic line solves.  A line of N interior unknowns is cut into chunks of C = chunk_len(N) points
-// (the smallest of 1, 2, 3, 4, 6, 8, 12, 16, ... (powers of two and three quarters of them) with
-// NCH * C >= N, so the partition depends on N alone); the chunks are padded past the line's end to C
+// (the smallest of 1, 2, 3, 4, 6, 8, 16 with NCH * C >= N, so the partition depends on N alone); the chunks are padded past the line's end to C
 // points each.  Each thread evaluates eq for all C points of its
 // chunk without a per-point branch, so the loads of all its points are issued together.
 //   eq(k) returns the equation of interior unknown k (1..N) with the ends already folded (a = 0 at
@@ -220,8 +219,9 @@ __device__ __forceinline__ int chunk_len(int N, int nch = NCH)
     int c = 1;
     while (c * nch < N)
         c <<= 1;
-    // three quarters of a power of two >= 4 when they suffice: fewer points past the end
-    if (c >= 4 && (3 * c / 4) * nch >= N)
+    // 3 or 6 when they suffice: fewer points past the end (12 is not used: its code path slowed the
+    // 16-point kernels by more than the padding it saves)
+    if (c >= 4 && c <= 8 && (3 * c / 4) * nch >= N)
         c = 3 * c / 4;
     return c;
 }
@@ -246,7 +246,7 @@ template <int M, typename F> __device__ __forceinline__ void with_chunk_len(int 
             return;
         }
     }
-    if constexpr (M >= 4) {
+    if constexpr (M >= 4 && M <= 8) {
         if (c <= 3 * M / 4) {
             f(std::integral_constant<int, 3 * M / 4>{});
             return;

@@ -65,7 +65,7 @@ def test_partitioned_kernel_within_tolerance(oracle, fp, tol, n):
     sequential Thomas sweep, and Crank-Nicolson carries that difference through the loop undamped; the
     bar is 1e-5 or the f32 oracle's own distance from the f64 oracle, whichever is larger, and the GPU's
     distance from the f64 oracle within 1.5 times the f32 oracle's (as accurate as the reference's f32)."""
-    # unknowns n - 2 take chunk lengths 1, 1, 2, 2, 3, 6, 8, 12, 16
+    # unknowns n - 2 take chunk lengths 1, 1, 2, 2, 3, 6, 8, 16, 16
     for extra in (dict(), dict(implicit_scheme=L.EULER, grid=L.NONUNIFORM, accumulation_point=10.0,
                                direction=L.FORWARD)):
         m = H.BsModel(n=n, nt=60, line_solver=L.LINE_PARTITIONED, **extra)
