@@ -478,7 +478,9 @@ def explicit_models(count=1024, seed=5, coeff_mode=L.COEFF_SEPARABLE):
 
 
 def bench_explicit(torch, log, count=1024, oracle_check=2):
-    """Explicit Euler book: per step bnd_a, euler, bnd_b over every problem, replayed from a hipGraph."""
+    """Explicit Euler book: 1024 problems of 50 x 30 points, each problem's whole loop in one workgroup
+    with its time levels resident in LDS (one launch per solve); the step-by-step kernel (every level
+    output) is timed per launch beside it."""
     from oracle import oracle as O
     from tests.helpers import oracle_solve
     models = explicit_models(count)
@@ -491,19 +493,26 @@ def bench_explicit(torch, log, count=1024, oracle_check=2):
     torch.cuda.synchronize()
     secs = time_stream(torch, lambda: plan.run_dev(pin, pout, stream, graph=True), 2) / 2
     ps = point_steps(models)
-    prof = L.run_profiled(plan, pin, pout, stream)
+    resident = plan.kernel_config()["explicit_resident"]
+    prof = L.run_profiled(plan, pin, pout, stream)  # (launch by launch)
     e_ms, e_bytes, e_n = prof["euler"]
     gbs = e_bytes / (e_ms / 1e3) / 1e9
+    # the resident loop reads 9 neighbours and writes 1 value per interior point-step in LDS (an upper
+    # figure: boundary points read fewer)
+    lds_gbs = 10 * 8 * ps / secs / 1e9
     worst = 0.0
     for q in range(oracle_check):
         c = oracle_solve(O, models[q])
         worst = max(worst, float(np.abs(outs[q].cpu().numpy() - c).max() / np.abs(c).max()))
-    log(f"explicit Euler book: {count} problems 50x30x7999, {secs * 1e3:.1f} ms, {ps / secs:.3e} grid-point-steps/s; "
-        f"k_euler {e_ms / e_n * 1e3:.2f} us per launch, {gbs:.0f} GB/s; vs oracle (normwise, {oracle_check} "
-        f"problems) {worst:.2e}")
+    log(f"explicit Euler book: {count} problems 50x30x7999, {secs * 1e3:.1f} ms, {ps / secs:.3e} grid-point-steps/s "
+        f"({'one resident launch, ' + format(lds_gbs, '.0f') + ' GB/s of LDS traffic' if resident else 'per-step launches'}); "
+        f"step-by-step k_euler {e_ms / e_n * 1e3:.2f} us per launch, {gbs:.0f} GB/s; vs oracle (normwise, "
+        f"{oracle_check} problems) {worst:.2e}")
     plan.close()
     return {"workload": f"heston_explicit_euler_{count}x50x30x7999", "ms": round(secs * 1e3, 2),
-            "grid_point_steps_per_s": ps / secs, "euler_kernel_GBs": round(gbs, 1),
+            "grid_point_steps_per_s": ps / secs, "resident_in_lds": resident,
+            "resident_lds_GBs": round(lds_gbs, 1) if resident else None,
+            "per_step_kernel_us": round(e_ms / e_n * 1e3, 2), "euler_kernel_GBs": round(gbs, 1),
             "euler_kernel_roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "euler_avg_launch_us": round(e_ms / e_n * 1e3, 2),
             "vs_oracle_normwise": worst, "problems_checked": oracle_check}
 

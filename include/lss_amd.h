@@ -144,8 +144,11 @@ int lss_adi_set_adjoint_overlap(lss_adi_plan *plan, int32_t mode);
  * the recomputation of the next segment with the adjoint of the current one (two segment storage
  * sets, see lss_adi_set_adjoint_overlap; -1 before first use), and the number of
  * concurrent groups a graph replay of lss_adi_run_dev runs the problems in (2 for plans of two or
- * more problems, else 1). */
-enum { LSS_KERNEL_CONFIG_LEN = 12 };
+ * more problems, else 1), and whether lss_adi_run_dev / lss_adi_run_host step an explicit Euler plan in
+ * one launch with the time levels resident in LDS (1: every problem has at most 1536 points, nx >= 4,
+ * ny <= 256, one coefficient set and one boundary set; lss_adi_run_levels always steps launch by
+ * launch). */
+enum { LSS_KERNEL_CONFIG_LEN = 13 };
 int lss_plan_kernel_config(const lss_adi_plan *plan, int32_t *config);
 int lss_adi_run_host(lss_adi_plan *plan, const void *const *u_in, void *const *u_out);
 /*

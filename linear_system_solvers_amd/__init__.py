@@ -47,7 +47,7 @@ HESTON_PARAMS = ("kappa", "theta_v", "sigma", "rho", "r")
 # SABR: nu, rho, beta and "alpha_axis", the scale lam of the alpha axis about y_lo (grid and price point
 # stretched together): dPrice/dalpha0 = d/dlam / alpha0 when y_lo = 0 and y_hi is a multiple of alpha0
 SABR_PARAMS = ("nu", "rho", "beta", "alpha_axis")
-KERNEL_CONFIG_LEN = 12
+KERNEL_CONFIG_LEN = 13
 KERNEL_KINDS = ("bnd_a", "xsweep1", "vsweep1", "hv_y3", "xsweep2", "vsweep2", "bnd_b", "euler")
 
 
@@ -301,7 +301,8 @@ class Plan:
         c = (C.c_int32 * KERNEL_CONFIG_LEN)()
         _check(lib().lss_plan_kernel_config(self._h, c))
         keys = ("line_solver", "s_lines_per_group", "v_lines_per_group", "xcd_order", "factor_pattern",
-                "s_chunk_bound", "v_chunk_bound", "tuned", "adjoint_pattern", "adjoint_s_lines_per_group", "adjoint_overlap", "graph_groups")
+                "s_chunk_bound", "v_chunk_bound", "tuned", "adjoint_pattern", "adjoint_s_lines_per_group", "adjoint_overlap", "graph_groups",
+                "explicit_resident")
         return dict(zip(keys, list(c)))
 
     def set_mapping(self, s_lines_per_group: int, v_lines_per_group: int, xcd_order: int = 0) -> None:

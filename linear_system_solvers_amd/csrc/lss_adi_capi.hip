@@ -34,6 +34,7 @@ template <typename T> __global__ void k_xsweep(const Prob<T> *, const Tile *, in
 template <typename T> __global__ void k_vsweep(const Prob<T> *, const Tile *, int, int);
 template <typename T> __global__ void k_hv_y3(const Prob<T> *, const Tile *, int);
 template <typename T, int PAT> __global__ void k_euler(const Prob<T> *, const Tile *, int);
+template <typename T, int PAT> __global__ void k_euler_res(const Prob<T> *, const Tile *);
 template <typename T>
 __global__ void k_tridiag_lane(int64_t, int, int, const T *, const T *, const T *, const T *, T *, T *, LineBC<T>);
 template <typename T, int M, int STAGE, int W, int PAT, int NC>
@@ -142,7 +143,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     // tile tables: rows [0,nx), cols [0,ny), S-lines [1,ny-2], v-lines [1,nx-2], interior rows
     // adjoint: S-line tiles of P_XTJ, v-line tiles of P_VROWS, one tile per row (all rows)
     // forward partitioned sweeps: S-line tiles of 16 (TB_X) or 8 (TB_X8), v-line tiles of 4 (TB_V) or 16 (TB_V16)
-    enum { TB_I = 0, TB_J, TB_X, TB_V, TB_R, TB_AX, TB_AV, TB_U, TB_X8, TB_V16, TB_X4, TB_E, TB_N };
+    enum { TB_I = 0, TB_J, TB_X, TB_V, TB_R, TB_AX, TB_AV, TB_U, TB_X8, TB_V16, TB_X4, TB_E, TB_P, TB_N };
     std::vector<Tile> tiles[TB_N];
     size_t tile_off[TB_N] = {};
     std::vector<int> prefix[TB_N];     // prefix[tb][q] = tiles of internal problems [0, q)
@@ -182,6 +183,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
     bool tuned = false;                // the autotuner has chosen xw and vr
     bool adj_ok = false;               // every line fits the partitioned solver (needed by the adjoint)
     bool euler = false;                // explicit Euler plan (no line sweeps)
+    bool euler_res = false;            // ... whose problems all fit k_euler_res (the loop resident in LDS)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -293,6 +295,11 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         }
         line_solver = descs[0].line_solver == LSS_LINE_PARTITIONED ? LINE_PARTITIONED : LINE_EXACT;
         euler = descs[0].splitting == LSS_EXPLICIT_EULER;
+        euler_res = euler;
+        for (int64_t q = 0; q < n; ++q)
+            euler_res = euler_res && descs[q].nx >= 4 && descs[q].ny <= ERES_T &&
+                        descs[q].nx * descs[q].ny <= (int64_t)ERES_T * ERES_K && descs[q].coeff_sets == 1 &&
+                        descs[q].bnd_sets == 1;
         {
             int64_t mx = 1, my = 1;
             for (int64_t q = 0; q < n; ++q) {
@@ -446,6 +453,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 case TB_U: lo = 0, hi = p.nx, step = 1; break;
                 case TB_X8: lo = 1, hi = p.ny - 1, step = 8; break;
                 case TB_X4: lo = 1, hi = p.ny - 1, step = 4; break;
+                case TB_P: lo = 0, hi = 1, step = 1; break;  // one tile per problem
                 case TB_V16: lo = 1, hi = p.nx - 1, step = 16 * vlines_per_wave(p.ny - 2, pmy); break;
                 default: lo = 1, hi = p.nx - 1, step = 1; break;
                 }
@@ -523,7 +531,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             const bool dr = p.scheme == SCHEME_DR, hv = p.scheme == SCHEME_HV;
             switch (k) {
             case LSS_K_BND_A: b += 5.0 * p.nx; break;
-            case LSS_K_XSWEEP1: b += 2 * nxy; break;
+            case LSS_K_XSWEEP1: b += 2 * nxy + (line_solver == LINE_PARTITIONED ? 5.0 * p.nx : 0); break;
             case LSS_K_VSWEEP1: b += 3 * nxy; break;
             case LSS_K_HV_Y3: b += hv ? 4 * nxy : 0; break;
             case LSS_K_XSWEEP2: b += dr ? 0 : ((hv && line_solver != LINE_PARTITIONED) ? 3 * nxy : 4 * nxy); break;
@@ -704,6 +712,7 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         cfg[9] = ck > 0 ? axw : 0;
         cfg[10] = ck > 0 ? (dbl ? 1 : 0) : -1;
         cfg[11] = ngroups;
+        cfg[12] = euler_res ? 1 : 0;
     }
 
     void set_overlap(int mode) override
@@ -803,9 +812,11 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
             post();
             return;
         }
-        pre(LSS_K_BND_A);
-        hipLaunchKernelGGL(k_bnd_a<T>, dim3(grid(TB_I)), dim3(TILE), 0, st, pd, td + toff(TB_I), s);
-        post();
+        if (line_solver == LINE_EXACT) {  // (the partitioned stage-1 S-sweep evaluates stage A itself)
+            pre(LSS_K_BND_A);
+            hipLaunchKernelGGL(k_bnd_a<T>, dim3(grid(TB_I)), dim3(TILE), 0, st, pd, td + toff(TB_I), s);
+            post();
+        }
         auto xs = [&](int stage) {
             if (line_solver == LINE_EXACT) {
                 hipLaunchKernelGGL(k_xsweep<T>, dim3(grid(TB_X)), dim3(TILE), 0, st, pd, td + toff(TB_X), s, stage);
@@ -1387,6 +1398,22 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         }
     }
 
+    // every step of every problem of a plan of small explicit Euler problems (euler_res), the levels
+    // resident in LDS; the result is where the step-by-step loop leaves it
+    void launch_euler_res(hipStream_t st)
+    {
+        const Prob<T> *pd = (const Prob<T> *)probs_dev.p;
+        const Tile *tp = (const Tile *)tiles_dev.p + tile_off[TB_P];
+        const dim3 g((unsigned)count), b(ERES_T);
+        if (fpat == PAT_HESTON)
+            hipLaunchKernelGGL((k_euler_res<T, PAT_HESTON>), g, b, 0, st, pd, tp);
+        else if (fpat == PAT_SABR0)
+            hipLaunchKernelGGL((k_euler_res<T, PAT_SABR0>), g, b, 0, st, pd, tp);
+        else
+            hipLaunchKernelGGL((k_euler_res<T, PAT_GENERIC>), g, b, 0, st, pd, tp);
+        HIPCHK(hipGetLastError());
+    }
+
     void launch_loop(hipStream_t st)
     {
         for (int s = 0; s < max_steps; ++s)
@@ -1449,7 +1476,9 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 HIPCHK(hipMemcpyAsync(p.a, u_in[order[q]], sizeof(T) * (size_t)p.nx * p.ny, kin, stream));
             }
         }
-        if (use_graph && ngroups > 1) {
+        if (euler_res) {  // the whole loop in one launch, one workgroup per problem
+            launch_euler_res(stream);
+        } else if (use_graph && ngroups > 1) {
             run_groups(stream);
         } else if (use_graph) {
             if (!graph_exec) {

@@ -125,6 +125,7 @@ struct Tile {
 constexpr int NPAR = 6;    // model parameters per problem
 constexpr int GTR = 8, GTC = 32;  // gather tiles of the adjoint: rows x columns (one thread per point)
 constexpr int ERPT = 8;           // rows per thread of the explicit step (its tiles: GTR * ERPT rows)
+constexpr int ERES_T = 256, ERES_K = 6;  // threads and points per thread of the resident explicit loop
 
 // v-lines per wavefront of the partitioned v-sweep for lines of N interior points and the plan's
 // chunk bound M: 2 when the line fits 32 chunks of at most M <= 4 points, else 1 (the plan's v-line

@@ -24,6 +24,7 @@ namespace lssamd {
 
 // -------------------------------------------------------------------------------------------------
 // Boundary stage A: colA0 = new v = 0 column, colA1 = v_max column (explicit_boundary.hpp:39-84, :181-200)
+// as a launch of its own (plans with exact lines; the partitioned stage-1 S-sweep evaluates it itself)
 // -------------------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(TILE) k_bnd_a(const Prob<T> *__restrict__ probs, const Tile *__restrict__ tiles,
@@ -33,31 +34,7 @@ __global__ void __launch_bounds__(TILE) k_bnd_a(const Prob<T> *__restrict__ prob
     const Prob<T> &p = probs[tl.prob];
     if (step >= p.steps || (int)threadIdx.x >= tl.count)
         return;
-    const int i = tl.first + threadIdx.x;
-    const int nx = p.nx, ny = p.ny, N = nx - 1;
-    const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
-    const gptr<const T> cs = coeff_set(p, step);
-    const gptr<const T> bs = bnd_set(p, step);
-    const T four = (T)4.0, three = (T)3.0, two = (T)2.0, one = (T)1.0;
-    const T delta = p.delta, ni = p.ni, rho = p.rho;
-    T v;
-    if (i == 0) {
-        v = bs[nx];  // h_lower(t, Y(0))
-    } else {
-        T D, E, F;
-        def_row0(p, cs, i, D, E, F);
-        const gptr<const T> ui = u + (int64_t)i * ny;
-        if (i == N) {
-            const T delta_ = two * p.h1 * bs[nx + ny];  // 2 h_1 * h_upper(t, Y(0)) X'(1)
-            v = ((one - three * ni * E + rho * F) * ui[0]) + (four * ni * E * ui[1]) - (ni * E * ui[2]) -
-                (delta * delta_ * D);
-        } else {
-            v = (-delta * D * ui[-ny]) + ((one - three * ni * E + rho * F) * ui[0]) + (delta * D * ui[ny]) -
-                (ni * E * ui[2]) + (four * ni * E * ui[1]);
-        }
-    }
-    G(p.colA0)[i] = v;
-    G(p.colA1)[i] = bs[i];  // v_upper(t, X(zeta_i))
+    bnd_a_row(p, step, tl.first + (int)threadIdx.x);
 }
 
 // -------------------------------------------------------------------------------------------------
@@ -373,6 +350,122 @@ __global__ void __launch_bounds__(GTR *GTC) k_euler(const Prob<T> *__restrict__ 
 }
 
 // -------------------------------------------------------------------------------------------------
+// The whole explicit Euler time loop of one small problem in one workgroup of ERES_T threads, its two
+// time levels resident in LDS (lss_heston_euler_solver_scheme.hpp:63-67, 78-83).  Per step: the
+// interior and boundary stage A from the old level into the new one; a barrier; boundary stage B (row 0,
+// and row nx - 1 extrapolated from rows nx - 2 and nx - 3 of the new level, as the reference reads
+// them); a barrier.  Every value takes the arithmetic of k_euler in the same order.  The thread's
+// points are numbered interior first, then the v = 0 column, then the v_max column, so that the
+// wavefronts mostly take one branch; their stencil weights and boundary values are formed once (the
+// problems have one coefficient set and one boundary set).  For problems of at most ERES_T * ERES_K
+// points with nx >= 4 and ny <= ERES_T; HBM is read once and written once.
+// -------------------------------------------------------------------------------------------------
+template <typename T, int PAT>
+__global__ void __launch_bounds__(ERES_T) __attribute__((amdgpu_waves_per_eu(4)))
+k_euler_res(const Prob<T> *__restrict__ probs, const Tile *__restrict__ tiles)
+{
+    __shared__ T lv[2][ERES_T * ERES_K];
+    const Tile tl = tiles[blockIdx.x];
+    const Prob<T> &p = probs[tl.prob];
+    const int nx = p.nx, ny = p.ny, n = nx * ny, steps = p.steps, tid = (int)threadIdx.x;
+    const int ni_ = (nx - 2) * (ny - 2), na = ni_ + (nx - 2);  // interior points, then the v = 0 column
+    const T one = (T)1.0, two = (T)2.0, three = (T)3.0, four = (T)4.0;
+    {
+        const gptr<const T> a = G((const T *)p.a);
+        for (int q = tid; q < n; q += ERES_T)
+            lv[0][q] = a[q];
+    }
+    // the thread's points: pt = kind << 16 | (r * ny + c); kind 1 interior, 2 v = 0, 3 v_max, 0 none
+    int pt[ERES_K];
+#pragma unroll
+    for (int k = 0; k < ERES_K; ++k) {
+        const int e = tid + k * ERES_T;
+        int r = 0, c = 0, kd = 0;
+        if (e < ni_) {
+            r = 1 + e / (ny - 2);
+            c = 1 + e % (ny - 2);
+            kd = 1;
+        } else if (e < na) {
+            r = 1 + (e - ni_);
+            kd = 2;
+        } else if (e < na + (nx - 2)) {
+            r = 1 + (e - na);
+            c = ny - 1;
+            kd = 3;
+        }
+        pt[k] = kd << 16 | (r * ny + c);
+    }
+    // interior: C, M, M~, 1 - Z - W, P~, P (C = gamma c); v = 0: -delta D, 1 - 3 ni E + rho F, delta D,
+    // ni E, 4 ni E; v_max: the boundary value
+    T wt[ERES_K][6];
+    T lo_j = (T)0, hup_j = (T)0;  // boundary stage B of column tid: h_lower and 2 h_1 h_upper X'(1)
+    int jj = 0;
+    {
+        const gptr<const T> cs = coeff_set(p, 0);
+        const gptr<const T> bs = bnd_set(p, 0);
+#pragma unroll
+        for (int k = 0; k < ERES_K; ++k) {
+            const int kd = pt[k] >> 16, o = pt[k] & 0xffff, r = o / ny, c = o - r * ny;
+            if (kd == 1) {
+                const Stencil<T> st = stencil_at<T, PAT>(p, cs, r, c);
+                wt[k][0] = p.gamma * st.C;
+                wt[k][1] = st.M;
+                wt[k][2] = st.Mt;
+                wt[k][3] = one - st.Z - st.W;
+                wt[k][4] = st.Pt;
+                wt[k][5] = st.P;
+            } else if (kd == 2) {
+                T D, E, F;
+                def_row0<T, PAT>(p, cs, r, D, E, F);
+                const T delta = p.delta, ni = p.ni, rho = p.rho;
+                wt[k][0] = -delta * D;
+                wt[k][1] = one - three * ni * E + rho * F;
+                wt[k][2] = delta * D;
+                wt[k][3] = ni * E;
+                wt[k][4] = four * ni * E;
+            } else if (kd == 3) {
+                wt[k][0] = bs[r];  // v_upper(t, X(zeta_r))
+            }
+        }
+        if (tid < ny) {
+            lo_j = bs[nx + tid];  // h_lower(t, Y(eta_j))
+            hup_j = two * p.h1 * bs[nx + ny + tid];
+            jj = (int)p.idx2[tid];
+        }
+    }
+    __syncthreads();
+    for (int s = 0; s < steps; ++s) {
+        const T *u = lv[s & 1];
+        T *w = lv[(s & 1) ^ 1];
+#pragma unroll
+        for (int k = 0; k < ERES_K; ++k) {
+            const int kd = pt[k] >> 16, o = pt[k] & 0xffff;
+            if (kd == 1) {
+                const T C = wt[k][0];
+                w[o] = C * u[o - ny - 1] + wt[k][1] * u[o - ny] - C * u[o - ny + 1] + wt[k][2] * u[o - 1] +
+                       wt[k][3] * u[o] + wt[k][4] * u[o + 1] - C * u[o + ny - 1] + wt[k][5] * u[o + ny] +
+                       C * u[o + ny + 1];
+            } else if (kd == 2) {
+                w[o] = (wt[k][0] * u[o - ny]) + (wt[k][1] * u[o]) + (wt[k][2] * u[o + ny]) - (wt[k][3] * u[o + 2]) +
+                       (wt[k][4] * u[o + 1]);
+            } else if (kd == 3) {
+                w[o] = wt[k][0];
+            }
+        }
+        __syncthreads();
+        if (tid < ny) {
+            const T e = (((four * w[(nx - 2) * ny + jj]) - w[(nx - 3) * ny + jj] - hup_j) / three);
+            w[tid] = lo_j;
+            w[(nx - 1) * ny + tid] = e;
+        }
+        __syncthreads();
+    }
+    const gptr<T> out = G((steps & 1) ? p.b : p.a);
+    for (int q = tid; q < n; q += ERES_T)
+        out[q] = lv[steps & 1][q];
+}
+
+// -------------------------------------------------------------------------------------------------
 // Batched tridiagonal systems (one lane per system), the reference's Thomas LU in its operation
 // order with the end rows folded by `bc` (LineBC; PLAIN ignores a[0] and c[n-1]).
 // Layout: contiguous x[k*n + i] (strided == 0) or interleaved x[i*B + k] (strided == 1).
@@ -433,6 +526,9 @@ __global__ void __launch_bounds__(256) k_tridiag_lane(int64_t B, int n, int stri
     template __global__ void k_xsweep<T>(const Prob<T> *, const Tile *, int, int);                            \
     template __global__ void k_vsweep<T>(const Prob<T> *, const Tile *, int, int);                            \
     template __global__ void k_hv_y3<T>(const Prob<T> *, const Tile *, int);                                  \
+    template __global__ void k_euler_res<T, PAT_GENERIC>(const Prob<T> *, const Tile *);                      \
+    template __global__ void k_euler_res<T, PAT_HESTON>(const Prob<T> *, const Tile *);                       \
+    template __global__ void k_euler_res<T, PAT_SABR0>(const Prob<T> *, const Tile *);                        \
     template __global__ void k_euler<T, PAT_GENERIC>(const Prob<T> *, const Tile *, int);                     \
     template __global__ void k_euler<T, PAT_HESTON>(const Prob<T> *, const Tile *, int);                      \
     template __global__ void k_euler<T, PAT_SABR0>(const Prob<T> *, const Tile *, int);                       \

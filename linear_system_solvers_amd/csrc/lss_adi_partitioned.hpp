@@ -7,6 +7,7 @@
 //                   shuffles.
 //   k_xsweep_p<W>:  a workgroup covers W adjacent S-lines (W = 16: one 128-byte row segment per
 //                   access) and 64 row chunks per line; the interface system is reduced through LDS.
+//                   The stage-1 sweep also evaluates boundary stage A (one launch less per step).
 //                   The HV stage-2 sweep evaluates the explicit Y3 stage pointwise inside its
 //                   right-hand side, so Y3 is never stored.
 // W and VR are the tile/lane mappings the plan's autotuner chooses between; they do not change the
@@ -241,9 +242,14 @@ __global__ void __launch_bounds__(W *NC) k_xsweep_p(const Prob<T> *__restrict__ 
         __syncthreads();
     }
     const lptr<const T> xs = (lptr<const T>)xsh;
-    if constexpr (STAGE == 1)
+    if constexpr (STAGE == 1) {
         xsweep_lines<T, M, W, 0, PAT, NC>(sh, p, tl, step, xs);
-    else if (p.scheme == SCHEME_CS)
+        // boundary stage A of the step, its rows spread over the problem's S-line tiles (the v-sweeps
+        // of the step read colA0 and colA1; this launch does not)
+        const int ntl = (p.ny - 2 + W - 1) / W, k = (tl.first - 1) / W;
+        for (int i = k * (W * NC) + (int)threadIdx.x; i < p.nx; i += ntl * (W * NC))
+            bnd_a_row<T, PAT>(p, step, i);
+    } else if (p.scheme == SCHEME_CS)
         xsweep_lines<T, M, W, 1, PAT, NC>(sh, p, tl, step, xs);
     else if (p.scheme == SCHEME_MCS)
         xsweep_lines<T, M, W, 2, PAT, NC>(sh, p, tl, step, xs);

@@ -152,6 +152,37 @@ template <typename T> __device__ __forceinline__ gptr<const T> bnd_set(const Pro
     return G(p.bnd + (p.bnd_sets > 1 ? (int64_t)step : 0) * p.bnd_stride);
 }
 
+// Boundary stage A of step `step` at row i (explicit_boundary.hpp:39-84, :181-200): colA0[i], the new
+// v = 0 value from the explicit degenerate row, and colA1[i], the v_max value.  It reads the old
+// level only; the v-sweeps of the step read what it writes.
+template <typename T, int PAT = PAT_GENERIC> __device__ __forceinline__ void bnd_a_row(const Prob<T> &p, int step, int i)
+{
+    const int nx = p.nx, ny = p.ny, N = nx - 1;
+    const gptr<const T> u = G((const T *)((step & 1) ? p.b : p.a));
+    const gptr<const T> cs = coeff_set(p, step);
+    const gptr<const T> bs = bnd_set(p, step);
+    const T four = (T)4.0, three = (T)3.0, two = (T)2.0, one = (T)1.0;
+    const T delta = p.delta, ni = p.ni, rho = p.rho;
+    T v;
+    if (i == 0) {
+        v = bs[nx];  // h_lower(t, Y(0))
+    } else {
+        T D, E, F;
+        def_row0<T, PAT>(p, cs, i, D, E, F);
+        const gptr<const T> ui = u + (int64_t)i * ny;
+        if (i == N) {
+            const T delta_ = two * p.h1 * bs[nx + ny];  // 2 h_1 * h_upper(t, Y(0)) X'(1)
+            v = ((one - three * ni * E + rho * F) * ui[0]) + (four * ni * E * ui[1]) - (ni * E * ui[2]) -
+                (delta * delta_ * D);
+        } else {
+            v = (-delta * D * ui[-ny]) + ((one - three * ni * E + rho * F) * ui[0]) + (delta * D * ui[ny]) -
+                (ni * E * ui[2]) + (four * ni * E * ui[1]);
+        }
+    }
+    G(p.colA0)[i] = v;
+    G(p.colA1)[i] = bs[i];  // v_upper(t, X(zeta_i))
+}
+
 // Two-dimensional point accessors: in(di, dj) is the value at (i + di, j + dj) of the point (i, j).
 // PAt: a pointer at the point and the row stride.  GRC: a global base, the point's row k and column
 // j; every access forms the offset of its row, min(k + di, kmax) * ny, so that neighbouring points

@@ -88,3 +88,46 @@ def test_explicit_and_adi_problems_do_not_share_a_plan_and_explicit_plans_have_n
     o = torch.empty_like(u0)
     with pytest.raises(L.LssError):
         plan.forward([u0.data_ptr()], [o.data_ptr()], ckpt_every=4)
+
+
+@pytest.mark.parametrize("fp", [L.FP64, L.FP32])
+@pytest.mark.parametrize("mode", [L.COEFF_FIELDS, L.COEFF_SEPARABLE])
+def test_resident_loop_equals_the_step_by_step_kernel(oracle, fp, mode):
+    """A plan of small explicit problems with time-independent data runs the whole loop in one launch,
+    the time levels resident in LDS; lss_adi_run_levels steps the same plan launch by launch.  Both
+    give the same bits, at the residency limits (1536 points, ny = 256, nx = 4) too, and equal the
+    oracle (bit for bit with FIELDS)."""
+    base = model_from_case(EXPLICIT[0])
+    dt = base.t_hi / (base.nt - 1)
+    for nx, ny, nt in [(50, 30, 401), (4, 30, 101), (6, 256, 21), (48, 32, 301), (13, 7, 1000)]:
+        m = model_from_case(EXPLICIT[0], nx=nx, ny=ny, nt=nt, coeff_mode=mode)
+        m.t_hi = m.t_lo + dt * (nt - 1)
+        plan = L.Plan.from_models([m], fp)
+        assert plan.kernel_config()["explicit_resident"] == 1
+        g = plan.run_host()[0]
+        lv = plan.run_levels_host()[0]
+        plan.close()
+        same = np.array_equal(g, lv[-1], equal_nan=True)
+        print(f"explicit {nx}x{ny}x{nt - 1} fp {fp} mode {mode}: resident equals step-by-step {same}", flush=True)
+        assert same
+        if (nx, ny) in ((50, 30), (13, 7)):
+            c = oracle_solve(oracle, m, "f64" if fp == L.FP64 else "f32")
+            assert np.isfinite(c).all()
+            if mode == L.COEFF_FIELDS:
+                assert np.array_equal(g, c)
+            else:
+                assert np.abs(g - c).max() <= (1e-12 if fp == L.FP64 else 1e-5) * np.abs(c).max()
+
+
+def test_plans_outside_the_residency_limits_step_launch_by_launch(oracle):
+    """More than 1536 points, or coefficients sampled per step (SABR's D(t)), keep the per-step kernel."""
+    big = model_from_case(EXPLICIT[0], nx=40, ny=40, nt=51)
+    big.t_hi = big.t_hi * 50 / 7999
+    sabr = [model_from_case(c, nt=51) for c in EXPLICIT if c["model"][0] == "sabr"][0]
+    sabr.t_hi = sabr.t_hi * 50 / 7999
+    for m in (big, sabr):
+        plan = L.Plan.from_models([m])
+        assert plan.kernel_config()["explicit_resident"] == 0
+        g = plan.run_host()[0]
+        plan.close()
+        assert np.array_equal(g, oracle_solve(oracle, m))
