@@ -22,6 +22,8 @@
 
 namespace lssamd {
 
+constexpr int LB = 8;  // points per batch of the exact line sweeps
+
 // -------------------------------------------------------------------------------------------------
 // Boundary stage A: colA0 = new v = 0 column, colA1 = v_max column (explicit_boundary.hpp:39-84, :181-200)
 // as a launch of its own (plans with exact lines; the partitioned stage-1 S-sweep evaluates it itself)
@@ -131,36 +133,70 @@ __global__ void __launch_bounds__(TILE) k_xsweep(const Prob<T> *__restrict__ pro
     auto at = [&](gptr<const T> b, int64_t q) { return PAt<gptr<const T>>{b + q, ny}; };
     const T *sp = kind == 0 ? src_set(p, step) : nullptr;
     const int64_t nxy = (int64_t)nx * ny;
+    // The sweeps run in batches of LB points: the loads of a batch are issued together before its
+    // stores (the arrays read and written are distinct), so a lane waits once per batch, not per point.
     T gprev = (T)0, sprev = (T)0;
-    for (int t = 1; t <= end; ++t) {
-        const int64_t q = (int64_t)t * ny + j;
-        const Stencil<T> s = stencil_at(p, cs, t, j);
-        const T low = (-theta * s.M), diag = (one + theta * s.Z), high = (-theta * s.P);
-        T r;
-        if (kind == 0) {
-            r = rhs_intermed_1(p, s, at(in, q));
-            // rhs_intermed_1_source (lss_heat_douglas_rachford_method.hpp:70-104)
-            if (sp)
-                r = r + (theta * p.rho * sp[nxy + q]) + ((one - theta) * p.rho * sp[q]);
-        } else if (kind == 1)
-            r = rhs_intermed_3<T, false>(p, s, at(in, q), at(inh, q), at(inhn, q));
-        else if (kind == 2)
-            r = rhs_intermed_3<T, true>(p, s, at(in, q), at(inh, q), at(inhn, q));
-        else
-            r = rhs_intermed_4(p, s, at(in, q), at(inh, q));
-        T g, sv;
-        fwd_step(tri, t == 1, t == end, low, diag, high, r, lo, hi, gprev, sprev, g, sv);
-        scr[q] = g;
-        out[q] = sv;
-        gprev = g;
-        sprev = sv;
+    for (int t0 = 1; t0 <= end; t0 += LB) {
+        T lw[LB], dg[LB], hg[LB], rr[LB];
+#pragma unroll
+        for (int k = 0; k < LB; ++k) {
+            const int t = t0 + k;
+            if (t > end)
+                break;
+            const int64_t q = (int64_t)t * ny + j;
+            const Stencil<T> s = stencil_at(p, cs, t, j);
+            lw[k] = (-theta * s.M);
+            dg[k] = (one + theta * s.Z);
+            hg[k] = (-theta * s.P);
+            T r;
+            if (kind == 0) {
+                r = rhs_intermed_1(p, s, at(in, q));
+                // rhs_intermed_1_source (lss_heat_douglas_rachford_method.hpp:70-104)
+                if (sp)
+                    r = r + (theta * p.rho * sp[nxy + q]) + ((one - theta) * p.rho * sp[q]);
+            } else if (kind == 1)
+                r = rhs_intermed_3<T, false>(p, s, at(in, q), at(inh, q), at(inhn, q));
+            else if (kind == 2)
+                r = rhs_intermed_3<T, true>(p, s, at(in, q), at(inh, q), at(inhn, q));
+            else
+                r = rhs_intermed_4(p, s, at(in, q), at(inh, q));
+            rr[k] = r;
+        }
+#pragma unroll
+        for (int k = 0; k < LB; ++k) {
+            const int t = t0 + k;
+            if (t > end)
+                break;
+            const int64_t q = (int64_t)t * ny + j;
+            T g, sv;
+            fwd_step(tri, t == 1, t == end, lw[k], dg[k], hg[k], rr[k], lo, hi, gprev, sprev, g, sv);
+            scr[q] = g;
+            out[q] = sv;
+            gprev = g;
+            sprev = sv;
+        }
     }
     T x = (tri == TRI_THOMAS) ? sprev : bwd_step(tri, gprev, sprev, hi);
     out[(int64_t)end * ny + j] = x;
-    for (int t = end - 1; t >= 1; --t) {
-        const int64_t q = (int64_t)t * ny + j;
-        x = bwd_step(tri, scr[q], out[q], x);
-        out[q] = x;
+    for (int t0 = end - 1; t0 >= 1; t0 -= LB) {
+        T gg[LB], ss[LB];
+#pragma unroll
+        for (int k = 0; k < LB; ++k) {
+            const int t = t0 - k;
+            if (t < 1)
+                break;
+            const int64_t q = (int64_t)t * ny + j;
+            gg[k] = scr[q];
+            ss[k] = out[q];
+        }
+#pragma unroll
+        for (int k = 0; k < LB; ++k) {
+            const int t = t0 - k;
+            if (t < 1)
+                break;
+            x = bwd_step(tri, gg[k], ss[k], x);
+            out[(int64_t)t * ny + j] = x;
+        }
     }
     out[j] = lo;
     out[(int64_t)(nx - 1) * ny + j] = hi;
@@ -208,23 +244,54 @@ __global__ void __launch_bounds__(TILE) k_vsweep(const Prob<T> *__restrict__ pro
     const gptr<const T> in = G(in0 + row);
     const gptr<const T> inh = G(inh0 + row);
     const gptr<T> out = G(out0 + row);
+    // (batches of LB points, as in k_xsweep)
     T gprev = (T)0, sprev = (T)0;
-    for (int t = 1; t <= end; ++t) {
-        const Stencil<T> s = stencil_at(p, cs, i, t);
-        const T low = (-theta * s.Mt), diag = (one + theta * s.W), high = (-theta * s.Pt);
-        const T r = rhs_v(p, s, in + t, inh + t);
-        T g, sv;
-        fwd_step(tri, t == 1, t == end, low, diag, high, r, lo, hi, gprev, sprev, g, sv);
-        scr[t] = g;
-        out[t] = sv;
-        gprev = g;
-        sprev = sv;
+    for (int t0 = 1; t0 <= end; t0 += LB) {
+        T lw[LB], dg[LB], hg[LB], rr[LB];
+#pragma unroll
+        for (int k = 0; k < LB; ++k) {
+            const int t = t0 + k;
+            if (t > end)
+                break;
+            const Stencil<T> s = stencil_at(p, cs, i, t);
+            lw[k] = (-theta * s.Mt);
+            dg[k] = (one + theta * s.W);
+            hg[k] = (-theta * s.Pt);
+            rr[k] = rhs_v(p, s, in + t, inh + t);
+        }
+#pragma unroll
+        for (int k = 0; k < LB; ++k) {
+            const int t = t0 + k;
+            if (t > end)
+                break;
+            T g, sv;
+            fwd_step(tri, t == 1, t == end, lw[k], dg[k], hg[k], rr[k], lo, hi, gprev, sprev, g, sv);
+            scr[t] = g;
+            out[t] = sv;
+            gprev = g;
+            sprev = sv;
+        }
     }
     T x = (tri == TRI_THOMAS) ? sprev : bwd_step(tri, gprev, sprev, hi);
     out[end] = x;
-    for (int t = end - 1; t >= 1; --t) {
-        x = bwd_step(tri, scr[t], out[t], x);
-        out[t] = x;
+    for (int t0 = end - 1; t0 >= 1; t0 -= LB) {
+        T gg[LB], ss[LB];
+#pragma unroll
+        for (int k = 0; k < LB; ++k) {
+            const int t = t0 - k;
+            if (t < 1)
+                break;
+            gg[k] = scr[t];
+            ss[k] = out[t];
+        }
+#pragma unroll
+        for (int k = 0; k < LB; ++k) {
+            const int t = t0 - k;
+            if (t < 1)
+                break;
+            x = bwd_step(tri, gg[k], ss[k], x);
+            out[t] = x;
+        }
     }
     out[0] = lo;
     out[ny - 1] = hi;
