@@ -302,18 +302,23 @@ def check_all(models, outs, log, exact_outs=None, threads: int = 0, tol: float =
 
 
 def exact_book(torch, log):
-    """The book through the exact path (coefficient fields, sequential Thomas lines): its surfaces."""
+    """The book through the exact path (coefficient fields, sequential Thomas lines): its surfaces, and
+    the time of one graph-replayed solve (the throughput cost of the reference's operation order)."""
     models = book_models(2048, coeff_mode=L.COEFF_FIELDS, line_solver=L.LINE_EXACT)
     t0 = time.time()
     plan = L.Plan.from_models(models, L.FP64)
     ins = [torch.from_numpy(plan.initial(p)).cuda() for p in range(plan.count)]
     outs = [torch.empty_like(u) for u in ins]
-    plan.run_dev([u.data_ptr() for u in ins], [u.data_ptr() for u in outs], torch.cuda.current_stream().cuda_stream,
-                 graph=False)
+    pin, pout = [u.data_ptr() for u in ins], [u.data_ptr() for u in outs]
+    stream = torch.cuda.current_stream().cuda_stream
+    plan.run_dev(pin, pout, stream, graph=True)
+    torch.cuda.synchronize()
+    secs = time_stream(torch, lambda: plan.run_dev(pin, pout, stream, graph=True), 1)
     res = [o.cpu().numpy() for o in outs]
     plan.close()
-    log(f"exact-path book solve (plan, solve, copy back) {time.time() - t0:.1f}s")
-    return res
+    log(f"exact-path book: {secs * 1e3:.1f} ms per solve ({point_steps(models) / secs:.3e} grid-point-steps/s); "
+        f"plan, two solves and copy back {time.time() - t0:.1f}s")
+    return res, secs
 
 
 def host_threads() -> int:
@@ -725,7 +730,10 @@ def main():
     }
     failed = 0
     if not args.no_check_all:
-        ca = check_all(models, [o.cpu().numpy() for o in outs], log, exact_outs=exact_book(torch, log))
+        ex_outs, ex_secs = exact_book(torch, log)
+        ca = check_all(models, [o.cpu().numpy() for o in outs], log, exact_outs=ex_outs)
+        ca["exact_path_book_ms"] = round(ex_secs * 1e3, 1)
+        ca["exact_path_over_fast_path_time"] = round(ex_secs / (ms_per_step / 1e3), 2)
         result["check_all"] = ca
         S, U = ca["stable"], ca["unstable"]
         within = S["within_tolerance"] + U["within_tolerance"]
