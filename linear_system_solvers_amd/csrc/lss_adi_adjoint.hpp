@@ -567,22 +567,23 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
 }
 
 // -------------------------------------------------------------------------------------------------
-// Gathers over two-dimensional tiles of GTR x GTC points (one thread per point).  Every equation point
-// e of the (GTR + 2) x (GTC + 2) points around the tile first forms, once, the nine contributions it
-// makes to the lambdas of the values it reads, c_e(oi, oj) for the value at e + (oi, oj) (the weight
-// of that value in each of e's equations times the equation's lambda), and leaves them in LDS; each
-// point of the tile then sums the nine contributions addressed to it.
+// Gathers over two-dimensional tiles of AGR x AGC points (one thread per point of the tile and its
+// halo ring).  Every equation point e of the (AGR + 2) x (AGC + 2) points around the tile first
+// forms, once, the nine contributions it makes to the lambdas of the values it reads, c_e(oi, oj) for
+// the value at e + (oi, oj) (the weight of that value in each of e's equations times the equation's
+// lambda), and leaves them in LDS; each point of the tile then sums the nine contributions addressed
+// to it.
 //   WHAT 0: lambda(U_s) from every equation that reads U_s (the stencils of the right-hand sides,
 //           the Dirichlet ends of the S-lines through idx2, boundary stage A), and the parameter
 //           terms of boundary stage A (its D, E, F on the v = 0 row)
 //   WHAT 1: lambda(Y2) (CS, MCS, HV), read by the adjoint of V1
 // -------------------------------------------------------------------------------------------------
 template <typename T, int WHAT, int PAT>
-__global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
+__global__ void __launch_bounds__(AGT) k_adj_g(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
                                                     const Tile *__restrict__ tiles, int step, double *__restrict__ slots)
 {
-    __shared__ T cst[9][GTR + 2][GTC + 2];  // c_e(oi, oj) at [(oi + 1) * 3 + oj + 1]
-    __shared__ double red[GTR * GTC / 64][NPAR];
+    __shared__ T cst[9][AGR + 2][AGC + 2];  // c_e(oi, oj) at [(oi + 1) * 3 + oj + 1]
+    __shared__ double red[AGT / 64][NPAR];
     const Tile tl = tiles[blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
     const Adj<T> &A = adjs[tl.prob];
@@ -606,8 +607,8 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
     if (run) {
         const gptr<const T> lv2 = G((const T *)A.lv2), lv1 = G((const T *)A.lv1), lx2 = G((const T *)A.lx2),
                             lx1 = G((const T *)A.lx1);
-        for (int idx = threadIdx.x; idx < (GTR + 2) * (GTC + 2); idx += GTR * GTC) {
-            const int lr = idx / (GTC + 2), lc = idx % (GTC + 2);
+        do {  // one thread per point of the tile and its halo ring (continue: no equation point)
+            const int lr = threadIdx.x / (AGC + 2), lc = threadIdx.x % (AGC + 2);
             const int ei = r0 - 1 + lr, ej = c0 - 1 + lc;
             if (!inside(p, ei, ej)) {  // not an equation point: no contributions
 #pragma unroll
@@ -659,12 +660,12 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
                     }
                 }
             });
-        }
+        } while (false);
     }
     __syncthreads();
-    const int tr = threadIdx.x / GTC, tc = threadIdx.x % GTC;
+    const int tr = threadIdx.x / AGC, tc = threadIdx.x % AGC;
     const int r = r0 + tr, c = c0 + tc;
-    if (run && tr < tl.count && c < ny) {
+    if (run && (int)threadIdx.x < AGR * AGC && tr < tl.count && c < ny) {
         // the contributions addressed to (r, c): from e = (r + di, c + dj) at offset (-di, -dj)
         T v = (T)0;
 #pragma unroll
@@ -753,7 +754,7 @@ __global__ void __launch_bounds__(GTR *GTC) k_adj_g(const Prob<T> *__restrict__ 
     }
     // (only the tiles of column 0 hold parameter terms: boundary stage A's D, E, F on the v = 0 row)
     if (WHAT == 0 && c0 == 0)
-        block_acc_store<GTR * GTC / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
+        block_acc_store<AGT / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
 
 // -------------------------------------------------------------------------------------------------

@@ -458,9 +458,9 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
                 default: lo = 1, hi = p.nx - 1, step = 1; break;
                 }
                 if (tb == TB_U || tb == TB_E) {  // two-dimensional tiles (adjoint gathers, explicit step)
-                    const int rows = tb == TB_U ? GTR : GTR * ERPT;
+                    const int rows = tb == TB_U ? AGR : GTR * ERPT, cols = tb == TB_U ? AGC : GTC;
                     for (int r = 0; r < p.nx; r += rows)
-                        for (int c = 0; c < p.ny; c += GTC)
+                        for (int c = 0; c < p.ny; c += cols)
                             tiles[tb].push_back({(int32_t)q, r, std::min(rows, p.nx - r), c});
                 } else {
                     for (int s = lo; s < hi; s += step) {
@@ -1236,11 +1236,11 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         const Adj<T> *ad = (const Adj<T> *)adj_dev.p;
         const Tile *tu = (const Tile *)tiles_dev.p + tile_off[TB_U];
         if (apat == PAT_HESTON)
-            hipLaunchKernelGGL((k_adj_g<T, WHAT, PAT_HESTON>), dim3(grid), dim3(GTR * GTC), 0, st, table, ad, tu, s, slot);
+            hipLaunchKernelGGL((k_adj_g<T, WHAT, PAT_HESTON>), dim3(grid), dim3(AGT), 0, st, table, ad, tu, s, slot);
         else if (apat == PAT_SABR0)
-            hipLaunchKernelGGL((k_adj_g<T, WHAT, PAT_SABR0>), dim3(grid), dim3(GTR * GTC), 0, st, table, ad, tu, s, slot);
+            hipLaunchKernelGGL((k_adj_g<T, WHAT, PAT_SABR0>), dim3(grid), dim3(AGT), 0, st, table, ad, tu, s, slot);
         else
-            hipLaunchKernelGGL((k_adj_g<T, WHAT, PAT_GENERIC>), dim3(grid), dim3(GTR * GTC), 0, st, table, ad, tu, s,
+            hipLaunchKernelGGL((k_adj_g<T, WHAT, PAT_GENERIC>), dim3(grid), dim3(AGT), 0, st, table, ad, tu, s,
                                slot);
     }
     void adj_v_m(int s, int last, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)

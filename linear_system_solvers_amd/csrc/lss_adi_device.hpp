@@ -123,7 +123,10 @@ struct Tile {
 // Reverse mode (adjoint) of the time loop
 // -------------------------------------------------------------------------------------------------
 constexpr int NPAR = 6;    // model parameters per problem
-constexpr int GTR = 8, GTC = 32;  // gather tiles of the adjoint: rows x columns (one thread per point)
+constexpr int GTR = 8, GTC = 32;  // explicit step: GTR x GTC threads, each ERPT rows of one column
+// gather tiles of the adjoint: AGR x AGC points; one thread per point of the tile and its halo ring
+// (AGT = (AGR + 2) x (AGC + 2) = 512), so that every equation point is formed in one pass
+constexpr int AGR = 14, AGC = 30, AGT = (AGR + 2) * (AGC + 2);
 constexpr int ERPT = 8;           // rows per thread of the explicit step (its tiles: GTR * ERPT rows)
 constexpr int ERES_T = 256, ERES_K = 6;  // threads and points per thread of the resident explicit loop
 
