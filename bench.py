@@ -482,6 +482,110 @@ def explicit_models(count=1024, seed=5, coeff_mode=L.COEFF_SEPARABLE):
     return models
 
 
+def bench_heat1d(torch, log, count=16384, n=256, nt=1001, oracle_check=2):
+    """1-D Black-Scholes book (the reference's one-dimensional implicit solver): count calls with the
+    strike, volatility and rate drawn per problem, n-point uniform grids, nt - 1 Crank-Nicolson steps,
+    each problem's whole loop in one launch.  Exact order (one lane per problem) and partitioned (one
+    wavefront per problem, the loop in registers); the exact path is checked against the oracle, the
+    partitioned against the exact path."""
+    from oracle import oracle as O
+    from linear_system_solvers_amd import heat1d as H
+    rng = np.random.default_rng(6)
+    draws = [(float(rng.uniform(8.0, 12.0)), float(rng.uniform(0.01, 0.1)), float(rng.uniform(0.15, 0.4)))
+             for _ in range(count)]
+    stream = torch.cuda.current_stream().cuda_stream
+    ps = count * n * (nt - 1)
+    res = {"workload": f"black_scholes_1d_{count}x{n}x{nt - 1}_cn", "grid_point_steps_per_solve": ps}
+    outs = {}
+    for ls, name in ((L.LINE_EXACT, "exact"), (L.LINE_PARTITIONED, "partitioned")):
+        models = [H.BsModel(strike=K, rate=r, sig=sg, n=n, nt=nt, line_solver=ls) for K, r, sg in draws]
+        plan = H.Heat1dPlan.from_bs_models(models, L.FP64)
+        u0 = torch.from_numpy(np.stack([plan.initial(p) for p in range(count)])).cuda()
+        out = torch.empty_like(u0)
+        stride = n * u0.element_size()
+        pin = [u0.data_ptr() + p * stride for p in range(count)]
+        pout = [out.data_ptr() + p * stride for p in range(count)]
+        plan.run_dev(pin, pout, stream=stream)
+        torch.cuda.synchronize()
+        secs = time_stream(torch, lambda: plan.run_dev(pin, pout, stream=stream), 3) / 3
+        outs[name] = out.cpu().numpy()
+        res[name] = {"ms": round(secs * 1e3, 3), "grid_point_steps_per_s": ps / secs,
+                     "us_per_step": round(secs / (nt - 1) * 1e6, 3)}
+        log(f"1-D Black-Scholes book ({name}): {count} problems {n}x{nt - 1}, {secs * 1e3:.2f} ms, "
+            f"{ps / secs:.3e} grid-point-steps/s")
+        if name == "exact":
+            same = 0
+            for q in range(oracle_check):
+                K, r, sg = draws[q]
+                cfg = O.Heat1dCfg(x_lo=0.0, x_hi=20.0, n=n, nt=nt, t_lo=0.0, t_hi=1.0, accumulation_point=0.0,
+                                  alpha_scale=3.0, nonuniform=False, implicit_scheme=O.CRANK_NICOLSON,
+                                  tridiagonal=O.THOMAS_LU, direction=O.BACKWARD)
+                o = O.heat1d_solve(O.bs_heat1d_pde(K, r, sg, 1.0, 20.0), cfg, "f64")
+                same += int(np.array_equal(outs[name][q], o))
+            res["exact_bit_identical_to_oracle"] = f"{same}/{oracle_check}"
+        plan.close()
+    a, b = outs["partitioned"], outs["exact"]
+    res["partitioned_vs_exact_max_normwise"] = float((np.abs(a - b).max(axis=1) / np.abs(b).max(axis=1)).max())
+    log(f"1-D Black-Scholes book: exact path bit-identical to the oracle on {res['exact_bit_identical_to_oracle']} "
+        f"checked; partitioned vs exact max normwise {res['partitioned_vs_exact_max_normwise']:.2e}")
+    return res
+
+
+def bench_levels(torch, log, count=64):
+    """Every time level (solve(container_3d&)): a book of count configs[0]-shaped problems (strike drawn
+    per problem) written level by level into device surfaces, against the same plan's final-level solve."""
+    from tests.helpers import config0
+    rng = np.random.default_rng(8)
+    models = [config0(strike=float(rng.uniform(80.0, 120.0)), coeff_mode=L.COEFF_SEPARABLE,
+                      line_solver=L.LINE_PARTITIONED) for _ in range(count)]
+    plan = L.Plan.from_models(models, L.FP64)
+    stream = torch.cuda.current_stream().cuda_stream
+    u0 = torch.from_numpy(np.stack([plan.initial(p) for p in range(count)])).cuda()
+    nx, ny, st = plan.shape(0)
+    out = torch.empty_like(u0)
+    lv = torch.empty((count, st + 1, nx, ny), dtype=u0.dtype, device=u0.device)
+    es = u0.element_size()
+    pin = [u0.data_ptr() + p * nx * ny * es for p in range(count)]
+    pout = [out.data_ptr() + p * nx * ny * es for p in range(count)]
+    plv = [lv.data_ptr() + p * (st + 1) * nx * ny * es for p in range(count)]
+    plan.run_dev(pin, pout, stream, graph=False)
+    plan.run_levels_dev(pin, plv, stream)
+    torch.cuda.synchronize()
+    t_final = time_stream(torch, lambda: plan.run_dev(pin, pout, stream, graph=False), 3) / 3
+    t_levels = time_stream(torch, lambda: plan.run_levels_dev(pin, plv, stream), 3) / 3
+    same = bool(torch.equal(lv[:, -1], out))
+    level_bytes = count * (st + 1) * nx * ny * es
+    plan.close()
+    log(f"every level: {count} problems {nx}x{ny}x{st}, levels {t_levels * 1e3:.2f} ms against final level only "
+        f"{t_final * 1e3:.2f} ms (launch by launch); {level_bytes / 1e9:.2f} GB of levels at "
+        f"{level_bytes / t_levels / 1e9:.0f} GB/s of solve time; last level equals the final-level solve: {same}")
+    return {"workload": f"heston_dr_{count}x{nx}x{ny}x{st}_every_level", "levels_ms": round(t_levels * 1e3, 3),
+            "final_level_ms": round(t_final * 1e3, 3), "level_bytes": level_bytes,
+            "last_level_equals_final": same}
+
+
+def bench_source(torch, log):
+    """Heat-source ADI (rhs_intermed_1_source of DR/CS/MCS/HV) through the C++ drop-in: a partitioned
+    batched_heston_book of 64 problems (100 x 50 points, 100 steps, the four schemes in turn) solved
+    from host containers without and with the source F = 0.05 v S (1 + t) (tests/cpp/dropin_extras.cpp,
+    source_bench); the source is sampled per step, so its plan reads two more surfaces per stage-1
+    point."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "cpp", "dropin_extras")
+    r = subprocess.run([exe, "source_bench"], capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"dropin_extras source_bench failed: {r.stderr[-400:]}")
+    res = {"workload": "heston_source_book_64x100x50x100_cpp_dropin"}
+    for ln in r.stdout.splitlines():
+        f = ln.split()
+        if len(f) == 4 and f[0] == "source_bench":
+            ms, ps = float(f[2]), float(f[3])
+            res[f[1]] = {"ms": round(ms, 3), "grid_point_steps_per_s": ps / (ms / 1e3)}
+    log(f"heat-source book (C++ drop-in, host containers): without source {res['without']['ms']:.2f} ms, with "
+        f"source {res['with']['ms']:.2f} ms ({res['with']['grid_point_steps_per_s']:.3e} grid-point-steps/s)")
+    return res
+
+
 def bench_explicit(torch, log, count=1024, oracle_check=2):
     """Explicit Euler book: 1024 problems of 50 x 30 points, each problem's whole loop in one workgroup
     with its time levels resident in LDS (one launch per solve); the step-by-step kernel (every level
@@ -632,7 +736,8 @@ def main():
     ap.add_argument("--counters", default="",
                     help="counter-collection run: S,V,X mapping (e.g. 8,4,1) set instead of autotuned; the book is "
                          "solved --steps times with no events and no timing (hipEvents are not used under --pmc)")
-    ap.add_argument("--only", choices=("adjoint", "sabr", "tridiag", "explicit"), default="",
+    ap.add_argument("--only", choices=("adjoint", "sabr", "tridiag", "explicit", "heat1d", "levels", "source"),
+                    default="",
                     help="run only this secondary workload (no book, no JSON bench line)")
     ap.add_argument("--no-check-all", action="store_true",
                     help="skip checking every book problem (fast and exact GPU paths) against the oracle on host "
@@ -668,7 +773,8 @@ def main():
         del x, y, a, b, c
     if args.only:  # one secondary workload alone (for kernel traces of it)
         fn = {"adjoint": bench_adjoint, "sabr": bench_sabr, "tridiag": bench_tridiag,
-              "explicit": bench_explicit}[args.only]
+              "explicit": bench_explicit, "heat1d": bench_heat1d, "levels": bench_levels,
+              "source": bench_source}[args.only]
         print(json.dumps({"only": args.only, "result": fn(torch, log)}), flush=True)
         return
     models = book_models(args.count)
@@ -768,6 +874,9 @@ def main():
         result["adjoint_over_forward"] = result["adjoint"]["ratio_gradient_over_forward"]
         result["sabr_book"] = bench_sabr(torch, log)
         result["explicit_euler_book"] = bench_explicit(torch, log)
+        result["heat1d_book"] = bench_heat1d(torch, log)
+        result["every_level"] = bench_levels(torch, log)
+        result["heat_source_book"] = bench_source(torch, log)
     if not args.no_cpu:
         cb = cpu_baseline(models, log)
         r1, n1, p1 = cb["one"]

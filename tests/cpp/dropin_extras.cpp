@@ -17,7 +17,11 @@
 //                          final surface (full precision)
 //   source_book            the four schemes with that source, solved one by one and as one partitioned
 //                          batched_heston_book; prints the largest relative difference of each
+//   source_bench           a partitioned batched_heston_book of 64 problems (100 x 50 points, 100 steps, the
+//                          four schemes in turn, strikes 80..120) without and with that source; prints
+//                          "source_bench <without|with> <best solve ms of 5> <grid-point-steps per solve>"
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <iomanip>
@@ -229,10 +233,35 @@ int source_book()
     return 0;
 }
 
+int source_bench()
+{
+    typedef container_2d<by_enum::Row, double, std::vector, std::allocator<double>> c2_t;
+    for (const bool with : {false, true}) {
+        std::vector<std::shared_ptr<general_heston_equation<double>>> eqs;
+        for (int q = 0; q < 64; ++q)
+            eqs.push_back(heston<double>(100, 50, 101, 80.0 + 40.0 * q / 63.0, 2.0, -0.4, SCHEMES[q % 4],
+                                         dflt::host_bwd_tlusolver_cn_solver_config_ptr, with));
+        batched_heston_book<double> b(eqs, false);
+        std::vector<c2_t> out;
+        for (auto &e : eqs)
+            out.emplace_back(e->grid().nx, e->grid().ny);
+        b.solve(out);  // (first solve: autotuning and graph capture)
+        double best = 1e30;
+        for (int r = 0; r < 5; ++r) {
+            const auto t0 = std::chrono::steady_clock::now();
+            b.solve(out);
+            best = std::min(best, std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+        }
+        std::cout << "source_bench " << (with ? "with" : "without") << " " << best << " " << 64 * 100 * 50 * 100
+                  << "\n";
+    }
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) {
-        std::cerr << "usage: dropin_extras <levels double|float | book | adjoint | source_book |\n"
+        std::cerr << "usage: dropin_extras <levels double|float | book | adjoint | source_book | source_bench |\n"
                      "                     source <0-3> <fwd|bwd> <double|float> <tlu|ds>>\n";
         return 2;
     }
@@ -248,6 +277,8 @@ int main(int argc, char **argv)
             return adjoint();
         if (what == "source_book")
             return source_book();
+        if (what == "source_bench")
+            return source_bench();
         if (what == "source" && argc > 5) {
             std::cout << std::setprecision(std::string(argv[4]) == "float" ? 9 : 17);
             const int sch = std::atoi(argv[2]);
