@@ -532,8 +532,9 @@ def bench_heat1d(torch, log, count=16384, n=256, nt=1001, oracle_check=2):
 
 
 def bench_levels(torch, log, count=64):
-    """Every time level (solve(container_3d&)): a book of count configs[0]-shaped problems (strike drawn
-    per problem) written level by level into device surfaces, against the same plan's final-level solve."""
+    """Every time level (solve(container_3d&)): a book of count Heston DR calls on the 100 x 50 uniform grid
+    with 100 steps (strike drawn per problem) written level by level into device surfaces, against the
+    same plan's final-level solve."""
     from tests.helpers import config0
     rng = np.random.default_rng(8)
     models = [config0(strike=float(rng.uniform(80.0, 120.0)), coeff_mode=L.COEFF_SEPARABLE,
