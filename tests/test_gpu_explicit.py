@@ -99,8 +99,9 @@ def test_resident_loop_equals_the_step_by_step_kernel(oracle, fp, mode):
     oracle (bit for bit with FIELDS)."""
     base = model_from_case(EXPLICIT[0])
     dt = base.t_hi / (base.nt - 1)
-    for nx, ny, nt in [(50, 30, 401), (4, 30, 101), (6, 256, 21), (48, 32, 301), (13, 7, 1000)]:
-        m = model_from_case(EXPLICIT[0], nx=nx, ny=ny, nt=nt, coeff_mode=mode)
+    for nx, ny, nt, dirn in [(50, 30, 401, L.BACKWARD), (4, 30, 101, L.BACKWARD), (6, 256, 21, L.BACKWARD),
+                             (48, 32, 301, L.BACKWARD), (13, 7, 1000, L.BACKWARD), (50, 30, 201, L.FORWARD)]:
+        m = model_from_case(EXPLICIT[0], nx=nx, ny=ny, nt=nt, coeff_mode=mode, direction=dirn)
         m.t_hi = m.t_lo + dt * (nt - 1)
         plan = L.Plan.from_models([m], fp)
         assert plan.kernel_config()["explicit_resident"] == 1
@@ -108,9 +109,10 @@ def test_resident_loop_equals_the_step_by_step_kernel(oracle, fp, mode):
         lv = plan.run_levels_host()[0]
         plan.close()
         same = np.array_equal(g, lv[-1], equal_nan=True)
-        print(f"explicit {nx}x{ny}x{nt - 1} fp {fp} mode {mode}: resident equals step-by-step {same}", flush=True)
+        print(f"explicit {nx}x{ny}x{nt - 1} dir {dirn} fp {fp} mode {mode}: resident equals step-by-step {same}",
+              flush=True)
         assert same
-        if (nx, ny) in ((50, 30), (13, 7)):
+        if (nx, ny) in ((50, 30), (13, 7)):  # (both directions of the reference's case)
             c = oracle_solve(oracle, m, "f64" if fp == L.FP64 else "f32")
             assert np.isfinite(c).all()
             if mode == L.COEFF_FIELDS:
