@@ -403,8 +403,8 @@ __global__ void __launch_bounds__(64 * VROWS) k_adj_v(const Prob<T> *__restrict_
 // -------------------------------------------------------------------------------------------------
 // KIND (compile time, so that the loads of neighbouring points are shared and issued together):
 // stage 1 of DR 0, of CS/MCS 1, of HV 2; stage 2 of CS 3, of MCS 4, of HV 5.
-template <typename T, int M, int W, int PAT, int KIND>
-__device__ __forceinline__ void adj_x_lines(T (*sh)[4][NCH][W], const Prob<T> &p, const Adj<T> &A, const Tile &tl,
+template <typename T, int M, int W, int PAT, int KIND, int NC = NCH>
+__device__ __forceinline__ void adj_x_lines(T (*sh)[4][NC][W], const Prob<T> &p, const Adj<T> &A, const Tile &tl,
                                             int step, double *acc, lptr<const T> xsh)
 {
     constexpr bool S2 = KIND >= 3, HV = KIND == 2 || KIND == 5, MCS = KIND == 4;
@@ -510,10 +510,10 @@ __device__ __forceinline__ void adj_x_lines(T (*sh)[4][NCH][W], const Prob<T> &p
         constexpr int QM = KIND == 3 ? (1 << 0) | (1 << 2) | (1 << 3) | (1 << 5) : 63;
         ts.template add<QM>(A, (int)min((uint32_t)k, kmax), j, gq);
     };
-    block_solve<T, M, W, true>(sh, N, col, chn, colok, eq, put);
+    block_solve<T, M, W, true, NC>(sh, N, col, chn, colok, eq, put);
     if (colok) {
         int s0, cnt;
-        chunk_span(N, chn, s0, cnt);
+        chunk_span(N, chn, s0, cnt, NC);
 #pragma unroll
         for (int t = 0; t < M; ++t)
             if (t < cnt)
@@ -522,15 +522,15 @@ __device__ __forceinline__ void adj_x_lines(T (*sh)[4][NCH][W], const Prob<T> &p
     ts.finish(A, 0, j, acc);
 }
 
-template <typename T, int M, int W, int PAT, int STAGE>
-__global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
-                                                  const Tile *__restrict__ tiles, int step, double *__restrict__ slots)
+template <typename T, int M, int W, int PAT, int STAGE, int NC>
+__global__ void __launch_bounds__(W *NC) k_adj_x(const Prob<T> *__restrict__ probs, const Adj<T> *__restrict__ adjs,
+                                                 const Tile *__restrict__ tiles, int step, double *__restrict__ slots)
 {
-    __shared__ T sh[2][4][NCH][W];
-    __shared__ double red[W * NCH / 64][NPAR];
+    __shared__ T sh[2][4][NC][W];
+    __shared__ double red[W * NC / 64][NPAR];
     // X-block factor vectors of a compile-time pattern in LDS (read at every point of an S-line)
     constexpr int NVX = fpat_nvec<PAT>(0);
-    __shared__ T xsh[NVX > 0 ? NVX * (NCH * M + 2) : 1];
+    __shared__ T xsh[NVX > 0 ? NVX * (NC * M + 2) : 1];
     const Tile tl = tiles[blockIdx.x];
     const Prob<T> &p = probs[tl.prob];
     const Adj<T> &A = adjs[tl.prob];
@@ -550,20 +550,20 @@ __global__ void __launch_bounds__(W *NCH) k_adj_x(const Prob<T> *__restrict__ pr
     const lptr<const T> xs = (lptr<const T>)xsh;
     if constexpr (STAGE == 1) {
         if (sch == SCHEME_DR)
-            adj_x_lines<T, M, W, PAT, 0>(sh, p, A, tl, step, acc, xs);
+            adj_x_lines<T, M, W, PAT, 0, NC>(sh, p, A, tl, step, acc, xs);
         else if (sch == SCHEME_HV)
-            adj_x_lines<T, M, W, PAT, 2>(sh, p, A, tl, step, acc, xs);
+            adj_x_lines<T, M, W, PAT, 2, NC>(sh, p, A, tl, step, acc, xs);
         else
-            adj_x_lines<T, M, W, PAT, 1>(sh, p, A, tl, step, acc, xs);
+            adj_x_lines<T, M, W, PAT, 1, NC>(sh, p, A, tl, step, acc, xs);
     } else {
         if (sch == SCHEME_CS)
-            adj_x_lines<T, M, W, PAT, 3>(sh, p, A, tl, step, acc, xs);
+            adj_x_lines<T, M, W, PAT, 3, NC>(sh, p, A, tl, step, acc, xs);
         else if (sch == SCHEME_MCS)
-            adj_x_lines<T, M, W, PAT, 4>(sh, p, A, tl, step, acc, xs);
+            adj_x_lines<T, M, W, PAT, 4, NC>(sh, p, A, tl, step, acc, xs);
         else
-            adj_x_lines<T, M, W, PAT, 5>(sh, p, A, tl, step, acc, xs);
+            adj_x_lines<T, M, W, PAT, 5, NC>(sh, p, A, tl, step, acc, xs);
     }
-    block_acc_store<W * NCH / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
+    block_acc_store<W * NC / 64>(acc, red, slots + (int64_t)blockIdx.x * NPAR);
 }
 
 // -------------------------------------------------------------------------------------------------
@@ -772,17 +772,21 @@ template <typename T> __global__ void __launch_bounds__(256) k_copy_jobs(const C
 }  // namespace lssamd
 
 // Instantiations (one translation unit per precision and pattern): LSS_INST_ADJ(T, PAT).
-#define LSS_INST_AX(T, M, W, PAT)                                                                              \
-    template __global__ void k_adj_x<T, M, W, PAT, 1>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *); \
-    template __global__ void k_adj_x<T, M, W, PAT, 2>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
+#define LSS_INST_AXN(T, M, W, PAT, NC)                                                                         \
+    template __global__ void k_adj_x<T, M, W, PAT, 1, NC>(const Prob<T> *, const Adj<T> *, const Tile *, int,       \
+                                                          double *);                                               \
+    template __global__ void k_adj_x<T, M, W, PAT, 2, NC>(const Prob<T> *, const Adj<T> *, const Tile *, int,       \
+                                                          double *);
+#define LSS_INST_AX(T, M, W, PAT) LSS_INST_AXN(T, M, W, PAT, 64)
 #define LSS_INST_AV(T, M, PAT) \
     template __global__ void k_adj_v<T, M, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
-// S-lines per adjoint S-sweep workgroup: 16 for chunk bounds up to 8, 8 for 16 (the plan's rule)
+// S-lines per adjoint S-sweep workgroup: 16 (64 chunks) for chunk bounds up to 8; lines of more than 512
+// points take 128 chunks of at most 8 points, 8 lines per workgroup (the plan's rule)
 #define LSS_INST_ADJ(T, PAT)                                                                                   \
     namespace lssamd {                                                                                         \
     LSS_INST_AV(T, 1, PAT) LSS_INST_AV(T, 2, PAT) LSS_INST_AV(T, 4, PAT) LSS_INST_AV(T, 8, PAT)                 \
     LSS_INST_AV(T, 16, PAT) LSS_INST_AX(T, 1, 16, PAT) LSS_INST_AX(T, 2, 16, PAT) LSS_INST_AX(T, 4, 16, PAT)    \
-    LSS_INST_AX(T, 8, 16, PAT) LSS_INST_AX(T, 16, 8, PAT)                                                       \
+    LSS_INST_AX(T, 8, 16, PAT) LSS_INST_AXN(T, 8, 8, PAT, 128)                                                  \
     template __global__ void k_adj_g<T, 0, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);       \
     template __global__ void k_adj_g<T, 1, PAT>(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);       \
     }

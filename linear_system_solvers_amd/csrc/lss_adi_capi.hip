@@ -42,7 +42,7 @@ __global__ void k_xsweep_p(const Prob<T> *, const Tile *, int, int);
 template <typename T, int M, int VR, int PAT> __global__ void k_vsweep_p(const Prob<T> *, const Tile *, int, int, int);
 template <typename T, int M, int PAT>
 __global__ void k_adj_v(const Prob<T> *, const Adj<T> *, const Tile *, int, int, double *);
-template <typename T, int M, int W, int PAT, int STAGE>
+template <typename T, int M, int W, int PAT, int STAGE, int NC>
 __global__ void k_adj_x(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
 template <typename T, int WHAT, int PAT>
 __global__ void k_adj_g(const Prob<T> *, const Adj<T> *, const Tile *, int, double *);
@@ -1165,9 +1165,8 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         lam_jobs_dev.~dev_buffer();
         new (&lam_jobs_dev) dev_buffer();
         lam_jobs_dev.alloc(2 * n * sizeof(CopyJob));
-        // parameter slots; S-lines of 16 chunks per point need the wider register budget of 8-line groups
-        // S-lines per adjoint S-sweep workgroup: 8 for 16-point chunks (register budget), else 16
-        // (measured faster than the forward's tuned width on the SABR book)
+        // parameter slots.  S-lines per adjoint S-sweep workgroup: 8 for S-lines of more than 512 points
+        // (128 chunks per line), else 16 (measured faster than the forward's tuned width on the SABR book)
         axw = pmx >= 16 ? 8 : 16;
         slot_tb[1] = slot_tb[3] = axw == 8 ? TB_X8 : TB_AX;
         nslots = 0;
@@ -1207,20 +1206,26 @@ template <typename T> struct PlanImpl final : lss_adi_plan {
         else
             adj_v_p<M, PAT_GENERIC>(s, last, table, slot, st, grid);
     }
-    template <int M, int W, int PAT>
+    template <int M, int W, int PAT, int NC = P_NCH>
     void adj_x_w(int s, int stage, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
     {
         const Adj<T> *ad = (const Adj<T> *)adj_dev.p;
-        const Tile *tx = (const Tile *)tiles_dev.p + tile_off[W == 8 ? TB_X8 : TB_AX];
+        const Tile *tx = (const Tile *)tiles_dev.p + tile_off[W == 8 ? TB_X8 : (W == 4 ? TB_X4 : TB_AX)];
         if (stage == 1)
-            hipLaunchKernelGGL((k_adj_x<T, M, W, PAT, 1>), dim3(grid), dim3(W * P_NCH), 0, st, table, ad, tx, s, slot);
+            hipLaunchKernelGGL((k_adj_x<T, M, W, PAT, 1, NC>), dim3(grid), dim3(W * NC), 0, st, table, ad, tx, s, slot);
         else
-            hipLaunchKernelGGL((k_adj_x<T, M, W, PAT, 2>), dim3(grid), dim3(W * P_NCH), 0, st, table, ad, tx, s, slot);
+            hipLaunchKernelGGL((k_adj_x<T, M, W, PAT, 2, NC>), dim3(grid), dim3(W * NC), 0, st, table, ad, tx, s, slot);
     }
     template <int M, int PAT>
     void adj_x_p(int s, int stage, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
     {
-        adj_x_w<M, (M >= 16 ? 8 : 16), PAT>(s, stage, table, slot, st, grid);  // = axw (M = pmx)
+        // S-lines of more than 512 points (M = pmx = 16): 128 chunks of at most 8 points and 8 lines per
+        // workgroup, so that a single large problem has twice the wavefronts (1024 x 512 MCS: backward
+        // 142 -> 136 ms); shorter lines: 64 chunks and 16 lines per workgroup (= axw)
+        if constexpr (M >= 16)
+            adj_x_w<8, 8, PAT, 128>(s, stage, table, slot, st, grid);
+        else
+            adj_x_w<M, 16, PAT>(s, stage, table, slot, st, grid);
     }
     template <int M> void adj_x(int s, int stage, const Prob<T> *table, double *slot, hipStream_t st, unsigned grid)
     {
